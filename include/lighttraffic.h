@@ -1,0 +1,202 @@
+/*
+ * lighttraffic.h — C ABI of the MI355X random-walk engine (liblighttraffic.so).
+ *
+ * The reference exposes its engine as compile-time C++ templates
+ * (LightTraffic<AppManager, App, Walker>, src/LightTraffic.h:11-45) driven by
+ * three CUDA programs (app/pagerank.cu, app/ppr.cu, app/genericwalk.cu) and a
+ * flag parser (src/Config.h:38-89).  This header replaces that plug-in surface
+ * with opaque handles, plain pointers and sizes: the three walk kinds are a
+ * closed set selected by the entry point, and every result is written into a
+ * caller-owned host buffer.  No torch or HIP types appear in any signature.
+ *
+ * Every entry point returns an lt_status; the library never calls exit().
+ * Handles are not thread-safe; two engines may coexist in one process.
+ */
+#ifndef LIGHTTRAFFIC_H
+#define LIGHTTRAFFIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Replace the reference's assert()/exit()/throw paths:
+ * gpuCall -> exit(code) (src/utils/GPUCall.h:3-11), assert(fp) on a missing
+ * .part file (src/partition/PartitionStrategy.h:80), runtime_error for a vertex
+ * larger than a partition (src/partition/PartitionStrategy.h:201-205). */
+typedef enum {
+    LT_OK = 0,
+    LT_E_IO = 1,               /* file missing, short read or write            */
+    LT_E_FORMAT = 2,           /* bcsr or .part content is inconsistent        */
+    LT_E_OVERSIZE_VERTEX = 3,  /* one vertex does not fit the partition size   */
+    LT_E_NOMEM = 4,            /* device or pinned host allocation failed      */
+    LT_E_HIP = 5,              /* HIP runtime error (message in lt_last_error) */
+    LT_E_ARG = 6,              /* invalid argument                             */
+    LT_E_NODEVICE = 7,         /* no HIP device visible                        */
+    LT_E_INTERNAL = 8          /* engine invariant violated (bug)              */
+} lt_status;
+
+typedef struct lt_graph lt_graph;         /* bcsr CSR in pinned host memory  */
+typedef struct lt_partition lt_partition; /* contiguous vertex-range split   */
+typedef struct lt_engine lt_engine;       /* GPU engine bound to one graph   */
+
+/* Partition scheduler policies.  GREEDY and FIFO restate the reference's two
+ * compile-time variants of Scheduler::nextPartition (src/Scheduler.h:368-388
+ * and :393-416, selected by -DFIFO_SCHE); LEARNED is new: an MLP scores the
+ * partitions and the engine falls back to GREEDY when it has no valid score. */
+typedef enum { LT_SCHED_GREEDY = 0, LT_SCHED_FIFO = 1, LT_SCHED_LEARNED = 2 } lt_sched_policy;
+
+/* How far one residency of a walker reaches.  PARTITION walks while the
+ * current vertex lies in the partition being processed, as the reference's
+ * randomWalkKernel loop does (src/WalkManager.h:239-240).  RESIDENT keeps
+ * walking while the vertex lies in any partition held in the HBM cache. */
+typedef enum { LT_SPAN_PARTITION = 0, LT_SPAN_RESIDENT = 1 } lt_span;
+
+typedef struct {
+    uint32_t num_device_partitions; /* -p: HBM cache slots (0 = all partitions)   */
+    uint64_t num_device_walkers;    /* -b: walkers resident at once (0 = all)     */
+    uint64_t zerocopy_threshold;    /* walkers below which a non-resident
+                                       partition is read over PCIe; UINT64_MAX =
+                                       reference default partition_bytes/256    */
+    uint64_t seed;                  /* Philox key                                 */
+    int32_t  policy;                /* lt_sched_policy                            */
+    int32_t  pipeline;              /* 1: compute other partitions while loading  */
+    int32_t  span;                  /* lt_span                                    */
+    int32_t  event_timers;          /* 1: per-phase HIP event timers              */
+    int32_t  record_trace;          /* 1: keep the scheduler decision trace       */
+    int32_t  reserved[7];
+} lt_config;
+
+/* Statistics of the last run call.  The first eight fields mirror the lines
+ * that Scheduler::timing prints (src/Scheduler.h:440-453). */
+typedef struct {
+    double   running_ms;            /* "Running Time" (walk phase only)          */
+    uint64_t iterations;            /* scheduler iterations                      */
+    uint64_t explicit_iterations;   /* iterations that used an HBM-resident copy */
+    double   graph_loading_ms;  uint64_t graph_loading_calls;
+    double   computing_ms;      uint64_t computing_calls;
+    double   zerocopy_ms;       uint64_t zerocopy_calls;
+    double   walker_loading_ms; uint64_t walker_loading_calls;
+    double   walker_evicting_ms; uint64_t walker_evicting_calls;
+    uint64_t h2d_bytes_explicit;    /* partition bytes copied host->HBM          */
+    uint64_t h2d_bytes_zerocopy;    /* estimated bytes read over PCIe in place   */
+    uint64_t moves;                 /* neighbour moves (rowptr pair + col read)  */
+    uint64_t teleports;             /* pagerank jumps to a uniform vertex        */
+    uint64_t visits;                /* visit-counter increments                  */
+    uint64_t residencies;           /* walker records read from a queue          */
+    uint64_t zerocopy_moves;        /* moves whose graph reads crossed PCIe      */
+    uint64_t nominal_steps;         /* w*l*runs (pagerank, genericwalk), w/prob  */
+    uint64_t kernel_launches;       /* walk-kernel launches                      */
+    double   walk_kernel_ms;        /* summed walk-kernel time (event timers)    */
+    uint64_t learned_decisions;     /* decisions taken from the MLP score        */
+    uint64_t fallback_decisions;    /* decisions that fell back to greedy        */
+    uint64_t reserved[6];
+} lt_stats;
+
+/* ---- errors ------------------------------------------------------------- */
+const char *lt_status_str(lt_status s);
+const char *lt_last_error(void);            /* detail of the last failure      */
+void lt_config_default(lt_config *cfg);
+
+/* ---- graph (src/GraphLoader.h:30-48 DiskGraph, :127-158 CPUGraph) -------- */
+/* bcsr: u32 num_node, u64 num_edge, u64 rowptr[V+1], u32 col[E]. */
+lt_status lt_graph_open(const char *path, lt_graph **out);
+lt_status lt_graph_from_csr(uint32_t num_node, uint64_t num_edge, const uint64_t *rowptr,
+                            const uint32_t *col, lt_graph **out);
+/* R-MAT generator on the GPU: ef*2^scale edge samples with quadrant
+ * probabilities a,b,c (d = 1-a-b-c) drawn from Philox(seed), then self-loops
+ * dropped, isolated vertices removed, edges symmetrised, sorted and
+ * deduplicated — the pipeline of tools/convert_from_txt.cc:66-218. */
+lt_status lt_graph_generate_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                                 uint64_t seed, lt_graph **out);
+lt_status lt_graph_save(const lt_graph *g, const char *path);
+uint32_t lt_graph_num_node(const lt_graph *g);
+uint64_t lt_graph_num_edge(const lt_graph *g);
+const uint64_t *lt_graph_rowptr(const lt_graph *g);
+const uint32_t *lt_graph_col(const lt_graph *g);
+void lt_graph_free(lt_graph *g);
+
+/* ---- partitions (src/partition/PartitionStrategy.h:65-338) -------------- */
+/* Greedy contiguous split, (nodes+1)*8 + edges*4 <= max_bytes per partition
+ * (EqualSizeStrategy::init, PartitionStrategy.h:189-235). */
+lt_status lt_partition_build(const lt_graph *g, uint64_t max_bytes, lt_partition **out);
+/* .part: u16 P, u64 partition_bytes, u32 node_offset[P+1], u64 edge_offset[P+1]
+ * written to <graph_path>.part (materialize, PartitionStrategy.h:325-337) and
+ * read back (MaterializedStrategy, PartitionStrategy.h:76-97). */
+lt_status lt_partition_save(const lt_partition *p, const char *graph_path);
+lt_status lt_partition_load(const char *graph_path, lt_partition **out);
+lt_status lt_partition_from_offsets(uint32_t num_partition, uint64_t partition_bytes,
+                                    const uint32_t *node_offset, const uint64_t *edge_offset,
+                                    lt_partition **out);
+uint32_t lt_partition_count(const lt_partition *p);
+uint64_t lt_partition_bytes(const lt_partition *p);
+const uint32_t *lt_partition_node_offsets(const lt_partition *p);
+const uint64_t *lt_partition_edge_offsets(const lt_partition *p);
+/* Host findPartition (PartitionStrategy.h:151-168): partition holding v. */
+uint32_t lt_partition_find(const lt_partition *p, uint32_t v);
+void lt_partition_free(lt_partition *p);
+
+/* ---- engine (src/LightTraffic.h:19-44, src/Device.h:22-59) -------------- */
+lt_status lt_engine_create(const lt_graph *g, const lt_partition *p, const lt_config *cfg,
+                           lt_engine **out);
+void lt_engine_destroy(lt_engine *e);
+
+/* pagerank (app/pagerank.cu:26-44): w walkers from uniform starts, l steps,
+ * teleport with probability 0.15, one visit counted per step.  counts_out
+ * (u32[V], may be NULL) receives the summed visit counts of all runs;
+ * endpoints_out (u32[w*runs], may be NULL) each walker's last vertex. */
+lt_status lt_run_pagerank(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, uint32_t *counts_out,
+                          uint32_t *endpoints_out);
+/* ppr (app/ppr.cu:22-42, src/walk/SingleSource.h:6-20): w walkers from
+ * source, length 2+Geom(prob) drawn up front, endpoint counted. endpoints_out
+ * (u32[w], may be NULL) receives each walker's final vertex. */
+lt_status lt_run_ppr(lt_engine *e, uint64_t w, uint32_t source, float prob, uint32_t *counts_out,
+                     uint32_t *endpoints_out);
+/* genericwalk (app/genericwalk.cu:22-32): fixed-length uniform walks that stop
+ * early at a vertex without neighbours.  The reference records nothing; this
+ * entry point can return each walker's final vertex (u32[w*runs]) and the
+ * number of moves it made (u32[w*runs]). */
+lt_status lt_run_genericwalk(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs,
+                             uint32_t *endpoints_out, uint32_t *moves_out);
+/* Batched PPR (no counterpart in the reference): nq queries, query q starts
+ * counts[q] walkers at sources[q]; walker ids are numbered query by query.
+ * endpoints_out: u32[sum counts] in walker-id order (may be NULL). */
+lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, const uint64_t *counts,
+                           float prob, uint32_t *endpoints_out);
+lt_status lt_engine_stats(const lt_engine *e, lt_stats *out);
+
+/* Scheduler decision trace of the last run (record_trace=1): one record per
+ * scheduler iteration.  kind: 0 explicit (resident or loaded), 1 zero-copy,
+ * 2 pipelined compute during a load.  loaded/evicted are -1 when none. */
+typedef struct {
+    int32_t partition;
+    int32_t kind;
+    int32_t loaded;
+    int32_t evicted;
+    uint64_t walkers;
+} lt_trace_rec;
+uint64_t lt_engine_trace_len(const lt_engine *e);
+lt_status lt_engine_trace(const lt_engine *e, lt_trace_rec *out, uint64_t cap);
+
+/* ---- pure scheduler rules (host, no GPU) -------------------------------- */
+/* Scheduler::nextPartition greedy (src/Scheduler.h:368-388): argmax of
+ * counts over p != prev unless sync; returns -1 when nothing is picked. */
+int32_t lt_sched_greedy_next(uint32_t P, const uint64_t *counts, int32_t prev, int32_t sync);
+/* GPUScheduler::findPartitionwithleastWalker (src/Scheduler.h:149-171):
+ * resident partition != except with the fewest walkers (lowest id on ties). */
+int32_t lt_sched_evict_min(uint32_t P, const uint64_t *counts, const uint8_t *resident, int32_t except);
+/* GPUScheduler::getComputedPartition (src/Scheduler.h:84-111), greedy variant:
+ * a resident partition != except whose queue holds >= page walkers and has the
+ * fewest walkers, else the resident one with the most (>0) walkers. */
+int32_t lt_sched_computed_partition(uint32_t P, const uint64_t *counts, const uint8_t *resident,
+                                    int32_t except, uint64_t page);
+
+/* ---- device probe -------------------------------------------------------- */
+lt_status lt_device_info(char *name, size_t cap, int *num_cu, uint64_t *hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIGHTTRAFFIC_H */

@@ -1,0 +1,339 @@
+"""Python bindings of liblighttraffic.so, the MI355X random-walk engine.
+
+The engine itself is C++/HIP behind the C ABI declared in
+include/lighttraffic.h; this module only loads that library with ctypes and
+wraps its handles so tests and the benchmark can drive it.  It mirrors the
+operator surface of the reference programs (app/pagerank.cu, app/ppr.cu,
+app/genericwalk.cu and the flags of src/Config.h:38-89): a graph in bcsr
+form, a materialised partitioning, and one run call per walk kind.
+
+Importing never falls back to another implementation: if the shared library
+has not been built, the import raises.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "liblighttraffic.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                      "or `make -C lighttraffic_amd/csrc`")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+LT_OK = 0
+STATUS = {0: "LT_OK", 1: "LT_E_IO", 2: "LT_E_FORMAT", 3: "LT_E_OVERSIZE_VERTEX", 4: "LT_E_NOMEM", 5: "LT_E_HIP",
+          6: "LT_E_ARG", 7: "LT_E_NODEVICE", 8: "LT_E_INTERNAL"}
+SCHED = {"greedy": 0, "fifo": 1, "learned": 2}
+SPAN = {"partition": 0, "resident": 1}
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i32 = ctypes.c_int32
+
+
+class LTError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        detail = _lib.lt_last_error().decode(errors="replace")
+        super().__init__(f"{where}: {STATUS.get(status, status)} ({detail})")
+
+
+class Config(ctypes.Structure):
+    """lt_config (include/lighttraffic.h)."""
+    _fields_ = [("num_device_partitions", _u32), ("num_device_walkers", _u64), ("zerocopy_threshold", _u64),
+                ("seed", _u64), ("policy", _i32), ("pipeline", _i32), ("span", _i32), ("event_timers", _i32),
+                ("record_trace", _i32), ("reserved", _i32 * 7)]
+
+
+class Stats(ctypes.Structure):
+    """lt_stats (include/lighttraffic.h)."""
+    _fields_ = [("running_ms", ctypes.c_double), ("iterations", _u64), ("explicit_iterations", _u64),
+                ("graph_loading_ms", ctypes.c_double), ("graph_loading_calls", _u64),
+                ("computing_ms", ctypes.c_double), ("computing_calls", _u64),
+                ("zerocopy_ms", ctypes.c_double), ("zerocopy_calls", _u64),
+                ("walker_loading_ms", ctypes.c_double), ("walker_loading_calls", _u64),
+                ("walker_evicting_ms", ctypes.c_double), ("walker_evicting_calls", _u64),
+                ("h2d_bytes_explicit", _u64), ("h2d_bytes_zerocopy", _u64), ("moves", _u64),
+                ("teleports", _u64), ("visits", _u64), ("residencies", _u64), ("zerocopy_moves", _u64),
+                ("nominal_steps", _u64), ("kernel_launches", _u64), ("walk_kernel_ms", ctypes.c_double),
+                ("learned_decisions", _u64), ("fallback_decisions", _u64), ("reserved", _u64 * 6)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class TraceRec(ctypes.Structure):
+    _fields_ = [("partition", _i32), ("kind", _i32), ("loaded", _i32), ("evicted", _i32), ("walkers", _u64)]
+
+
+def _sig(name, restype, *argtypes):
+    f = getattr(_lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+    return f
+
+
+_sig("lt_status_str", ctypes.c_char_p, ctypes.c_int)
+_sig("lt_last_error", ctypes.c_char_p)
+_sig("lt_config_default", None, ctypes.POINTER(Config))
+_sig("lt_graph_open", ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(_vp))
+_sig("lt_graph_from_csr", ctypes.c_int, _u32, _u64, _vp, _vp, ctypes.POINTER(_vp))
+_sig("lt_graph_generate_rmat", ctypes.c_int, _u32, _u32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64,
+     ctypes.POINTER(_vp))
+_sig("lt_graph_save", ctypes.c_int, _vp, ctypes.c_char_p)
+_sig("lt_graph_num_node", _u32, _vp)
+_sig("lt_graph_num_edge", _u64, _vp)
+_sig("lt_graph_rowptr", _vp, _vp)
+_sig("lt_graph_col", _vp, _vp)
+_sig("lt_graph_free", None, _vp)
+_sig("lt_partition_build", ctypes.c_int, _vp, _u64, ctypes.POINTER(_vp))
+_sig("lt_partition_save", ctypes.c_int, _vp, ctypes.c_char_p)
+_sig("lt_partition_load", ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(_vp))
+_sig("lt_partition_from_offsets", ctypes.c_int, _u32, _u64, _vp, _vp, ctypes.POINTER(_vp))
+_sig("lt_partition_count", _u32, _vp)
+_sig("lt_partition_bytes", _u64, _vp)
+_sig("lt_partition_node_offsets", _vp, _vp)
+_sig("lt_partition_edge_offsets", _vp, _vp)
+_sig("lt_partition_find", _u32, _vp, _u32)
+_sig("lt_partition_free", None, _vp)
+_sig("lt_engine_create", ctypes.c_int, _vp, _vp, ctypes.POINTER(Config), ctypes.POINTER(_vp))
+_sig("lt_engine_destroy", None, _vp)
+_sig("lt_run_pagerank", ctypes.c_int, _vp, _u64, _u32, _u32, _vp, _vp)
+_sig("lt_run_ppr", ctypes.c_int, _vp, _u64, _u32, ctypes.c_float, _vp, _vp)
+_sig("lt_run_genericwalk", ctypes.c_int, _vp, _u64, _u32, _u32, _vp, _vp)
+_sig("lt_run_ppr_batch", ctypes.c_int, _vp, _u32, _vp, _vp, ctypes.c_float, _vp)
+_sig("lt_engine_stats", ctypes.c_int, _vp, ctypes.POINTER(Stats))
+_sig("lt_engine_trace_len", _u64, _vp)
+_sig("lt_engine_trace", ctypes.c_int, _vp, ctypes.POINTER(TraceRec), _u64)
+_sig("lt_sched_greedy_next", _i32, _u32, _vp, _i32, _i32)
+_sig("lt_sched_evict_min", _i32, _u32, _vp, _vp, _i32)
+_sig("lt_sched_computed_partition", _i32, _u32, _vp, _vp, _i32, _u64)
+_sig("lt_device_info", ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
+     ctypes.POINTER(_u64))
+
+
+def lib():
+    return _lib
+
+
+def _check(status, where):
+    if status != LT_OK:
+        raise LTError(status, where)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def default_config(**kw):
+    c = Config()
+    _lib.lt_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        if k == "policy" and isinstance(v, str):
+            v = SCHED[v]
+        if k == "span" and isinstance(v, str):
+            v = SPAN[v]
+        setattr(c, k, v)
+    return c
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    ncu = ctypes.c_int()
+    hbm = _u64()
+    _check(_lib.lt_device_info(name, 256, ctypes.byref(ncu), ctypes.byref(hbm)), "lt_device_info")
+    return {"name": name.value.decode(), "num_cu": ncu.value, "hbm_bytes": hbm.value}
+
+
+class Graph:
+    """A bcsr graph (u32 vertex ids, u64 edge offsets) owned by the library."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def open(cls, path):
+        h = _vp()
+        _check(_lib.lt_graph_open(path.encode(), ctypes.byref(h)), f"lt_graph_open({path})")
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, rowptr, col):
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.uint64)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        h = _vp()
+        _check(_lib.lt_graph_from_csr(len(rowptr) - 1, len(col), rowptr.ctypes.data, col.ctypes.data,
+                                      ctypes.byref(h)), "lt_graph_from_csr")
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=None):
+        if seed is None:
+            seed = 0x5EED0000 + scale
+        h = _vp()
+        _check(_lib.lt_graph_generate_rmat(scale, edge_factor, a, b, c, seed, ctypes.byref(h)),
+               "lt_graph_generate_rmat")
+        return cls(h)
+
+    def save(self, path):
+        _check(_lib.lt_graph_save(self._h, path.encode()), f"lt_graph_save({path})")
+
+    @property
+    def num_node(self):
+        return _lib.lt_graph_num_node(self._h)
+
+    @property
+    def num_edge(self):
+        return _lib.lt_graph_num_edge(self._h)
+
+    def rowptr(self):
+        """Zero-copy numpy view of rowptr (valid while the graph lives)."""
+        n = self.num_node + 1
+        return np.ctypeslib.as_array(ctypes.cast(_lib.lt_graph_rowptr(self._h), ctypes.POINTER(_u64)), (n,))
+
+    def col(self):
+        n = max(self.num_edge, 1)
+        return np.ctypeslib.as_array(ctypes.cast(_lib.lt_graph_col(self._h), ctypes.POINTER(_u32)), (n,))[
+            :self.num_edge]
+
+    def close(self):
+        if self._h:
+            _lib.lt_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Partition:
+    """Contiguous vertex-range partitions (the .part file)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def build(cls, graph, max_bytes):
+        h = _vp()
+        _check(_lib.lt_partition_build(graph._h, max_bytes, ctypes.byref(h)), "lt_partition_build")
+        return cls(h)
+
+    @classmethod
+    def load(cls, graph_path):
+        h = _vp()
+        _check(_lib.lt_partition_load(graph_path.encode(), ctypes.byref(h)), "lt_partition_load")
+        return cls(h)
+
+    def save(self, graph_path):
+        _check(_lib.lt_partition_save(self._h, graph_path.encode()), "lt_partition_save")
+
+    @property
+    def count(self):
+        return _lib.lt_partition_count(self._h)
+
+    @property
+    def bytes(self):
+        return _lib.lt_partition_bytes(self._h)
+
+    def node_offsets(self):
+        n = self.count + 1
+        return np.ctypeslib.as_array(ctypes.cast(_lib.lt_partition_node_offsets(self._h), ctypes.POINTER(_u32)),
+                                     (n,)).copy()
+
+    def edge_offsets(self):
+        n = self.count + 1
+        return np.ctypeslib.as_array(ctypes.cast(_lib.lt_partition_edge_offsets(self._h), ctypes.POINTER(_u64)),
+                                     (n,)).copy()
+
+    def find(self, v):
+        return _lib.lt_partition_find(self._h, v)
+
+    def close(self):
+        if self._h:
+            _lib.lt_partition_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Engine:
+    """GPU engine bound to one graph and partitioning (lt_engine)."""
+
+    def __init__(self, graph, partition, config=None, **kw):
+        cfg = config if config is not None else default_config(**kw)
+        h = _vp()
+        _check(_lib.lt_engine_create(graph._h, partition._h, ctypes.byref(cfg), ctypes.byref(h)),
+               "lt_engine_create")
+        self._h = h
+        self.graph = graph
+        self.partition = partition
+        self.config = cfg
+
+    def pagerank(self, w, l=10, runs=1, endpoints=False):
+        counts = np.zeros(self.graph.num_node, np.uint32)
+        ep = np.zeros(w * runs, np.uint32) if endpoints else None
+        _check(_lib.lt_run_pagerank(self._h, w, l, runs, counts.ctypes.data, _ptr(ep)), "lt_run_pagerank")
+        return (counts, ep) if endpoints else counts
+
+    def ppr(self, w, source=0, prob=0.15, endpoints=False):
+        counts = np.zeros(self.graph.num_node, np.uint32)
+        ep = np.zeros(w, np.uint32) if endpoints else None
+        _check(_lib.lt_run_ppr(self._h, w, source, prob, counts.ctypes.data, _ptr(ep)), "lt_run_ppr")
+        return (counts, ep) if endpoints else counts
+
+    def genericwalk(self, w, l=10, runs=1, endpoints=True, moves=False):
+        ep = np.zeros(w * runs, np.uint32) if endpoints else None
+        mv = np.zeros(w * runs, np.uint32) if moves else None
+        _check(_lib.lt_run_genericwalk(self._h, w, l, runs, _ptr(ep), _ptr(mv)), "lt_run_genericwalk")
+        return ep, mv
+
+    def ppr_batch(self, sources, counts, prob=0.15, endpoints=True):
+        sources = np.ascontiguousarray(sources, dtype=np.uint32)
+        counts = np.ascontiguousarray(counts, dtype=np.uint64)
+        ep = np.zeros(int(counts.sum()), np.uint32) if endpoints else None
+        _check(_lib.lt_run_ppr_batch(self._h, len(sources), sources.ctypes.data, counts.ctypes.data, prob,
+                                     _ptr(ep)), "lt_run_ppr_batch")
+        return ep
+
+    def stats(self):
+        s = Stats()
+        _check(_lib.lt_engine_stats(self._h, ctypes.byref(s)), "lt_engine_stats")
+        return s.as_dict()
+
+    def trace(self):
+        n = _lib.lt_engine_trace_len(self._h)
+        buf = (TraceRec * max(n, 1))()
+        _check(_lib.lt_engine_trace(self._h, buf, n), "lt_engine_trace")
+        return [(r.partition, r.kind, r.loaded, r.evicted, r.walkers) for r in buf[:n]]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lt_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def sched_greedy_next(counts, prev, sync):
+    c = np.ascontiguousarray(counts, dtype=np.uint64)
+    return _lib.lt_sched_greedy_next(len(c), c.ctypes.data, prev, int(sync))
+
+
+def sched_evict_min(counts, resident, except_):
+    c = np.ascontiguousarray(counts, dtype=np.uint64)
+    r = np.ascontiguousarray(resident, dtype=np.uint8)
+    return _lib.lt_sched_evict_min(len(c), c.ctypes.data, r.ctypes.data, except_)
+
+
+def sched_computed_partition(counts, resident, except_, page):
+    c = np.ascontiguousarray(counts, dtype=np.uint64)
+    r = np.ascontiguousarray(resident, dtype=np.uint8)
+    return _lib.lt_sched_computed_partition(len(c), c.ctypes.data, r.ctypes.data, except_, page)

@@ -1,0 +1,188 @@
+// lt_cli.cpp — the pagerank / ppr / genericwalk programs on top of
+// liblighttraffic.so.  Built three times with LT_APP_NAME set; stdout lines
+// keep the reference formats (src/Config.h:88, src/LightTraffic.h:26-30,
+// src/Scheduler.h:440-453, app/pagerank.cu:119-135, app/ppr.cu:96-112) so
+// the reference's benchmark greps (benchmark/figure9/run.sh:40,
+// benchmark/figure13/iter_analysis.sh:15-17) read them unchanged.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "lighttraffic.h"
+#include "lt_cli.h"
+
+#ifndef LT_APP_NAME
+#define LT_APP_NAME "pagerank"
+#endif
+
+static int fail(lt_status s, const char *what) {
+    fprintf(stderr, "%s: %s (%s)\n", what, lt_status_str(s), lt_last_error());
+    return 1;
+}
+
+int main(int argc, char **argv) {
+    const std::string app = LT_APP_NAME;
+    CliConfig c;
+    if (!cli_parse(argc, argv, c)) {
+        fprintf(stderr,
+                "usage: %s -f <graph.bcsr> [-m [-s MB]] [-p slots] [-b walkers] [-w walkers] [-l length]\n"
+                "          [--runs n] [--source v] [--prob p] [--scheduler greedy|fifo|learned]\n"
+                "          [--no-pipeline] [--zerocopy-threshold n] [--span partition|resident]\n"
+                "          [--seed n] [--timers] [--json] [--queries file]\n",
+                app.c_str());
+        return 1;
+    }
+    lt_graph *g = nullptr;
+    lt_status s = lt_graph_open(c.file.c_str(), &g);
+    if (s != LT_OK) return fail(s, "open graph");
+    lt_partition *part = nullptr;
+    if (c.materialize) {
+        // Config.h:62-66 + EqualSizeStrategy::init's report (PartitionStrategy.h:225-234)
+        s = lt_partition_build(g, 1024ull * 1024ull * c.partition_mb, &part);
+        if (s != LT_OK) return fail(s, "partition");
+        s = lt_partition_save(part, c.file.c_str());
+        if (s != LT_OK) return fail(s, "write .part");
+        const uint64_t *rp = lt_graph_rowptr(g);
+        uint64_t maxdeg = 0;
+        uint32_t maxv = 0;
+        for (uint32_t v = 0; v < lt_graph_num_node(g); v++)
+            if (rp[v + 1] - rp[v] > maxdeg) {
+                maxdeg = rp[v + 1] - rp[v];
+                maxv = v;
+            }
+        printf("node with max degree: %u, degree: %lu\n", maxv, (unsigned long)maxdeg);
+    } else {
+        s = lt_partition_load(c.file.c_str(), &part);
+        if (s != LT_OK) return fail(s, "read .part");
+    }
+    const uint32_t V = lt_graph_num_node(g);
+    const uint32_t P = lt_partition_count(part);
+    const uint64_t w = c.has_w ? c.w : 2ull * V;
+    const uint64_t pdev = c.has_p ? std::min<uint64_t>(c.p, P) : P;
+    const uint64_t wdev = c.has_b ? std::min<uint64_t>(c.b, w) : w;
+
+    // batched ppr queries: one "source count" pair per line
+    std::vector<uint32_t> qsrc;
+    std::vector<uint64_t> qcnt;
+    uint64_t total = w;
+    if (!c.queries.empty()) {
+        FILE *qf = fopen(c.queries.c_str(), "r");
+        if (!qf) {
+            fprintf(stderr, "cannot open %s\n", c.queries.c_str());
+            return 1;
+        }
+        unsigned long long a, b;
+        while (fscanf(qf, "%llu %llu", &a, &b) == 2) {
+            qsrc.push_back((uint32_t)a);
+            qcnt.push_back(b);
+        }
+        fclose(qf);
+        total = 0;
+        for (auto x : qcnt) total += x;
+    }
+    printf("num nodes: %u, num edges: %lu, num walkers: %lu\n", V, (unsigned long)lt_graph_num_edge(g),
+           (unsigned long)total);
+
+    lt_config cfg;
+    lt_config_default(&cfg);
+    cfg.num_device_partitions = (uint32_t)pdev;
+    cfg.num_device_walkers = wdev;
+    cfg.seed = c.seed;
+    cfg.policy = c.scheduler == "fifo" ? LT_SCHED_FIFO : c.scheduler == "learned" ? LT_SCHED_LEARNED : LT_SCHED_GREEDY;
+    cfg.pipeline = c.pipeline ? 1 : 0;
+    cfg.span = c.span == "resident" ? LT_SPAN_RESIDENT : LT_SPAN_PARTITION;
+    cfg.event_timers = c.timers ? 1 : 0;
+    if (c.has_zc) cfg.zerocopy_threshold = c.zc;
+    lt_engine *e = nullptr;
+    s = lt_engine_create(g, part, &cfg, &e);
+    if (s == LT_E_ARG && P > 3500) {
+        // the reference's LDS budget check (Config.h:77-80)
+        printf("GPU shared memory overflow! Please consider reduce the number of partitions or batch size.\n");
+        return 1;
+    }
+    if (s != LT_OK) return fail(s, "engine");
+    printf("number of partitons: %u, graph memory pool: %lu MB, walker memory pool: %lu MB\n", P,
+           (unsigned long)(lt_partition_bytes(part) * pdev / 1024 / 1024),
+           (unsigned long)(std::min(wdev, total) * 16 / 1024 / 1024));
+
+    std::vector<uint32_t> counts(V, 0);
+    if (app == "pagerank") {
+        s = lt_run_pagerank(e, w, c.length, (uint32_t)c.runs, counts.data(), nullptr);
+    } else if (app == "ppr") {
+        if (!qsrc.empty()) {
+            std::vector<uint32_t> ends(total);
+            s = lt_run_ppr_batch(e, (uint32_t)qsrc.size(), qsrc.data(), qcnt.data(), c.prob, ends.data());
+            for (auto v : ends) counts[v]++;
+        } else {
+            s = lt_run_ppr(e, w, c.source, c.prob, counts.data(), nullptr);
+        }
+    } else {
+        s = lt_run_genericwalk(e, w, c.length, (uint32_t)c.runs, nullptr, nullptr);
+    }
+    if (s != LT_OK) return fail(s, "run");
+
+    lt_stats st;
+    lt_engine_stats(e, &st);
+    printf("Running Time: %f ms\n", st.running_ms);
+    printf("iterations: %lu, explicit: %lu\n", (unsigned long)st.iterations, (unsigned long)st.explicit_iterations);
+    if (!c.timers) printf("Warning: GPU timer is disabled, the following statistics is wrong!\n");
+    printf("graph loading time: %f ms, calls: %lu\n", st.graph_loading_ms, (unsigned long)st.graph_loading_calls);
+    printf("computing time: %f ms, calls: %lu\n", st.computing_ms, (unsigned long)st.computing_calls);
+    printf("zero copy time: %f ms, calls: %lu\n", st.zerocopy_ms, (unsigned long)st.zerocopy_calls);
+    printf("walker loading time: %f ms, calls: %lu\n", st.walker_loading_ms, (unsigned long)st.walker_loading_calls);
+    printf("walker evicting time: %f ms, calls: %lu\n", st.walker_evicting_ms,
+           (unsigned long)st.walker_evicting_calls);
+    printf("H2D bytes: explicit %llu, zero-copy %llu\n", (unsigned long long)st.h2d_bytes_explicit,
+           (unsigned long long)st.h2d_bytes_zerocopy);
+    printf("steps: nominal %llu, moves %llu, teleports %llu\n", (unsigned long long)st.nominal_steps,
+           (unsigned long long)st.moves, (unsigned long long)st.teleports);
+    printf("throughput: %f steps/s\n", st.running_ms > 0 ? st.nominal_steps / (st.running_ms * 1e-3) : 0.0);
+
+    const uint64_t *rp = lt_graph_rowptr(g);
+    int rc = 0;
+    if (app == "pagerank") {
+        // result(i) and check() of app/pagerank.cu:89-101
+        auto result = [&](uint32_t i) {
+            return (float)((counts[i] / ((double)w * c.runs / (V + 0.0))) / (c.length + 0.0));
+        };
+        for (uint32_t i = 0, printed = 0; i < V && printed < 20; i++) {
+            const uint64_t deg = rp[i + 1] - rp[i];
+            if (deg > 1000) {
+                printf("node %u, degree %lu, value %f\n", i, (unsigned long)deg, result(i));
+                printed++;
+            }
+        }
+        printf("node 0, degree %lu, value %f\n", (unsigned long)rp[1], result(0));
+        size_t sum = 0;
+        for (auto x : counts) sum += x;
+        const size_t expected = (size_t)w * c.length * c.runs;
+        printf("sum: %zu, %zu\n", sum, expected);
+        printf(sum == expected ? "result correct\n" : "result incorrect\n");
+        if (sum != expected)
+            printf("[Warning] ill-conditioned graph such as Yahoo has a very high-degree vertex whose value could "
+                   "exceed the max value of uint32.\n");
+    } else if (app == "ppr") {
+        const uint32_t target = 2;
+        if (target < V) printf("PageRank value of node %u: %f\n", target, (float)(counts[target] / (total / (V + 0.0))));
+        size_t sum = 0;
+        for (auto x : counts) sum += x;
+        printf("sum: %lu, %lu\n", (unsigned long)sum, (unsigned long)total);
+        printf(sum == total ? "result correct\n" : "result incorrect\n");
+    }
+    if (c.json) {
+        printf("{\"app\": \"%s\", \"running_ms\": %.3f, \"nominal_steps\": %llu, \"moves\": %llu, "
+               "\"h2d_bytes_explicit\": %llu, \"h2d_bytes_zerocopy\": %llu, \"iterations\": %llu, "
+               "\"explicit\": %llu, \"graph_loads\": %llu}\n",
+               app.c_str(), st.running_ms, (unsigned long long)st.nominal_steps, (unsigned long long)st.moves,
+               (unsigned long long)st.h2d_bytes_explicit, (unsigned long long)st.h2d_bytes_zerocopy,
+               (unsigned long long)st.iterations, (unsigned long long)st.explicit_iterations,
+               (unsigned long long)st.graph_loading_calls);
+    }
+    lt_engine_destroy(e);
+    lt_partition_free(part);
+    lt_graph_free(g);
+    return rc;
+}

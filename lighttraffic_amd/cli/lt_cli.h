@@ -1,0 +1,102 @@
+// lt_cli.h — the flag surface of the reference programs (src/Config.h:38-89).
+//
+// Reference flags, same names and defaults:
+//   -f/--file <bcsr>            graph (required)
+//   -p/--partition <n>          HBM cache slots            (default: all partitions)
+//   -b/--batch <n>              walkers resident at once   (default: all walkers)
+//   -w/--walker <n>             walkers                    (default: 2 * num nodes)
+//   -s/--partition-size <MB>    partition size with -m     (default: 32)
+//   -l/--length <n>             walk length                (default: 10)
+//   --runs <n>                  epochs                     (default: 1)
+//   --trimmed <n>               parsed and unused, as in the reference
+//   --source <v>                ppr source                 (default: 0)
+//   --prob <p>                  ppr stop probability       (default: 0.15)
+//   -m/--materialized-partition build <file>.part instead of reading it
+// Flags added by this engine (they only add behaviour):
+//   --scheduler greedy|fifo|learned, --no-pipeline, --zerocopy-threshold <n>,
+//   --span partition|resident, --seed <n>, --timers, --json, --queries <file>
+#pragma once
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <sstream>
+#include <string>
+
+struct CliConfig {
+    std::string file;
+    bool has_p = false, has_b = false, has_w = false;
+    uint64_t p = 0, b = 0, w = 0;
+    uint64_t partition_mb = 32;
+    uint32_t length = 10;
+    int runs = 1;
+    uint32_t trimmed = 0;
+    uint32_t source = 0;
+    float prob = 0.15f;
+    bool materialize = false;
+    std::string scheduler = "greedy";
+    bool pipeline = true;
+    bool has_zc = false;
+    uint64_t zc = 0;
+    std::string span = "partition";
+    uint64_t seed = 0x4C54524Bull;
+    bool timers = false;
+    bool json = false;
+    std::string queries;
+};
+
+template <typename T>
+static T cli_num(const char *s) {
+    T v{};
+    std::istringstream iss(s);
+    iss >> v;
+    return v;
+}
+
+// Accepts "-x v", "--long v" and "--long=v".
+static bool cli_parse(int argc, char **argv, CliConfig &c) {
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        std::string val;
+        bool has_eq = false;
+        const size_t eq = a.find('=');
+        if (a.rfind("--", 0) == 0 && eq != std::string::npos) {
+            val = a.substr(eq + 1);
+            a = a.substr(0, eq);
+            has_eq = true;
+        }
+        auto next = [&](const char *name) -> const char * {
+            if (has_eq) return strdup(val.c_str());
+            if (i + 1 >= argc) {
+                fprintf(stderr, "missing value for %s\n", name);
+                exit(1);
+            }
+            return argv[++i];
+        };
+        if (a == "-f" || a == "--file") c.file = next("file");
+        else if (a == "-p" || a == "--partition") { c.p = cli_num<uint64_t>(next("partition")); c.has_p = true; }
+        else if (a == "-b" || a == "--batch") { c.b = cli_num<uint64_t>(next("batch")); c.has_b = true; }
+        else if (a == "-w" || a == "--walker") { c.w = cli_num<uint64_t>(next("walker")); c.has_w = true; }
+        else if (a == "-s" || a == "--partition-size") c.partition_mb = cli_num<uint64_t>(next("partition-size"));
+        else if (a == "-l" || a == "--length") c.length = cli_num<uint32_t>(next("length"));
+        else if (a == "--runs") c.runs = cli_num<int>(next("runs"));
+        else if (a == "--trimmed") c.trimmed = cli_num<uint32_t>(next("trimmed"));
+        else if (a == "--source") c.source = cli_num<uint32_t>(next("source"));
+        else if (a == "--prob") c.prob = cli_num<float>(next("prob"));
+        else if (a == "-m" || a == "--materialized-partition") c.materialize = true;
+        else if (a == "--scheduler") c.scheduler = next("scheduler");
+        else if (a == "--no-pipeline") c.pipeline = false;
+        else if (a == "--zerocopy-threshold") { c.zc = cli_num<uint64_t>(next("zerocopy-threshold")); c.has_zc = true; }
+        else if (a == "--span") c.span = next("span");
+        else if (a == "--seed") c.seed = cli_num<uint64_t>(next("seed"));
+        else if (a == "--timers") c.timers = true;
+        else if (a == "--json") c.json = true;
+        else if (a == "--queries") c.queries = next("queries");
+        else {
+            fprintf(stderr, "unknown option %s\n", argv[i]);
+            return false;
+        }
+    }
+    return !c.file.empty();
+}

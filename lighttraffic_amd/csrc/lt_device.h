@@ -1,0 +1,103 @@
+// lt_device.h — device-side data layout and kernels of the walk engine.
+//
+// Walker queues (role of DeviceListPool, src/buffer/Pool.h:372-462): one slab
+// of fixed-size chunks of walker records, a per-partition table of chunk ids
+// (frame) and a per-partition walker count (qlen).  Chunks are handed out by a
+// bump pointer over a free stack while a kernel runs and returned by a
+// single-workgroup fix-up kernel between launches, so no host round trip is
+// needed to grow a queue (the reference pops pages with <<<1,1>>> kernels and
+// reads the page pointers back, Pool.h:616-640).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lt_rng.h"
+
+namespace lt {
+
+constexpr int kBlock = 256;                 // threads per workgroup (4 wave64)
+constexpr int kWpt = 4;                     // walkers per thread per pass
+constexpr int kBlockWalkers = kBlock * kWpt;  // 1024 walkers per workgroup pass
+constexpr int kChunkLog2 = 12;
+constexpr uint32_t kChunk = 1u << kChunkLog2;  // 4096 walkers = 64 KiB per chunk
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kMaxPartitions = 3400;   // append-stage LDS stays within 64 KiB
+constexpr uint32_t kSliceChunks = 2048;     // chunks of one source queue per launch
+constexpr uint32_t kLostCap = 8192;         // chunks lost to allocation races per launch
+
+// Walker record: one dwordx4.  The reference's walkers are {current, step}
+// (app/pagerank.cu:8-11) or {current, step, id} (app/genericwalk.cu:6-10);
+// the id is needed here because every random draw is keyed by it.
+struct __attribute__((aligned(16))) Walker {
+    uint32_t cur, step, id_lo, id_hi;
+};
+
+struct PoolCtl {
+    uint32_t top;       // free chunks on the stack at launch start
+    uint32_t popped;    // chunks taken during the launch
+    uint32_t nlost;     // chunks taken but not linked (lost a CAS)
+    uint32_t overflow;  // bit 0: pool exhausted, bit 1: lost list full, bit 2: frame index, bit 3: bad chunk id
+};
+
+struct Queues {
+    Walker *slab;
+    uint32_t *frame;       // [P][maxc] chunk ids, kEmpty when unused
+    uint32_t *qlen;        // [P] walkers queued per partition
+    uint32_t *free_stack;  // [nchunks]
+    PoolCtl *ctl;
+    uint32_t *lost;        // [kLostCap]
+    uint32_t maxc;         // chunk slots per partition
+    uint32_t nchunks;
+};
+
+struct Parts {
+    const uint32_t *node_off;  // [P+1]
+    const uint64_t *edge_off;  // [P+1]
+    const uint32_t *lut;       // partition of vertex (j << lut_shift)
+    const uint64_t *const *rowptr_tab;  // [P] HBM slot rowptr, nullptr if not resident
+    const uint32_t *const *col_tab;     // [P]
+    uint32_t lut_shift;
+    uint32_t P;
+};
+
+enum AppKind : int { APP_PAGERANK = 0, APP_PPR = 1, APP_GENERIC = 2 };
+
+struct AppArgs {
+    uint64_t seed;
+    uint32_t V;
+    uint32_t thresh;      // pagerank: teleport threshold; ppr: stop threshold
+    uint32_t length;      // pagerank/genericwalk: l
+    uint32_t source;      // single-source ppr
+    uint32_t *counts;     // u32[V] or nullptr
+    uint32_t *endpoints;  // u32[N] indexed by walker id, or nullptr
+    uint32_t *moves_out;  // u32[N] (genericwalk), or nullptr
+    const uint64_t *q_off;  // batched ppr: [nq+1] first walker id of each query
+    const uint32_t *q_src;  // batched ppr: [nq] sources
+    uint32_t nq;
+};
+
+enum StatSlot { ST_MOVES = 0, ST_TELEPORTS, ST_VISITS, ST_RESIDENCIES, ST_ZC_MOVES, ST_ZC_READS, ST_DROPPED, kNumStats = 8 };
+
+struct WalkArgs {
+    AppArgs app;
+    Queues q;
+    Parts pt;
+    uint32_t src;         // partition whose queue is consumed
+    uint32_t chunk_lo;    // first chunk of the slice
+    uint32_t chunk_hi;    // one past the last chunk of the slice
+    const uint64_t *rowptr;  // rowptr slice of src (HBM slot or mapped host)
+    const uint32_t *col;     // col slice of src
+    int zerocopy;         // src slice is read over PCIe
+    int span_resident;    // keep walking through HBM-resident partitions
+    unsigned long long *stats;
+};
+
+struct InitArgs {
+    AppArgs app;
+    Queues q;
+    Parts pt;
+    uint64_t id_lo, id_hi;  // walker ids created by this launch
+    unsigned long long *stats;
+};
+
+}  // namespace lt

@@ -1,0 +1,773 @@
+// lt_engine.hip — the GPU engine behind lt_engine_* / lt_run_*.
+//
+// Host side of the hot path: the HBM partition cache (role of GPUGraph,
+// src/GraphLoader.h:160-240), the walker queues (GPUListPool,
+// src/buffer/Pool.h:530-646), the per-partition processing step
+// (GPUWalkManager::processInMemory, src/WalkManager.h:362-404) and the plain
+// scheduler loop (GPUScheduler / Scheduler, src/Scheduler.h:13-454).
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "lt_internal.h"
+#include "lt_kernels.h"
+#include "lt_sched.h"
+
+#define LT_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t _e = (call);                                                                        \
+        if (_e != hipSuccess) {                                                                        \
+            lt::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(_e));            \
+            return LT_E_HIP;                                                                           \
+        }                                                                                              \
+    } while (0)
+
+#define LT_TRY(expr)                    \
+    do {                                \
+        lt_status _s = (expr);          \
+        if (_s != LT_OK) return _s;     \
+    } while (0)
+
+namespace lt {
+
+// Reference page size (numBlock * threadPerBlock * walkerPerThread,
+// src/Type.h:10-14): the "full queue" threshold of getComputedPartition.
+constexpr uint64_t kRefPage = 82ull * 512 * 4;
+
+template <typename T>
+static lt_status dmalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc of %zu bytes failed: %s", sizeof(T) * count, hipGetErrorString(e));
+        return LT_E_NOMEM;
+    }
+    return LT_OK;
+}
+
+template <typename T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// Accumulates HIP-event intervals on one stream (GPUTimer, src/utils/Stopwatch.h:49-104).
+struct EventTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> open;
+    std::vector<hipEvent_t> pool;
+    double ms = 0;
+    uint64_t calls = 0;
+    bool enabled = false;
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    hipEvent_t begin(hipStream_t s) {
+        calls++;
+        if (!enabled) return nullptr;
+        hipEvent_t a = take();
+        (void)hipEventRecord(a, s);
+        return a;
+    }
+    void end(hipEvent_t a, hipStream_t s) {
+        if (!enabled || !a) return;
+        hipEvent_t b = take();
+        (void)hipEventRecord(b, s);
+        open.emplace_back(a, b);
+    }
+    void collect() {
+        for (auto &pr : open) {
+            float t = 0;
+            if (hipEventSynchronize(pr.second) == hipSuccess && hipEventElapsedTime(&t, pr.first, pr.second) == hipSuccess)
+                ms += t;
+            pool.push_back(pr.first);
+            pool.push_back(pr.second);
+        }
+        open.clear();
+    }
+    void reset() {
+        collect();
+        ms = 0;
+        calls = 0;
+    }
+    ~EventTimer() {
+        collect();
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct RunSpec {
+    int app = APP_PAGERANK;
+    uint64_t total = 0;     // walker ids [0, total)
+    uint32_t length = 0;    // pagerank / genericwalk l
+    uint32_t source = 0;    // ppr
+    float prob = 0.15f;     // ppr
+    uint64_t nominal = 0;   // nominal steps for the stats
+    // batched ppr
+    const uint32_t *q_src = nullptr;
+    const uint64_t *q_cnt = nullptr;
+    uint32_t nq = 0;
+};
+
+struct Engine {
+    const lt_graph *g = nullptr;
+    lt_partition part;
+    lt_config cfg;
+    int device = 0;
+    uint32_t V = 0, P = 0;
+    hipStream_t s_compute = nullptr, s_copy = nullptr;
+    int num_cu = 0;
+    int max_grid = 0;
+
+    // partition table
+    uint32_t *d_node_off = nullptr;
+    uint64_t *d_edge_off = nullptr;
+    uint32_t *d_lut = nullptr;
+    uint32_t lut_shift = 0;
+    const uint64_t **d_rowptr_tab = nullptr;
+    const uint32_t **d_col_tab = nullptr;
+
+    // HBM partition cache
+    uint32_t nslots = 0;
+    uint8_t *d_slots = nullptr;
+    std::vector<int32_t> slot_of;      // partition -> slot or -1
+    std::vector<int32_t> part_in_slot; // slot -> partition or -1
+    std::vector<uint32_t> free_slots;
+    std::vector<uint8_t> resident;     // partition -> 0/1
+    uint64_t zc_threshold = 0;
+    const uint64_t *zc_rowptr = nullptr;  // device view of the pinned host graph
+    const uint32_t *zc_col = nullptr;
+
+    // walker queues
+    uint64_t wcap = 0;  // walkers the pool is sized for
+    Queues q{};
+    uint32_t *d_meta = nullptr;   // [P] qlen followed by PoolCtl
+    uint32_t *h_meta = nullptr;   // pinned mirror
+    std::vector<uint64_t> counts; // host view of qlen
+
+    // outputs and statistics
+    uint32_t *d_counts = nullptr;
+    size_t counts_cap = 0;
+    uint32_t *d_endpoints = nullptr;
+    size_t endpoints_cap = 0;
+    uint32_t *d_moves = nullptr;
+    size_t moves_cap = 0;
+    uint64_t *d_qoff = nullptr;
+    uint32_t *d_qsrc = nullptr;
+    size_t q_cap = 0;
+    unsigned long long *d_stats = nullptr;
+
+    lt_stats st{};
+    std::vector<lt_trace_rec> trace;
+    EventTimer t_load, t_compute, t_zc, t_wload, t_wevict, t_kernel;
+    Scheduler sched;
+
+    ~Engine();
+    lt_status init(const lt_graph *g_, const lt_partition *p_, const lt_config *c);
+    lt_status ensure_pool(uint64_t walkers);
+    lt_status ensure_outputs(const RunSpec &rs, bool want_endpoints, bool want_moves);
+    lt_status run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoints_out, uint32_t *moves_out);
+
+    // scheduler primitives
+    lt_status read_meta();
+    lt_status process(uint32_t p, bool zerocopy, uint64_t walkers_hint, int trace_kind);
+    lt_status evict(uint32_t p);
+    lt_status load_async(uint32_t p);
+    lt_status set_table(uint32_t p, bool present);
+    lt_status scheduler_loop();
+    lt_status run_partition(uint32_t p, int32_t &prev_loaded);
+
+    AppArgs app{};
+    int app_kind = APP_PAGERANK;
+    size_t lds_bytes() const { return append_lds_bytes(P); }
+    uint64_t partition_bytes(uint32_t p) const {
+        return ((uint64_t)part.node_off[p + 1] - part.node_off[p] + 1) * 8 +
+               (part.edge_off[p + 1] - part.edge_off[p]) * 4;
+    }
+};
+
+Engine::~Engine() {
+    if (s_compute) (void)hipStreamSynchronize(s_compute);
+    if (s_copy) (void)hipStreamSynchronize(s_copy);
+    dfree(d_node_off);
+    dfree(d_edge_off);
+    dfree(d_lut);
+    dfree(d_rowptr_tab);
+    dfree(d_col_tab);
+    dfree(d_slots);
+    dfree(q.slab);
+    dfree(q.frame);
+    dfree(q.free_stack);
+    dfree(q.lost);
+    dfree(d_meta);
+    if (h_meta) (void)hipHostFree(h_meta);
+    dfree(d_counts);
+    dfree(d_endpoints);
+    dfree(d_moves);
+    dfree(d_qoff);
+    dfree(d_qsrc);
+    dfree(d_stats);
+    if (s_compute) (void)hipStreamDestroy(s_compute);
+    if (s_copy) (void)hipStreamDestroy(s_copy);
+}
+
+lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_config *c) {
+    g = g_;
+    part = *p_;
+    cfg = *c;
+    V = g->V;
+    P = part.P();
+    if (part.node_off[P] != g->V || part.edge_off[P] != g->E) {
+        set_error("partition covers %u vertices / %llu edges, graph has %u / %llu", part.node_off[P],
+                  (unsigned long long)part.edge_off[P], g->V, (unsigned long long)g->E);
+        return LT_E_FORMAT;
+    }
+    for (uint32_t i = 0; i < P; i++) {
+        if (part.edge_off[i] != g->rowptr[part.node_off[i]]) {
+            set_error("partition %u edge offset does not match the graph", i);
+            return LT_E_FORMAT;
+        }
+        if (partition_bytes(i) > part.bytes) {
+            set_error("partition %u needs %llu bytes > partition size %llu", i,
+                      (unsigned long long)partition_bytes(i), (unsigned long long)part.bytes);
+            return LT_E_FORMAT;
+        }
+    }
+    if (P > kMaxPartitions) {
+        set_error("%u partitions exceed the engine limit %u; use a larger partition size", P, kMaxPartitions);
+        return LT_E_ARG;
+    }
+    if (V == 0) {
+        set_error("empty graph");
+        return LT_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device visible");
+        return LT_E_NODEVICE;
+    }
+    if (!g->pinned) {
+        set_error("graph was loaded without a HIP device; reopen it");
+        return LT_E_ARG;
+    }
+    LT_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    LT_HIP(hipGetDeviceProperties(&prop, device));
+    num_cu = prop.multiProcessorCount;
+    LT_HIP(hipStreamCreateWithFlags(&s_compute, hipStreamNonBlocking));
+    LT_HIP(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
+    int occ = 0;
+    LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, walk_kernel<APP_PAGERANK>, kBlock, lds_bytes()));
+    max_grid = std::max(1, occ) * num_cu;
+
+    // partition table + vertex -> partition lookup table
+    LT_TRY(dmalloc(&d_node_off, P + 1));
+    LT_TRY(dmalloc(&d_edge_off, P + 1));
+    LT_HIP(hipMemcpy(d_node_off, part.node_off.data(), 4 * (P + 1), hipMemcpyHostToDevice));
+    LT_HIP(hipMemcpy(d_edge_off, part.edge_off.data(), 8 * (P + 1), hipMemcpyHostToDevice));
+    uint32_t bits = 0;
+    while (bits < 32 && (1ull << bits) < (uint64_t)V) bits++;
+    lut_shift = bits > 16 ? bits - 16 : 0;
+    const size_t nlut = (((size_t)V - 1) >> lut_shift) + 1;
+    std::vector<uint32_t> lut(nlut);
+    for (size_t j = 0; j < nlut; j++) lut[j] = part.find((uint32_t)(j << lut_shift));
+    LT_TRY(dmalloc(&d_lut, nlut));
+    LT_HIP(hipMemcpy(d_lut, lut.data(), 4 * nlut, hipMemcpyHostToDevice));
+    LT_TRY(dmalloc(&d_rowptr_tab, P));
+    LT_TRY(dmalloc(&d_col_tab, P));
+    LT_HIP(hipMemset(d_rowptr_tab, 0, sizeof(void *) * P));
+    LT_HIP(hipMemset(d_col_tab, 0, sizeof(void *) * P));
+
+    // HBM cache slots (GPUGraph, src/GraphLoader.h:513-520)
+    nslots = cfg.num_device_partitions == 0 ? P : std::min(cfg.num_device_partitions, P);
+    const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
+    LT_TRY(dmalloc(&d_slots, (size_t)nslots * slot_bytes));
+    slot_of.assign(P, -1);
+    part_in_slot.assign(nslots, -1);
+    resident.assign(P, 0);
+    free_slots.clear();
+    for (uint32_t s = 0; s < nslots; s++) free_slots.push_back(nslots - 1 - s);
+    zc_threshold = cfg.zerocopy_threshold == UINT64_MAX ? part.bytes / 128 / 2 : cfg.zerocopy_threshold;
+    LT_HIP(hipHostGetDevicePointer((void **)&zc_rowptr, g->rowptr, 0));
+    LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
+
+    LT_TRY(dmalloc(&d_meta, P + 4));
+    LT_HIP(hipHostMalloc((void **)&h_meta, 4 * (P + 4), hipHostMallocDefault));
+    LT_TRY(dmalloc(&d_stats, kNumStats));
+    counts.assign(P, 0);
+
+    bool timers = cfg.event_timers != 0;
+    t_load.enabled = t_compute.enabled = t_zc.enabled = t_wload.enabled = t_wevict.enabled = timers;
+    t_kernel.enabled = timers;
+    sched.init(P, cfg);
+    return LT_OK;
+}
+
+// Size the chunk pool for `walkers` queued walkers: every queue may hold a
+// partly filled chunk, the slice being consumed keeps its chunks until the
+// fix-up, and allocation races may strand up to kLostCap chunks per launch.
+lt_status Engine::ensure_pool(uint64_t walkers) {
+    if (walkers <= wcap && q.slab) return LT_OK;
+    dfree(q.slab);
+    dfree(q.frame);
+    dfree(q.free_stack);
+    dfree(q.lost);
+    const uint64_t wchunks = (walkers + kChunk - 1) / kChunk;
+    const uint64_t slice = std::min<uint64_t>(kSliceChunks, wchunks + 1);
+    const uint64_t nchunks = wchunks + P + slice + 2 + kLostCap;
+    const uint64_t maxc = wchunks + 2;
+    if (nchunks >= 0xFFFFFFF0ull || maxc * kChunk > 0xFFFFFFFFull) {
+        set_error("%llu walkers exceed the queue limits; lower -b", (unsigned long long)walkers);
+        return LT_E_ARG;
+    }
+    q.nchunks = (uint32_t)nchunks;
+    q.maxc = (uint32_t)maxc;
+    LT_TRY(dmalloc(&q.slab, (size_t)nchunks * kChunk));
+    LT_TRY(dmalloc(&q.frame, (size_t)P * maxc));
+    LT_TRY(dmalloc(&q.free_stack, nchunks));
+    LT_TRY(dmalloc(&q.lost, kLostCap));
+    q.qlen = d_meta;
+    q.ctl = (PoolCtl *)(d_meta + P);
+    wcap = walkers;
+    return LT_OK;
+}
+
+lt_status Engine::ensure_outputs(const RunSpec &rs, bool want_endpoints, bool want_moves) {
+    if (counts_cap < V) {
+        dfree(d_counts);
+        LT_TRY(dmalloc(&d_counts, V));
+        counts_cap = V;
+    }
+    if (want_endpoints && endpoints_cap < rs.total) {
+        dfree(d_endpoints);
+        LT_TRY(dmalloc(&d_endpoints, rs.total));
+        endpoints_cap = rs.total;
+    }
+    if (want_moves && moves_cap < rs.total) {
+        dfree(d_moves);
+        LT_TRY(dmalloc(&d_moves, rs.total));
+        moves_cap = rs.total;
+    }
+    if (rs.nq && q_cap < rs.nq) {
+        dfree(d_qoff);
+        dfree(d_qsrc);
+        LT_TRY(dmalloc(&d_qoff, rs.nq + 1));
+        LT_TRY(dmalloc(&d_qsrc, rs.nq));
+        q_cap = rs.nq;
+    }
+    return LT_OK;
+}
+
+lt_status Engine::read_meta() {
+    LT_HIP(hipMemcpyAsync(h_meta, d_meta, 4 * (P + 4), hipMemcpyDeviceToHost, s_compute));
+    LT_HIP(hipStreamSynchronize(s_compute));
+    const PoolCtl *c = (const PoolCtl *)(h_meta + P);
+    if (c->overflow) {
+        set_error("walker queue pool overflow (flags 0x%x, top %u)", c->overflow, c->top);
+        return LT_E_INTERNAL;
+    }
+    for (uint32_t i = 0; i < P; i++) counts[i] = h_meta[i];
+    return LT_OK;
+}
+
+lt_status Engine::set_table(uint32_t p, bool present) {
+    const uint64_t *rp = nullptr;
+    const uint32_t *cp = nullptr;
+    if (present) {
+        const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
+        uint8_t *base = d_slots + (size_t)slot_of[p] * slot_bytes;
+        rp = (const uint64_t *)base;
+        cp = (const uint32_t *)(base + ((uint64_t)part.node_off[p + 1] - part.node_off[p] + 1) * 8);
+    }
+    set_table_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, p, rp, cp);
+    LT_HIP(hipGetLastError());
+    return LT_OK;
+}
+
+lt_status Engine::evict(uint32_t p) {
+    const int32_t s = slot_of[p];
+    if (s < 0) return LT_OK;
+    LT_TRY(set_table(p, false));
+    slot_of[p] = -1;
+    part_in_slot[s] = -1;
+    resident[p] = 0;
+    free_slots.push_back((uint32_t)s);
+    return LT_OK;
+}
+
+// GPUGraph::loadAsync (src/GraphLoader.h:531-552): rowptr slice at slot offset
+// 0, col slice right after it, both on the copy stream.  The compute stream
+// must not be reading the slot (the caller evicted it after a sync).
+lt_status Engine::load_async(uint32_t p) {
+    if (free_slots.empty()) {
+        set_error("no free cache slot for partition %u", p);
+        return LT_E_INTERNAL;
+    }
+    const uint32_t s = free_slots.back();
+    free_slots.pop_back();
+    const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
+    uint8_t *base = d_slots + (size_t)s * slot_bytes;
+    const uint32_t nb = part.node_off[p], ne = part.node_off[p + 1];
+    const uint64_t eb = part.edge_off[p], ee = part.edge_off[p + 1];
+    const size_t rbytes = ((size_t)ne - nb + 1) * 8, cbytes = (size_t)(ee - eb) * 4;
+    hipEvent_t t = t_load.begin(s_copy);
+    LT_HIP(hipMemcpyAsync(base, g->rowptr + nb, rbytes, hipMemcpyHostToDevice, s_copy));
+    if (cbytes) LT_HIP(hipMemcpyAsync(base + rbytes, g->col + eb, cbytes, hipMemcpyHostToDevice, s_copy));
+    t_load.end(t, s_copy);
+    st.h2d_bytes_explicit += rbytes + cbytes;
+    slot_of[p] = (int32_t)s;
+    part_in_slot[s] = (int32_t)p;
+    return LT_OK;
+}
+
+// Consume the whole queue of partition p in slices of kSliceChunks chunks:
+// walk kernel, then fix-up kernel per slice (processInMemory with all
+// batches, src/WalkManager.h:375-388).
+lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers, int trace_kind) {
+    if (walkers == 0) return LT_OK;
+    WalkArgs a{};
+    a.app = app;
+    a.q = q;
+    a.pt.node_off = d_node_off;
+    a.pt.edge_off = d_edge_off;
+    a.pt.lut = d_lut;
+    a.pt.rowptr_tab = d_rowptr_tab;
+    a.pt.col_tab = d_col_tab;
+    a.pt.lut_shift = lut_shift;
+    a.pt.P = P;
+    a.src = p;
+    a.zerocopy = zerocopy ? 1 : 0;
+    a.span_resident = cfg.span == LT_SPAN_RESIDENT ? 1 : 0;
+    a.stats = d_stats;
+    if (zerocopy) {
+        a.rowptr = zc_rowptr + part.node_off[p];
+        a.col = zc_col + part.edge_off[p];
+    } else {
+        const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
+        const uint8_t *base = d_slots + (size_t)slot_of[p] * slot_bytes;
+        a.rowptr = (const uint64_t *)base;
+        a.col = (const uint32_t *)(base + ((uint64_t)part.node_off[p + 1] - part.node_off[p] + 1) * 8);
+    }
+    EventTimer &phase = zerocopy ? t_zc : t_compute;
+    hipEvent_t tp = phase.begin(s_compute);
+    const uint64_t nch = (walkers + kChunk - 1) / kChunk;
+    for (uint64_t lo = 0; lo < nch; lo += kSliceChunks) {
+        const uint64_t hi = std::min<uint64_t>(nch, lo + kSliceChunks);
+        const bool last = hi == nch;
+        a.chunk_lo = (uint32_t)lo;
+        a.chunk_hi = last ? 0xFFFFFFFFu : (uint32_t)hi;
+        const uint64_t sw = std::min<uint64_t>(walkers, hi * kChunk) - lo * kChunk;
+        const int grid = (int)std::min<uint64_t>((sw + kBlockWalkers - 1) / kBlockWalkers, (uint64_t)max_grid);
+        hipEvent_t tk = t_kernel.begin(s_compute);
+        switch (app_kind) {
+            case APP_PAGERANK: walk_kernel<APP_PAGERANK><<<grid, kBlock, lds_bytes(), s_compute>>>(a); break;
+            case APP_PPR: walk_kernel<APP_PPR><<<grid, kBlock, lds_bytes(), s_compute>>>(a); break;
+            default: walk_kernel<APP_GENERIC><<<grid, kBlock, lds_bytes(), s_compute>>>(a); break;
+        }
+        LT_HIP(hipGetLastError());
+        t_kernel.end(tk, s_compute);
+        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, p, (uint32_t)lo, last ? 0xFFFFFFFFu : (uint32_t)hi, last ? 1 : 0);
+        LT_HIP(hipGetLastError());
+        st.kernel_launches++;
+    }
+    phase.end(tp, s_compute);
+    if (cfg.record_trace) trace.push_back(lt_trace_rec{(int32_t)p, trace_kind, -1, -1, walkers});
+    return LT_OK;
+}
+
+// GPUScheduler::run (src/Scheduler.h:311-330) without host walker spill.
+lt_status Engine::run_partition(uint32_t p, int32_t &prev_loaded) {
+    st.iterations++;
+    const uint64_t n = counts[p];
+    // shouldZerocopy (src/Scheduler.h:243-252): counts were read this iteration.
+    if (!resident[p] && n < zc_threshold) {
+        st.zerocopy_calls++;
+        return process(p, true, n, 1);
+    }
+    st.explicit_iterations++;
+    int32_t loaded = -1, evicted = -1;
+    // graphLoadingPhase (src/Scheduler.h:173-213)
+    if (!resident[p]) {
+        if (free_slots.empty()) {
+            const int32_t victim = sched.evict_choice(counts.data(), resident.data(), prev_loaded);
+            if (victim < 0) {
+                set_error("no partition to evict");
+                return LT_E_INTERNAL;
+            }
+            LT_TRY(evict((uint32_t)victim));
+            evicted = victim;
+            sched.on_evict((uint32_t)victim);
+        }
+        LT_TRY(load_async(p));
+        st.graph_loading_calls++;
+        loaded = (int32_t)p;
+        sched.on_load(p);
+        if (cfg.pipeline) {
+            // compute resident partitions while the copy is in flight
+            while (hipStreamQuery(s_copy) == hipErrorNotReady) {
+                LT_TRY(read_meta());
+                const int32_t c = sched.computed_choice(counts.data(), resident.data(), (int32_t)p, kRefPage);
+                if (c < 0) break;
+                st.computing_calls++;
+                LT_TRY(process((uint32_t)c, false, counts[c], 2));
+                LT_HIP(hipStreamSynchronize(s_compute));
+            }
+        }
+        LT_HIP(hipStreamSynchronize(s_copy));
+        resident[p] = 1;
+        LT_TRY(set_table(p, true));
+    }
+    prev_loaded = (int32_t)p;
+    if (cfg.record_trace) trace.push_back(lt_trace_rec{(int32_t)p, 3, loaded, evicted, n});
+    st.computing_calls++;
+    return process(p, false, n, 0);
+}
+
+// Scheduler::start (src/Scheduler.h:425-438).
+lt_status Engine::scheduler_loop() {
+    int32_t prev = -1, prev_loaded = -1;
+    bool sync = false;
+    sched.begin_run();
+    while (true) {
+        LT_TRY(read_meta());
+        const int32_t c = sched.next(counts.data(), resident.data(), prev, sync);
+        if (c == -2) return LT_E_INTERNAL;
+        if (c < 0) {
+            if (sync) break;
+            sync = true;
+            continue;
+        }
+        sync = false;
+        prev = c;
+        LT_TRY(run_partition((uint32_t)c, prev_loaded));
+    }
+    LT_HIP(hipDeviceSynchronize());
+    return LT_OK;
+}
+
+lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoints_out, uint32_t *moves_out) {
+    LT_HIP(hipSetDevice(device));
+    memset(&st, 0, sizeof(st));
+    trace.clear();
+    t_load.reset();
+    t_compute.reset();
+    t_zc.reset();
+    t_wload.reset();
+    t_wevict.reset();
+    t_kernel.reset();
+    const uint64_t wave = cfg.num_device_walkers ? std::min<uint64_t>(cfg.num_device_walkers, rs.total) : rs.total;
+    LT_TRY(ensure_pool(std::max<uint64_t>(wave, 1)));
+    LT_TRY(ensure_outputs(rs, endpoints_out != nullptr, moves_out != nullptr));
+
+    app_kind = rs.app;
+    app = AppArgs{};
+    app.seed = cfg.seed;
+    app.V = V;
+    app.length = rs.length;
+    app.source = rs.source;
+    app.counts = d_counts;
+    app.endpoints = endpoints_out ? d_endpoints : nullptr;
+    app.moves_out = moves_out ? d_moves : nullptr;
+    if (rs.app == APP_PAGERANK) app.thresh = prob_threshold(1.0 - (double)0.85f);
+    else if (rs.app == APP_PPR) app.thresh = prob_threshold((double)rs.prob);
+    if (rs.app == APP_PPR && app.thresh == 0) {
+        set_error("ppr stop probability %g rounds to zero", (double)rs.prob);
+        return LT_E_ARG;
+    }
+    if (rs.nq) {
+        std::vector<uint64_t> off(rs.nq + 1, 0);
+        for (uint32_t i = 0; i < rs.nq; i++) off[i + 1] = off[i] + rs.q_cnt[i];
+        LT_HIP(hipMemcpy(d_qoff, off.data(), 8 * (rs.nq + 1), hipMemcpyHostToDevice));
+        LT_HIP(hipMemcpy(d_qsrc, rs.q_src, 4 * rs.nq, hipMemcpyHostToDevice));
+        app.q_off = d_qoff;
+        app.q_src = d_qsrc;
+        app.nq = rs.nq;
+    }
+    LT_HIP(hipMemsetAsync(d_counts, 0, 4 * (size_t)V, s_compute));
+    LT_HIP(hipMemsetAsync(d_stats, 0, 8 * kNumStats, s_compute));
+    if (moves_out) LT_HIP(hipMemsetAsync(d_moves, 0, 4 * rs.total, s_compute));
+
+    double running_ms = 0;
+    for (uint64_t id0 = 0; id0 < rs.total; id0 += wave) {
+        const uint64_t id1 = std::min<uint64_t>(rs.total, id0 + wave);
+        // createWalkers: untimed, as in the reference (src/LightTraffic.h:33-44)
+        reset_queues_kernel<<<1024, 256, 0, s_compute>>>(q, P);
+        LT_HIP(hipGetLastError());
+        InitArgs ia{};
+        ia.app = app;
+        ia.q = q;
+        ia.pt.node_off = d_node_off;
+        ia.pt.edge_off = d_edge_off;
+        ia.pt.lut = d_lut;
+        ia.pt.rowptr_tab = d_rowptr_tab;
+        ia.pt.col_tab = d_col_tab;
+        ia.pt.lut_shift = lut_shift;
+        ia.pt.P = P;
+        ia.id_lo = id0;
+        ia.id_hi = id1;
+        ia.stats = d_stats;
+        const int grid = (int)std::min<uint64_t>((id1 - id0 + kBlockWalkers - 1) / kBlockWalkers, (uint64_t)max_grid);
+        switch (rs.app) {
+            case APP_PAGERANK: init_kernel<APP_PAGERANK><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
+            case APP_PPR: init_kernel<APP_PPR><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
+            default: init_kernel<APP_GENERIC><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
+        }
+        LT_HIP(hipGetLastError());
+        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0);
+        LT_HIP(hipGetLastError());
+        LT_HIP(hipStreamSynchronize(s_compute));
+        const auto t0 = std::chrono::steady_clock::now();
+        LT_TRY(scheduler_loop());
+        const auto t1 = std::chrono::steady_clock::now();
+        running_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    LT_TRY(read_meta());
+    for (uint32_t i = 0; i < P; i++)
+        if (counts[i]) {
+            set_error("partition %u still holds %llu walkers after the run", i, (unsigned long long)counts[i]);
+            return LT_E_INTERNAL;
+        }
+
+    // reduce (app/pagerank.cu:71-74): results back to the host, untimed
+    if (counts_out) LT_HIP(hipMemcpy(counts_out, d_counts, 4 * (size_t)V, hipMemcpyDeviceToHost));
+    if (endpoints_out) LT_HIP(hipMemcpy(endpoints_out, d_endpoints, 4 * rs.total, hipMemcpyDeviceToHost));
+    if (moves_out) LT_HIP(hipMemcpy(moves_out, d_moves, 4 * rs.total, hipMemcpyDeviceToHost));
+    unsigned long long hs[kNumStats];
+    LT_HIP(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
+    st.running_ms = running_ms;
+    st.moves = hs[ST_MOVES];
+    st.teleports = hs[ST_TELEPORTS];
+    st.visits = hs[ST_VISITS];
+    st.residencies = hs[ST_RESIDENCIES];
+    st.zerocopy_moves = hs[ST_ZC_MOVES];
+    st.h2d_bytes_zerocopy = hs[ST_ZC_READS] * 64ull;
+    st.nominal_steps = rs.nominal;
+    t_load.collect();
+    t_compute.collect();
+    t_zc.collect();
+    t_kernel.collect();
+    st.graph_loading_ms = t_load.ms;
+    st.computing_ms = t_compute.ms;
+    st.zerocopy_ms = t_zc.ms;
+    st.walker_loading_ms = 0;
+    st.walker_evicting_ms = 0;
+    st.walk_kernel_ms = t_kernel.ms;
+    st.learned_decisions = sched.learned_decisions;
+    st.fallback_decisions = sched.fallback_decisions;
+    return LT_OK;
+}
+
+}  // namespace lt
+
+struct lt_engine : lt::Engine {};
+
+using namespace lt;
+
+extern "C" {
+
+lt_status lt_engine_create(const lt_graph *g, const lt_partition *p, const lt_config *cfg, lt_engine **out) {
+    if (!g || !p || !out) return LT_E_ARG;
+    lt_config c;
+    if (cfg) c = *cfg;
+    else lt_config_default(&c);
+    std::unique_ptr<lt_engine> e(new lt_engine);
+    lt_status s = e->init(g, p, &c);
+    if (s != LT_OK) return s;
+    *out = e.release();
+    return LT_OK;
+}
+
+void lt_engine_destroy(lt_engine *e) { delete e; }
+
+lt_status lt_run_pagerank(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, uint32_t *counts_out,
+                          uint32_t *endpoints_out) {
+    if (!e || w == 0 || runs == 0) return LT_E_ARG;
+    RunSpec rs;
+    rs.app = APP_PAGERANK;
+    rs.total = w * runs;
+    rs.length = l;
+    rs.nominal = w * (uint64_t)l * runs;
+    return e->run(rs, counts_out, endpoints_out, nullptr);
+}
+
+lt_status lt_run_ppr(lt_engine *e, uint64_t w, uint32_t source, float prob, uint32_t *counts_out,
+                     uint32_t *endpoints_out) {
+    if (!e || w == 0 || !(prob > 0.0f)) return LT_E_ARG;
+    if (source >= e->V) {
+        set_error("source %u >= number of vertices %u", source, e->V);
+        return LT_E_ARG;
+    }
+    RunSpec rs;
+    rs.app = APP_PPR;
+    rs.total = w;
+    rs.source = source;
+    rs.prob = prob;
+    rs.nominal = (uint64_t)((double)w / (double)prob);
+    return e->run(rs, counts_out, endpoints_out, nullptr);
+}
+
+lt_status lt_run_genericwalk(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, uint32_t *endpoints_out,
+                             uint32_t *moves_out) {
+    if (!e || w == 0 || runs == 0) return LT_E_ARG;
+    RunSpec rs;
+    rs.app = APP_GENERIC;
+    rs.total = w * runs;
+    rs.length = l;
+    rs.nominal = w * (uint64_t)l * runs;
+    return e->run(rs, nullptr, endpoints_out, moves_out);
+}
+
+lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, const uint64_t *counts, float prob,
+                           uint32_t *endpoints_out) {
+    if (!e || nq == 0 || !sources || !counts || !(prob > 0.0f)) return LT_E_ARG;
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+        if (sources[i] >= e->V) {
+            set_error("query %u source %u >= number of vertices %u", i, sources[i], e->V);
+            return LT_E_ARG;
+        }
+        total += counts[i];
+    }
+    if (total == 0) return LT_E_ARG;
+    RunSpec rs;
+    rs.app = APP_PPR;
+    rs.total = total;
+    rs.prob = prob;
+    rs.q_src = sources;
+    rs.q_cnt = counts;
+    rs.nq = nq;
+    rs.nominal = (uint64_t)((double)total / (double)prob);
+    return e->run(rs, nullptr, endpoints_out, nullptr);
+}
+
+lt_status lt_engine_stats(const lt_engine *e, lt_stats *out) {
+    if (!e || !out) return LT_E_ARG;
+    *out = e->st;
+    return LT_OK;
+}
+
+uint64_t lt_engine_trace_len(const lt_engine *e) { return e ? e->trace.size() : 0; }
+
+lt_status lt_engine_trace(const lt_engine *e, lt_trace_rec *out, uint64_t cap) {
+    if (!e || !out) return LT_E_ARG;
+    const uint64_t n = std::min<uint64_t>(cap, e->trace.size());
+    if (n) memcpy(out, e->trace.data(), n * sizeof(lt_trace_rec));
+    return LT_OK;
+}
+
+}  // extern "C"

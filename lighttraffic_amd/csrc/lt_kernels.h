@@ -1,0 +1,486 @@
+// lt_kernels.h — walk, init and queue kernels (included by lt_engine.hip only).
+//
+// One walker per lane, kWpt walkers per thread.  A workgroup pass takes 1024
+// walker records from the source queue (coalesced dwordx4 loads), walks each
+// one while its vertex stays inside the span being processed, then appends
+// the walkers that left to their destination partitions' queues:
+//   1. LDS histogram of destination partitions (ds atomics, returns the rank
+//      of the walker inside its bucket),
+//   2. one global atomicAdd per (workgroup, destination) reserves a range of
+//      queue slots, and an exclusive scan of the histogram gives each bucket
+//      its place in an LDS staging buffer,
+//   3. walkers are staged in LDS grouped by destination and written out by
+//      consecutive lanes to consecutive queue slots.
+// This is the two-level reshuffle of insertToQueueKernel
+// (src/WalkManager.h:60-190) fused into the walk kernel
+// (randomWalkKernel, src/WalkManager.h:229-246), so a residency costs one
+// read and one write of the 16-byte record instead of two of each.
+#pragma once
+#include "lt_device.h"
+
+namespace lt {
+
+__device__ __forceinline__ uint32_t find_part(const Parts &pt, uint32_t v) {
+    uint32_t p = pt.lut[v >> pt.lut_shift];
+    while (v >= pt.node_off[p + 1]) ++p;
+    return p;
+}
+
+__device__ __forceinline__ uint32_t load_agent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Chunk k of partition part's queue: link a fresh chunk unless another lane
+// linked one first (then return ours to the lost list).  Returns kEmpty if the
+// pool is exhausted (flagged in ctl->overflow; the host turns it into an error).
+__device__ uint32_t alloc_chunk(const Queues &q, uint32_t part, uint32_t k) {
+    if (k >= q.maxc) {
+        atomicOr(&q.ctl->overflow, 4u);
+        return kEmpty;
+    }
+    uint32_t *slot = &q.frame[(size_t)part * q.maxc + k];
+    const uint32_t top = q.ctl->top;
+    const uint32_t i = atomicAdd(&q.ctl->popped, 1u);
+    if (i >= top) {
+        atomicOr(&q.ctl->overflow, 1u);
+        return load_agent(slot);
+    }
+    const uint32_t c = q.free_stack[top - 1 - i];
+    const uint32_t old = atomicCAS(slot, kEmpty, c);
+    if (old == kEmpty) return c;
+    const uint32_t j = atomicAdd(&q.ctl->nlost, 1u);
+    if (j < kLostCap) q.lost[j] = c;
+    else atomicOr(&q.ctl->overflow, 2u);
+    return old;
+}
+
+__device__ __forceinline__ uint32_t get_chunk(const Queues &q, uint32_t part, uint32_t k) {
+    if (k >= q.maxc) {
+        atomicOr(&q.ctl->overflow, 4u);
+        return kEmpty;
+    }
+    const uint32_t c = load_agent(&q.frame[(size_t)part * q.maxc + k]);
+    return c != kEmpty ? c : alloc_chunk(q, part, k);
+}
+
+// LDS layout of the append stage for P partitions.
+struct AppendLds {
+    uint32_t *hist;   // [P]
+    uint32_t *base;   // [P] first reserved queue slot
+    uint32_t *scan;   // [P] exclusive prefix of hist
+    Walker *stage;    // [kBlockWalkers]
+    uint32_t *slot;   // [kBlockWalkers]
+    uint32_t *dst;    // [kBlockWalkers]
+};
+
+__host__ __device__ inline size_t append_lds_bytes(uint32_t P) {
+    size_t b = 3 * (size_t)P * 4;
+    b = (b + 15) & ~(size_t)15;
+    return b + (size_t)kBlockWalkers * (16 + 4 + 4);
+}
+
+__device__ __forceinline__ AppendLds carve_lds(uint8_t *smem, uint32_t P) {
+    AppendLds L;
+    L.hist = (uint32_t *)smem;
+    L.base = L.hist + P;
+    L.scan = L.base + P;
+    size_t off = ((size_t)3 * P * 4 + 15) & ~(size_t)15;
+    L.stage = (Walker *)(smem + off);
+    L.slot = (uint32_t *)(L.stage + kBlockWalkers);
+    L.dst = L.slot + kBlockWalkers;
+    return L;
+}
+
+// Exclusive scan of in[0..n) into out[], returns the total (all threads).
+__device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *s_wsum) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t per = (n + kBlock - 1) / kBlock;
+    const uint32_t beg = min(tid * per, n), end = min(beg + per, n);
+    uint32_t sum = 0;
+    for (uint32_t i = beg; i < end; i++) sum += in[i];
+    uint32_t x = sum;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kBlock / 64; w++) {
+        const uint32_t s = s_wsum[w];
+        before += (w < wave) ? s : 0u;
+        total += s;
+    }
+    uint32_t run = before + x - sum;
+    for (uint32_t i = beg; i < end; i++) {
+        out[i] = run;
+        run += in[i];
+    }
+    return total;
+}
+
+// Append the walkers with dst[k] != kEmpty to their partitions' queues.
+// Collective: every thread of the workgroup must call it.
+__device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, const Walker (&w)[kWpt],
+                             const uint32_t (&dst)[kWpt], uint32_t *s_wsum) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < P; i += kBlock) L.hist[i] = 0;
+    __syncthreads();
+    uint32_t rank[kWpt];
+#pragma unroll
+    for (int k = 0; k < kWpt; k++)
+        if (dst[k] != kEmpty) rank[k] = atomicAdd(&L.hist[dst[k]], 1u);
+    __syncthreads();
+    const uint32_t total = block_exclusive_scan(L.hist, L.scan, P, s_wsum);
+    for (uint32_t i = tid; i < P; i += kBlock) {
+        const uint32_t c = L.hist[i];
+        if (c) {
+            const uint32_t b = atomicAdd(&q.qlen[i], c);
+            L.base[i] = b;
+            // The range that contains a chunk's first slot links that chunk.
+            const uint32_t first = (b + kChunk - 1) >> kChunkLog2;
+            if (((uint64_t)first << kChunkLog2) < (uint64_t)b + c) alloc_chunk(q, i, first);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kWpt; k++) {
+        if (dst[k] != kEmpty) {
+            const uint32_t pos = L.scan[dst[k]] + rank[k];
+            L.stage[pos] = w[k];
+            L.slot[pos] = L.base[dst[k]] + rank[k];
+            L.dst[pos] = dst[k];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < total; i += kBlock) {
+        const uint32_t part = L.dst[i], s = L.slot[i];
+        const uint32_t c = get_chunk(q, part, s >> kChunkLog2);
+        if (c < q.nchunks) q.slab[(size_t)c * kChunk + (s & (kChunk - 1))] = L.stage[i];
+        else atomicOr(&q.ctl->overflow, 8u);
+    }
+    __syncthreads();
+}
+
+struct Counters {
+    uint64_t moves = 0, tele = 0, visits = 0, resid = 0, zc_moves = 0, zc_reads = 0;
+};
+
+__device__ void flush_counters(const Counters &c, unsigned long long *stats) {
+    uint64_t v[6] = {c.moves, c.tele, c.visits, c.resid, c.zc_moves, c.zc_reads};
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        uint64_t x = v[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if ((threadIdx.x & 63u) == 0 && x) atomicAdd(&stats[i], (unsigned long long)x);
+    }
+}
+
+// The graph span a walker is currently walking in.
+struct Span {
+    const uint64_t *rowptr;  // rowptr[v - nb] = global offset of v's first edge
+    const uint32_t *col;     // col[e - eb]
+    uint64_t eb;
+    uint32_t nb, ne;
+    int zc;
+};
+
+__device__ __forceinline__ uint64_t walker_id(const Walker &w) { return ((uint64_t)w.id_hi << 32) | w.id_lo; }
+
+// One update of app/pagerank.cu:26-44: count the visit, then teleport with
+// probability 0.15 or move to a uniform neighbour.  A vertex without
+// neighbours teleports (the reference divides by zero there).
+__device__ __forceinline__ void step_pagerank(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
+    atomicAdd(&a.counts[w.cur], 1u);
+    ct.visits++;
+    w.step -= 1;
+    if (w.step == 0) return;
+    const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_PAGERANK);
+    bool tele = x.x < a.thresh;
+    if (!tele) {
+        const uint64_t *r = sp.rowptr + (w.cur - sp.nb);
+        const uint64_t b = r[0], d = r[1] - b;
+        if (d == 0) {
+            tele = true;
+        } else {
+            w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
+            ct.moves++;
+            if (sp.zc) { ct.zc_moves++; ct.zc_reads += 2; }
+        }
+    }
+    if (tele) {
+        w.cur = (uint32_t)bounded(x.y, x.z, a.V);
+        ct.tele++;
+    }
+}
+
+// One update of app/ppr.cu:22-42: the endpoint is counted when one step is
+// left; a vertex without neighbours is counted and ends the walk.
+__device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
+    if (w.step == 1) {
+        if (a.counts) atomicAdd(&a.counts[w.cur], 1u);
+        ct.visits++;
+    }
+    w.step -= 1;
+    if (w.step == 0) return;
+    const uint64_t *r = sp.rowptr + (w.cur - sp.nb);
+    const uint64_t b = r[0], d = r[1] - b;
+    if (d > 0) {
+        const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_PPR);
+        w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
+        ct.moves++;
+        if (sp.zc) { ct.zc_moves++; ct.zc_reads += 2; }
+    } else {
+        if (a.counts) atomicAdd(&a.counts[w.cur], 1u);
+        ct.visits++;
+        w.step = 0;
+        if (sp.zc) ct.zc_reads++;
+    }
+}
+
+// One update of app/genericwalk.cu:22-32: a uniform neighbour move, or stop
+// at a vertex without neighbours.  Records the number of moves made.
+__device__ __forceinline__ void step_generic(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
+    const uint64_t *r = sp.rowptr + (w.cur - sp.nb);
+    const uint64_t b = r[0], d = r[1] - b;
+    if (d > 0) {
+        const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_GENERIC);
+        w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
+        ct.moves++;
+        if (sp.zc) { ct.zc_moves++; ct.zc_reads += 2; }
+        w.step -= 1;
+        if (w.step == 0 && a.moves_out) a.moves_out[walker_id(w)] = a.length;
+    } else {
+        if (a.moves_out) a.moves_out[walker_id(w)] = a.length - w.step;
+        w.step = 0;
+        if (sp.zc) ct.zc_reads++;
+    }
+}
+
+template <int APP>
+__device__ __forceinline__ void app_step(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
+    if (APP == APP_PAGERANK) step_pagerank(w, sp, a, ct);
+    else if (APP == APP_PPR) step_ppr(w, sp, a, ct);
+    else step_generic(w, sp, a, ct);
+}
+
+__device__ __forceinline__ void finish_walker(const Walker &w, const AppArgs &a) {
+    if (a.endpoints) a.endpoints[walker_id(w)] = w.cur;
+}
+
+// Walk one residency; returns the destination partition or kEmpty when the
+// walk is over.  With span_resident the walker continues into any partition
+// whose slot is resident (and into the source slice itself).
+template <int APP>
+__device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, const Span &src_sp, Counters &ct) {
+    Span sp = src_sp;
+    while (true) {
+        while (w.step > 0 && (w.cur - sp.nb) < (sp.ne - sp.nb)) app_step<APP>(w, sp, a.app, ct);
+        if (w.step == 0) {
+            finish_walker(w, a.app);
+            return kEmpty;
+        }
+        const uint32_t p = find_part(a.pt, w.cur);
+        if (!a.span_resident) return p;
+        if (p == a.src) {
+            sp = src_sp;
+            continue;
+        }
+        const uint64_t *rp = a.pt.rowptr_tab[p];
+        if (!rp) return p;
+        sp.rowptr = rp;
+        sp.col = a.pt.col_tab[p];
+        sp.nb = a.pt.node_off[p];
+        sp.ne = a.pt.node_off[p + 1];
+        sp.eb = a.pt.edge_off[p];
+        sp.zc = 0;
+    }
+}
+
+template <int APP>
+__global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    const AppendLds L = carve_lds(smem, a.pt.P);
+    const uint32_t n = a.q.qlen[a.src];
+    const uint64_t lo64 = (uint64_t)a.chunk_lo << kChunkLog2;
+    const uint64_t hi64 = min((uint64_t)n, (uint64_t)a.chunk_hi << kChunkLog2);
+    Span src_sp;
+    src_sp.rowptr = a.rowptr;
+    src_sp.col = a.col;
+    src_sp.nb = a.pt.node_off[a.src];
+    src_sp.ne = a.pt.node_off[a.src + 1];
+    src_sp.eb = a.pt.edge_off[a.src];
+    src_sp.zc = a.zerocopy;
+    Counters ct;
+    for (uint64_t base = lo64 + (uint64_t)blockIdx.x * kBlockWalkers; base < hi64;
+         base += (uint64_t)gridDim.x * kBlockWalkers) {
+        const uint32_t chunk = a.q.frame[(size_t)a.src * a.q.maxc + (uint32_t)(base >> kChunkLog2)];
+        const bool chunk_ok = chunk < a.q.nchunks;
+        if (!chunk_ok && threadIdx.x == 0) atomicOr(&a.q.ctl->overflow, 8u);
+        const Walker *cw = a.q.slab + (size_t)(chunk_ok ? chunk : 0) * kChunk + (uint32_t)(base & (kChunk - 1));
+        Walker w[kWpt];
+        uint32_t dst[kWpt];
+#pragma unroll
+        for (int k = 0; k < kWpt; k++) {
+            const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+            dst[k] = kEmpty;
+            if (chunk_ok && i < hi64) {
+                w[k] = cw[k * kBlock + threadIdx.x];
+                ct.resid++;
+            } else {
+                w[k].step = 0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kWpt; k++) {
+            const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+            if (chunk_ok && i < hi64) dst[k] = walk_residency<APP>(w[k], a, src_sp, ct);
+        }
+        block_append(a.q, a.pt.P, L, w, dst, s_wsum);
+    }
+    flush_counters(ct, a.stats);
+}
+
+// PPR length (src/walk/SingleSource.h:12-17): 2 + number of draws that do not
+// stop before the first one that does.
+__device__ __forceinline__ uint32_t ppr_length(uint64_t seed, uint64_t id, uint32_t thresh) {
+    uint32_t F = 0;
+    for (uint32_t j = 0;; j++) {
+        const u32x4 x = draw(seed, id, j, TAG_PPRLEN);
+        if (x.x < thresh) return 2u + F;
+        if (x.y < thresh) return 3u + F;
+        if (x.z < thresh) return 4u + F;
+        if (x.w < thresh) return 5u + F;
+        F += 4;
+        if (F > 0xFFFFFF00u) return 2u + F;
+    }
+}
+
+// Create walkers [id_lo, id_hi) and append them to their start partitions'
+// queues (GPUInitWalker::create + insertPushedToQueue, src/walk/InitWalk.h:16-20,
+// src/WalkManager.h:341-350).
+template <int APP>
+__global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    const AppendLds L = carve_lds(smem, a.pt.P);
+    for (uint64_t base = a.id_lo + (uint64_t)blockIdx.x * kBlockWalkers; base < a.id_hi;
+         base += (uint64_t)gridDim.x * kBlockWalkers) {
+        Walker w[kWpt];
+        uint32_t dst[kWpt];
+#pragma unroll
+        for (int k = 0; k < kWpt; k++) {
+            const uint64_t id = base + (uint64_t)k * kBlock + threadIdx.x;
+            dst[k] = kEmpty;
+            w[k].id_lo = (uint32_t)id;
+            w[k].id_hi = (uint32_t)(id >> 32);
+            if (id >= a.id_hi) {
+                w[k].cur = 0;
+                w[k].step = 0;
+                continue;
+            }
+            if (APP == APP_PPR) {
+                uint32_t src = a.app.source;
+                if (a.app.q_off) {
+                    uint32_t lo = 0, hi = a.app.nq;  // last query with q_off[q] <= id
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (a.app.q_off[mid] <= id) lo = mid;
+                        else hi = mid;
+                    }
+                    src = a.app.q_src[lo];
+                }
+                w[k].cur = src;
+                w[k].step = ppr_length(a.app.seed, id, a.app.thresh);
+            } else {
+                const u32x4 x = draw(a.app.seed, id, 0, TAG_START);
+                w[k].cur = (uint32_t)bounded(x.x, x.y, a.app.V);
+                w[k].step = a.app.length;
+            }
+            if (w[k].step > 0) {
+                dst[k] = find_part(a.pt, w[k].cur);
+            } else {
+                if (APP == APP_GENERIC && a.app.moves_out) a.app.moves_out[id] = 0;
+                finish_walker(w[k], a.app);
+            }
+        }
+        block_append(a.q, a.pt.P, L, w, dst, s_wsum);
+    }
+}
+
+// Return the popped-but-unused state of the pool, the lost chunks and the
+// consumed source chunks [chunk_lo, chunk_hi) to the free stack; clear the
+// source queue after its last slice.  One workgroup.
+__global__ __launch_bounds__(kBlock) void fixup_kernel(Queues q, uint32_t src, uint32_t chunk_lo, uint32_t chunk_hi,
+                                                       int last) {
+    __shared__ uint32_t s_top, s_nlost, s_cnt;
+    if (threadIdx.x == 0) {
+        PoolCtl c = *q.ctl;
+        uint32_t popped = c.popped;
+        if (popped > c.top) popped = c.top;
+        uint32_t nlost = c.nlost < kLostCap ? c.nlost : kLostCap;
+        uint32_t cnt = 0;
+        if (src != kEmpty) {
+            const uint32_t nch = (q.qlen[src] + kChunk - 1) >> kChunkLog2;
+            const uint32_t hi = min(chunk_hi, nch);
+            cnt = hi > chunk_lo ? hi - chunk_lo : 0;
+        }
+        s_top = c.top - popped;
+        s_nlost = nlost;
+        s_cnt = cnt;
+        if (s_top + nlost + cnt > q.nchunks) {
+            q.ctl->overflow |= 16u;
+            s_nlost = 0;
+            s_cnt = 0;
+        }
+    }
+    __syncthreads();
+    const uint32_t top = s_top, nlost = s_nlost, cnt = s_cnt;
+    for (uint32_t i = threadIdx.x; i < nlost; i += kBlock) q.free_stack[top + i] = q.lost[i];
+    for (uint32_t j = threadIdx.x; j < cnt; j += kBlock) {
+        uint32_t *f = &q.frame[(size_t)src * q.maxc + chunk_lo + j];
+        q.free_stack[top + nlost + j] = *f;
+        *f = kEmpty;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        q.ctl->top = top + nlost + cnt;
+        q.ctl->popped = 0;
+        q.ctl->nlost = 0;
+        if (last && src != kEmpty) q.qlen[src] = 0;
+    }
+}
+
+// Empty every queue and put all chunks on the free stack.
+__global__ void reset_queues_kernel(Queues q, uint32_t P) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = tid; i < (size_t)P * q.maxc; i += stride) q.frame[i] = kEmpty;
+    for (size_t i = tid; i < q.nchunks; i += stride) q.free_stack[i] = (uint32_t)i;
+    for (size_t i = tid; i < P; i += stride) q.qlen[i] = 0;
+    if (tid == 0) {
+        q.ctl->top = q.nchunks;
+        q.ctl->popped = 0;
+        q.ctl->nlost = 0;
+        q.ctl->overflow = 0;
+    }
+}
+
+__global__ void set_table_kernel(const uint64_t **rowptr_tab, const uint32_t **col_tab, uint32_t p,
+                                 const uint64_t *rp, const uint32_t *cp) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        rowptr_tab[p] = rp;
+        col_tab[p] = cp;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = tid; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace lt
