@@ -98,7 +98,7 @@ int main(int argc, char **argv) {
     if (c.has_zc) cfg.zerocopy_threshold = c.zc;
     lt_engine *e = nullptr;
     s = lt_engine_create(g, part, &cfg, &e);
-    if (s == LT_E_ARG && P > 3500) {
+    if (s == LT_E_ARG && P > 2900) {
         // the reference's LDS budget check (Config.h:77-80)
         printf("GPU shared memory overflow! Please consider reduce the number of partitions or batch size.\n");
         return 1;

@@ -21,7 +21,7 @@ constexpr int kBlockWalkers = kBlock * kWpt;  // 1024 walkers per workgroup pass
 constexpr int kChunkLog2 = 12;
 constexpr uint32_t kChunk = 1u << kChunkLog2;  // 4096 walkers = 64 KiB per chunk
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr uint32_t kMaxPartitions = 3400;   // append-stage LDS stays within 64 KiB
+constexpr uint32_t kMaxPartitions = 2900;   // append-stage LDS stays within 64 KiB
 constexpr uint32_t kSliceChunks = 2048;     // chunks of one source queue per launch
 constexpr uint32_t kLostCap = 8192;         // chunks lost to allocation races per launch
 

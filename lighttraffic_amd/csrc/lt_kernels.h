@@ -54,40 +54,49 @@ __device__ uint32_t alloc_chunk(const Queues &q, uint32_t part, uint32_t k) {
     return old;
 }
 
+// Chunk k of a queue whose first slot another workgroup reserved.  That
+// workgroup took its slot range with an earlier atomicAdd, so it is running
+// and links the chunk within a few microseconds: wait for the link (bounded),
+// then link it ourselves if it still has not appeared.
 __device__ __forceinline__ uint32_t get_chunk(const Queues &q, uint32_t part, uint32_t k) {
     if (k >= q.maxc) {
         atomicOr(&q.ctl->overflow, 4u);
         return kEmpty;
     }
-    const uint32_t c = load_agent(&q.frame[(size_t)part * q.maxc + k]);
+    const uint32_t *slot = &q.frame[(size_t)part * q.maxc + k];
+    uint32_t c = load_agent(slot);
+    for (int it = 0; c == kEmpty && it < 4096; it++) {
+        __builtin_amdgcn_s_sleep(4);
+        c = load_agent(slot);
+    }
     return c != kEmpty ? c : alloc_chunk(q, part, k);
 }
 
 // LDS layout of the append stage for P partitions.
 struct AppendLds {
-    uint32_t *hist;   // [P]
-    uint32_t *base;   // [P] first reserved queue slot
-    uint32_t *scan;   // [P] exclusive prefix of hist
-    Walker *stage;    // [kBlockWalkers]
-    uint32_t *slot;   // [kBlockWalkers]
-    uint32_t *dst;    // [kBlockWalkers]
+    uint32_t *hist;   // [P] bucket sizes, then the first reserved queue slot
+    uint32_t *scan;   // [P+1] exclusive prefix of the bucket sizes
+    uint32_t *ch0;    // [P] chunk holding the first reserved slot
+    uint32_t *ch1;    // [P] chunk holding the last reserved slot
+    Walker *stage;    // [kBlockWalkers] walkers grouped by destination
+    uint16_t *dst;    // [kBlockWalkers] destination of each staged walker
 };
 
 __host__ __device__ inline size_t append_lds_bytes(uint32_t P) {
-    size_t b = 3 * (size_t)P * 4;
+    size_t b = (4 * (size_t)P + 1) * 4;
     b = (b + 15) & ~(size_t)15;
-    return b + (size_t)kBlockWalkers * (16 + 4 + 4);
+    return b + (size_t)kBlockWalkers * (16 + 2);
 }
 
 __device__ __forceinline__ AppendLds carve_lds(uint8_t *smem, uint32_t P) {
     AppendLds L;
     L.hist = (uint32_t *)smem;
-    L.base = L.hist + P;
-    L.scan = L.base + P;
-    size_t off = ((size_t)3 * P * 4 + 15) & ~(size_t)15;
+    L.scan = L.hist + P;
+    L.ch0 = L.scan + P + 1;
+    L.ch1 = L.ch0 + P;
+    size_t off = ((4 * (size_t)P + 1) * 4 + 15) & ~(size_t)15;
     L.stage = (Walker *)(smem + off);
-    L.slot = (uint32_t *)(L.stage + kBlockWalkers);
-    L.dst = L.slot + kBlockWalkers;
+    L.dst = (uint16_t *)(L.stage + kBlockWalkers);
     return L;
 }
 
@@ -122,7 +131,12 @@ __device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint
 }
 
 // Append the walkers with dst[k] != kEmpty to their partitions' queues.
-// Collective: every thread of the workgroup must call it.
+// Collective: every thread of the workgroup must call it.  Chunk ids are
+// resolved once per (workgroup, destination): a reservation of at most
+// kBlockWalkers <= kChunk slots touches at most two chunks; the range that
+// contains a chunk's first slot links it, and a later range that arrives
+// before that link is published links it itself (the CAS loser's chunk goes
+// to the lost list).
 __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, const Walker (&w)[kWpt],
                              const uint32_t (&dst)[kWpt], uint32_t *s_wsum) {
     const uint32_t tid = threadIdx.x;
@@ -134,14 +148,29 @@ __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, co
         if (dst[k] != kEmpty) rank[k] = atomicAdd(&L.hist[dst[k]], 1u);
     __syncthreads();
     const uint32_t total = block_exclusive_scan(L.hist, L.scan, P, s_wsum);
+    if (tid == 0) L.scan[P] = total;
+    __syncthreads();  // the scan reads hist segments that the loop below overwrites
+    // Pass 1: reserve and link the chunks this workgroup owns.  Pass 2 waits
+    // for chunks owned by others only after every owned link is published,
+    // so no owner ever waits before publishing.
     for (uint32_t i = tid; i < P; i += kBlock) {
         const uint32_t c = L.hist[i];
         if (c) {
             const uint32_t b = atomicAdd(&q.qlen[i], c);
-            L.base[i] = b;
-            // The range that contains a chunk's first slot links that chunk.
-            const uint32_t first = (b + kChunk - 1) >> kChunkLog2;
-            if (((uint64_t)first << kChunkLog2) < (uint64_t)b + c) alloc_chunk(q, i, first);
+            const uint32_t k0 = b >> kChunkLog2, k1 = (b + c - 1) >> kChunkLog2;
+            const uint32_t c0 = (b & (kChunk - 1)) == 0 ? alloc_chunk(q, i, k0) : kEmpty;
+            L.ch0[i] = c0;
+            L.ch1[i] = k1 != k0 ? alloc_chunk(q, i, k1) : c0;
+            L.hist[i] = b;
+        }
+    }
+    for (uint32_t i = tid; i < P; i += kBlock) {
+        const uint32_t b = L.hist[i];
+        if (L.scan[i + 1] != L.scan[i] && (b & (kChunk - 1)) != 0) {
+            const uint32_t c0 = get_chunk(q, i, b >> kChunkLog2);
+            const uint32_t k1 = (b + (L.scan[i + 1] - L.scan[i]) - 1) >> kChunkLog2;
+            L.ch0[i] = c0;
+            if (k1 == (b >> kChunkLog2)) L.ch1[i] = c0;
         }
     }
     __syncthreads();
@@ -150,14 +179,15 @@ __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, co
         if (dst[k] != kEmpty) {
             const uint32_t pos = L.scan[dst[k]] + rank[k];
             L.stage[pos] = w[k];
-            L.slot[pos] = L.base[dst[k]] + rank[k];
-            L.dst[pos] = dst[k];
+            L.dst[pos] = (uint16_t)dst[k];
         }
     }
     __syncthreads();
     for (uint32_t i = tid; i < total; i += kBlock) {
-        const uint32_t part = L.dst[i], s = L.slot[i];
-        const uint32_t c = get_chunk(q, part, s >> kChunkLog2);
+        const uint32_t part = L.dst[i];
+        const uint32_t b = L.hist[part];
+        const uint32_t s = b + (i - L.scan[part]);
+        const uint32_t c = (s >> kChunkLog2) == (b >> kChunkLog2) ? L.ch0[part] : L.ch1[part];
         if (c < q.nchunks) q.slab[(size_t)c * kChunk + (s & (kChunk - 1))] = L.stage[i];
         else atomicOr(&q.ctl->overflow, 8u);
     }
