@@ -41,6 +41,7 @@ typedef enum {
 typedef struct lt_graph lt_graph;         /* bcsr CSR in pinned host memory  */
 typedef struct lt_partition lt_partition; /* contiguous vertex-range split   */
 typedef struct lt_engine lt_engine;       /* GPU engine bound to one graph   */
+typedef struct lt_scorer lt_scorer;       /* learned partition scorer (MLP)  */
 
 /* Partition scheduler policies.  GREEDY and FIFO restate the reference's two
  * compile-time variants of Scheduler::nextPartition (src/Scheduler.h:368-388
@@ -90,9 +91,11 @@ typedef struct {
     uint64_t nominal_steps;         /* w*l*runs (pagerank, genericwalk), w/prob  */
     uint64_t kernel_launches;       /* walk-kernel launches                      */
     double   walk_kernel_ms;        /* summed walk-kernel time (event timers)    */
-    uint64_t learned_decisions;     /* decisions taken from the MLP score        */
-    uint64_t fallback_decisions;    /* decisions that fell back to greedy        */
-    uint64_t reserved[6];
+    uint64_t learned_decisions;     /* evictions chosen by the MLP keep score    */
+    uint64_t fallback_decisions;    /* evictions that fell back to the plain rule */
+    uint64_t prefetch_issued;       /* partitions copied ahead of their turn     */
+    uint64_t prefetch_hits;         /* prefetched partitions picked next         */
+    uint64_t reserved[4];
 } lt_stats;
 
 /* ---- errors ------------------------------------------------------------- */
@@ -168,8 +171,9 @@ lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, c
 lt_status lt_engine_stats(const lt_engine *e, lt_stats *out);
 
 /* Scheduler decision trace of the last run (record_trace=1): one record per
- * scheduler iteration.  kind: 0 explicit (resident or loaded), 1 zero-copy,
- * 2 pipelined compute during a load.  loaded/evicted are -1 when none. */
+ * decision.  kind: 0 partition already resident, 1 zero-copy, 2 pipelined
+ * compute during a load, 3 partition loaded (evicted = the slot's previous
+ * partition or -1).  walkers = queue length of the chosen partition. */
 typedef struct {
     int32_t partition;
     int32_t kind;
@@ -179,6 +183,56 @@ typedef struct {
 } lt_trace_rec;
 uint64_t lt_engine_trace_len(const lt_engine *e);
 lt_status lt_engine_trace(const lt_engine *e, lt_trace_rec *out, uint64_t cap);
+/* State seen by each traced decision: queue lengths (u32[len*P]), residency
+ * (u8[len*P]) and the last 8 loaded partitions (i32[len*8], most recent
+ * first, -1 = none); any output may be NULL.  cap counts records. */
+lt_status lt_engine_trace_state(const lt_engine *e, uint32_t *counts_out, uint8_t *resident_out,
+                                int32_t *recent_out, uint64_t cap);
+
+/* ---- learned scheduler (no counterpart in the reference) ---------------- */
+/* Per-partition MLP, 16 features -> 256 -> 256 -> 2, BF16 weights, FP32
+ * accumulation, trained and evaluated by HIP kernels.  Output 0 scores the
+ * partition the scheduler will need from the host next (used to prefetch it
+ * while the current partition is walked); output 1 predicts the partition's
+ * share of future walker traffic (the resident partition with the lowest
+ * value is evicted).  Engines use it when cfg.policy == LT_SCHED_LEARNED and
+ * a trained scorer is attached; otherwise they fall back to the plain rules. */
+lt_status lt_scorer_create(const lt_partition *p, uint64_t seed, lt_scorer **out);
+void lt_scorer_free(lt_scorer *s);
+uint64_t lt_scorer_num_params(void);
+/* Add the decisions of e's last traced run (record_trace=1) as training
+ * samples: with use_next, the label of output 0 is the partition the traced
+ * scheduler loaded or read over PCIe at its next decision; with use_keep, the
+ * label of output 1 is each partition's share of the walkers the trace
+ * processes after the decision (times P).  Returns the samples added. */
+lt_status lt_scorer_add_trace(lt_scorer *s, const lt_engine *e, int32_t use_next, int32_t use_keep,
+                              uint64_t *added);
+/* Adam on the accumulated samples; loss_out[2] = last epoch's mean losses. */
+lt_status lt_scorer_fit(lt_scorer *s, uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *loss_out);
+/* Explicit samples: n decisions of P partitions each (layouts as in
+ * lt_engine_trace_state); keep_label f32[n*P] may be NULL; wa/wb are the
+ * per-decision loss weights of the two outputs (NULL = 1 and 0). */
+lt_status lt_scorer_train(lt_scorer *s, uint64_t n, const uint32_t *counts, const uint8_t *resident,
+                          const int32_t *recent, const int32_t *cur, const float *w_total,
+                          const int32_t *next_label, const float *keep_label, const float *wa,
+                          const float *wb, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                          float *loss_out);
+/* One forward + backward without an update: losses (f32[2]) and the
+ * gradient w.r.t. the parameters (f32[lt_scorer_num_params()]). */
+lt_status lt_scorer_loss_grad(lt_scorer *s, uint64_t n, const uint32_t *counts, const uint8_t *resident,
+                              const int32_t *recent, const int32_t *cur, const float *w_total,
+                              const int32_t *next_label, const float *keep_label, const float *wa,
+                              const float *wb, float *loss_out, float *grad_out);
+/* Scores (f32[P*2]) of one decision. */
+lt_status lt_scorer_infer(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const int32_t *recent,
+                          int32_t cur, float w_total, float *out);
+/* FP32 master parameters: W1[16][256] b1[256] W2[256][256] b2[256] W3[256][2] b3[2]. */
+lt_status lt_scorer_get_params(const lt_scorer *s, float *out);
+lt_status lt_scorer_set_params(lt_scorer *s, const float *in);
+/* Weights file: "LTMLP1\0\0", u32 16, 256, 2, 0, then the BF16 parameters. */
+lt_status lt_scorer_save(const lt_scorer *s, const char *path);
+lt_status lt_scorer_load(lt_scorer *s, const char *path);
+lt_status lt_engine_set_scorer(lt_engine *e, lt_scorer *s);
 
 /* ---- pure scheduler rules (host, no GPU) -------------------------------- */
 /* Scheduler::nextPartition greedy (src/Scheduler.h:368-388): argmax of

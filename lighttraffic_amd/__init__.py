@@ -63,7 +63,8 @@ class Stats(ctypes.Structure):
                 ("h2d_bytes_explicit", _u64), ("h2d_bytes_zerocopy", _u64), ("moves", _u64),
                 ("teleports", _u64), ("visits", _u64), ("residencies", _u64), ("zerocopy_moves", _u64),
                 ("nominal_steps", _u64), ("kernel_launches", _u64), ("walk_kernel_ms", ctypes.c_double),
-                ("learned_decisions", _u64), ("fallback_decisions", _u64), ("reserved", _u64 * 6)]
+                ("learned_decisions", _u64), ("fallback_decisions", _u64), ("prefetch_issued", _u64),
+                ("prefetch_hits", _u64), ("reserved", _u64 * 4)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
@@ -112,6 +113,21 @@ _sig("lt_run_ppr_batch", ctypes.c_int, _vp, _u32, _vp, _vp, ctypes.c_float, _vp)
 _sig("lt_engine_stats", ctypes.c_int, _vp, ctypes.POINTER(Stats))
 _sig("lt_engine_trace_len", _u64, _vp)
 _sig("lt_engine_trace", ctypes.c_int, _vp, ctypes.POINTER(TraceRec), _u64)
+_sig("lt_engine_trace_state", ctypes.c_int, _vp, _vp, _vp, _vp, _u64)
+_sig("lt_engine_set_scorer", ctypes.c_int, _vp, _vp)
+_sig("lt_scorer_create", ctypes.c_int, _vp, _u64, ctypes.POINTER(_vp))
+_sig("lt_scorer_free", None, _vp)
+_sig("lt_scorer_num_params", _u64)
+_sig("lt_scorer_add_trace", ctypes.c_int, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_u64))
+_sig("lt_scorer_fit", ctypes.c_int, _vp, _u32, _u32, ctypes.c_float, _u64, _vp)
+_sig("lt_scorer_train", ctypes.c_int, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32,
+     ctypes.c_float, _u64, _vp)
+_sig("lt_scorer_loss_grad", ctypes.c_int, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("lt_scorer_infer", ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, _vp)
+_sig("lt_scorer_get_params", ctypes.c_int, _vp, _vp)
+_sig("lt_scorer_set_params", ctypes.c_int, _vp, _vp)
+_sig("lt_scorer_save", ctypes.c_int, _vp, ctypes.c_char_p)
+_sig("lt_scorer_load", ctypes.c_int, _vp, ctypes.c_char_p)
 _sig("lt_sched_greedy_next", _i32, _u32, _vp, _i32, _i32)
 _sig("lt_sched_evict_min", _i32, _u32, _vp, _vp, _i32)
 _sig("lt_sched_computed_partition", _i32, _u32, _vp, _vp, _i32, _u64)
@@ -204,9 +220,9 @@ class Graph:
             :self.num_edge]
 
     def close(self):
-        if self._h:
+        if self._h and _lib is not None:
             _lib.lt_graph_free(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -255,9 +271,9 @@ class Partition:
         return _lib.lt_partition_find(self._h, v)
 
     def close(self):
-        if self._h:
+        if self._h and _lib is not None:
             _lib.lt_partition_free(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -313,10 +329,25 @@ class Engine:
         _check(_lib.lt_engine_trace(self._h, buf, n), "lt_engine_trace")
         return [(r.partition, r.kind, r.loaded, r.evicted, r.walkers) for r in buf[:n]]
 
+    def trace_state(self):
+        """(counts (len,P) u32, resident (len,P) u8, recent (len,8) i32) seen by each traced decision."""
+        n = _lib.lt_engine_trace_len(self._h)
+        P = self.partition.count
+        c = np.zeros((n, P), np.uint32)
+        r = np.zeros((n, P), np.uint8)
+        rc = np.zeros((n, 8), np.int32)
+        _check(_lib.lt_engine_trace_state(self._h, c.ctypes.data, r.ctypes.data, rc.ctypes.data, n),
+               "lt_engine_trace_state")
+        return c, r, rc
+
+    def set_scorer(self, scorer):
+        _check(_lib.lt_engine_set_scorer(self._h, None if scorer is None else scorer._h), "lt_engine_set_scorer")
+        self._scorer = scorer
+
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and _lib is not None:
             _lib.lt_engine_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -337,3 +368,88 @@ def sched_computed_partition(counts, resident, except_, page):
     c = np.ascontiguousarray(counts, dtype=np.uint64)
     r = np.ascontiguousarray(resident, dtype=np.uint8)
     return _lib.lt_sched_computed_partition(len(c), c.ctypes.data, r.ctypes.data, except_, page)
+
+
+class Scorer:
+    """Learned partition scorer: per-partition BF16 MLP trained by HIP kernels."""
+
+    NUM_PARAMS = int(_lib.lt_scorer_num_params())
+
+    def __init__(self, partition, seed=1):
+        h = _vp()
+        _check(_lib.lt_scorer_create(partition._h, seed, ctypes.byref(h)), "lt_scorer_create")
+        self._h = h
+        self.P = partition.count
+
+    def add_trace(self, engine, use_next=True, use_keep=True):
+        n = _u64()
+        _check(_lib.lt_scorer_add_trace(self._h, engine._h, int(use_next), int(use_keep), ctypes.byref(n)),
+               "lt_scorer_add_trace")
+        return n.value
+
+    def fit(self, epochs=20, batch=32, lr=1e-3, seed=7):
+        loss = np.zeros(2, np.float32)
+        _check(_lib.lt_scorer_fit(self._h, epochs, batch, lr, seed, loss.ctypes.data), "lt_scorer_fit")
+        return loss
+
+    @staticmethod
+    def _arrays(samples, P):
+        n = len(samples)
+        counts = np.ascontiguousarray(np.stack([s["counts"] for s in samples]), dtype=np.uint32)
+        resident = np.ascontiguousarray(np.stack([s["resident"] for s in samples]), dtype=np.uint8)
+        recent = np.ascontiguousarray(np.stack([s["recent"] for s in samples]), dtype=np.int32)
+        cur = np.array([s["cur"] for s in samples], np.int32)
+        wt = np.array([s["w_total"] for s in samples], np.float32)
+        nxt = np.array([s["next"] for s in samples], np.int32)
+        keep = np.ascontiguousarray(np.stack([s["keep"] for s in samples]), dtype=np.float32)
+        wa = np.array([s["wa"] for s in samples], np.float32)
+        wb = np.array([s["wb"] for s in samples], np.float32)
+        assert counts.shape == (n, P)
+        return counts, resident, recent, cur, wt, nxt, keep, wa, wb
+
+    def loss_grad(self, samples):
+        arrs = self._arrays(samples, self.P)
+        loss = np.zeros(2, np.float32)
+        grad = np.zeros(self.NUM_PARAMS, np.float32)
+        _check(_lib.lt_scorer_loss_grad(self._h, len(samples), *[a.ctypes.data for a in arrs], loss.ctypes.data,
+                                        grad.ctypes.data), "lt_scorer_loss_grad")
+        return loss, grad
+
+    def train(self, samples, epochs=10, batch=32, lr=1e-3, seed=7):
+        arrs = self._arrays(samples, self.P)
+        loss = np.zeros(2, np.float32)
+        _check(_lib.lt_scorer_train(self._h, len(samples), *[a.ctypes.data for a in arrs], epochs, batch, lr, seed,
+                                    loss.ctypes.data), "lt_scorer_train")
+        return loss
+
+    def infer(self, counts, resident, recent, cur, w_total):
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        resident = np.ascontiguousarray(resident, dtype=np.uint8)
+        recent = np.ascontiguousarray(recent, dtype=np.int32)
+        out = np.zeros((self.P, 2), np.float32)
+        _check(_lib.lt_scorer_infer(self._h, counts.ctypes.data, resident.ctypes.data, recent.ctypes.data, cur,
+                                    w_total, out.ctypes.data), "lt_scorer_infer")
+        return out
+
+    def get_params(self):
+        out = np.zeros(self.NUM_PARAMS, np.float32)
+        _check(_lib.lt_scorer_get_params(self._h, out.ctypes.data), "lt_scorer_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(_lib.lt_scorer_set_params(self._h, p.ctypes.data), "lt_scorer_set_params")
+
+    def save(self, path):
+        _check(_lib.lt_scorer_save(self._h, path.encode()), "lt_scorer_save")
+
+    def load(self, path):
+        _check(_lib.lt_scorer_load(self._h, path.encode()), "lt_scorer_load")
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.lt_scorer_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()

@@ -17,6 +17,7 @@
 
 #include "lt_internal.h"
 #include "lt_kernels.h"
+#include "lt_mlp.h"
 #include "lt_sched.h"
 
 #define LT_HIP(call)                                                                                   \
@@ -173,6 +174,13 @@ struct Engine {
 
     lt_stats st{};
     std::vector<lt_trace_rec> trace;
+    std::vector<uint32_t> trace_counts;  // queue lengths seen by each traced decision
+    std::vector<uint8_t> trace_resident; // residency seen by each traced decision
+    std::vector<int32_t> trace_recent;   // last kRecent loads at each traced decision
+    Scorer *scorer = nullptr;             // learned policy (not owned)
+    std::vector<float> scores;            // scorer output of the current decision
+    bool scores_valid = false;
+    uint64_t w_run = 0;                   // walkers of the current wave
     EventTimer t_load, t_compute, t_zc, t_wload, t_wevict, t_kernel;
     Scheduler sched;
 
@@ -184,11 +192,16 @@ struct Engine {
 
     // scheduler primitives
     lt_status read_meta();
-    lt_status process(uint32_t p, bool zerocopy, uint64_t walkers_hint, int trace_kind);
+    lt_status process(uint32_t p, bool zerocopy, uint64_t walkers_hint);
+    void record(uint32_t p, int kind, int32_t loaded, int32_t evicted);
     lt_status evict(uint32_t p);
     lt_status load_async(uint32_t p);
     lt_status set_table(uint32_t p, bool present);
     lt_status scheduler_loop();
+    lt_status scheduler_loop_learned();
+    lt_status infer_scores(int32_t cur);
+    int32_t learned_victim(int32_t a, int32_t b);
+    void recent_loads(int32_t *out) const;
     lt_status run_partition(uint32_t p, int32_t &prev_loaded);
 
     AppArgs app{};
@@ -438,7 +451,7 @@ lt_status Engine::load_async(uint32_t p) {
 // Consume the whole queue of partition p in slices of kSliceChunks chunks:
 // walk kernel, then fix-up kernel per slice (processInMemory with all
 // batches, src/WalkManager.h:375-388).
-lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers, int trace_kind) {
+lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
     if (walkers == 0) return LT_OK;
     WalkArgs a{};
     a.app = app;
@@ -486,8 +499,66 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers, int trace
         st.kernel_launches++;
     }
     phase.end(tp, s_compute);
-    if (cfg.record_trace) trace.push_back(lt_trace_rec{(int32_t)p, trace_kind, -1, -1, walkers});
     return LT_OK;
+}
+
+// One traced decision: the choice and the queue lengths it was taken on.
+void Engine::record(uint32_t p, int kind, int32_t loaded, int32_t evicted) {
+    if (!cfg.record_trace) return;
+    trace.push_back(lt_trace_rec{(int32_t)p, kind, loaded, evicted, counts[p]});
+    for (uint32_t i = 0; i < P; i++) trace_counts.push_back((uint32_t)counts[i]);
+    trace_resident.insert(trace_resident.end(), resident.begin(), resident.end());
+    int32_t rec[kRecent];
+    recent_loads(rec);
+    trace_recent.insert(trace_recent.end(), rec, rec + kRecent);
+}
+
+void Engine::recent_loads(int32_t *out) const {
+    const auto &h = sched.history;
+    for (int k = 0; k < kRecent; k++) out[k] = k < (int)h.size() ? h[h.size() - 1 - k] : -1;
+}
+
+// Scores of every partition for the decision that chose `cur`.
+lt_status Engine::infer_scores(int32_t cur) {
+    scores_valid = false;
+    if (!scorer || !scorer_trained(scorer)) return LT_OK;
+    std::vector<uint32_t> c32(P);
+    for (uint32_t i = 0; i < P; i++) c32[i] = (uint32_t)std::min<uint64_t>(counts[i], 0xFFFFFFFFu);
+    DecisionState d;
+    d.counts = c32.data();
+    d.resident = resident.data();
+    recent_loads(d.recent);
+    d.cur = cur;
+    d.w_total = (float)w_run;
+    scores.resize((size_t)P * kOut);
+    LT_TRY(scorer_infer(scorer, d, nullptr, scores.data()));
+    bool ok = true;
+    for (float v : scores) ok = ok && std::isfinite(v);
+    scores_valid = ok;
+    return LT_OK;
+}
+
+// Resident partition (other than a and b) with the lowest keep score; falls
+// back to the plain rule without valid scores.
+int32_t Engine::learned_victim(int32_t a, int32_t b) {
+    if (scores_valid) {
+        int32_t v = -1;
+        float best = 0.f;
+        for (uint32_t i = 0; i < P; i++) {
+            if (!resident[i] || (int32_t)i == a || (int32_t)i == b) continue;
+            const float k = scores[(size_t)i * kOut + 1];
+            if (v < 0 || k < best) {
+                best = k;
+                v = (int32_t)i;
+            }
+        }
+        if (v >= 0) {
+            sched.learned_decisions++;
+            return v;
+        }
+    }
+    sched.fallback_decisions++;
+    return lt_sched_evict_min(P, counts.data(), resident.data(), a);
 }
 
 // GPUScheduler::run (src/Scheduler.h:311-330) without host walker spill.
@@ -497,12 +568,13 @@ lt_status Engine::run_partition(uint32_t p, int32_t &prev_loaded) {
     // shouldZerocopy (src/Scheduler.h:243-252): counts were read this iteration.
     if (!resident[p] && n < zc_threshold) {
         st.zerocopy_calls++;
-        return process(p, true, n, 1);
+        record(p, 1, -1, -1);
+        return process(p, true, n);
     }
     st.explicit_iterations++;
-    int32_t loaded = -1, evicted = -1;
     // graphLoadingPhase (src/Scheduler.h:173-213)
     if (!resident[p]) {
+        int32_t evicted = -1;
         if (free_slots.empty()) {
             const int32_t victim = sched.evict_choice(counts.data(), resident.data(), prev_loaded);
             if (victim < 0) {
@@ -513,9 +585,9 @@ lt_status Engine::run_partition(uint32_t p, int32_t &prev_loaded) {
             evicted = victim;
             sched.on_evict((uint32_t)victim);
         }
+        record(p, 3, (int32_t)p, evicted);
         LT_TRY(load_async(p));
         st.graph_loading_calls++;
-        loaded = (int32_t)p;
         sched.on_load(p);
         if (cfg.pipeline) {
             // compute resident partitions while the copy is in flight
@@ -524,18 +596,22 @@ lt_status Engine::run_partition(uint32_t p, int32_t &prev_loaded) {
                 const int32_t c = sched.computed_choice(counts.data(), resident.data(), (int32_t)p, kRefPage);
                 if (c < 0) break;
                 st.computing_calls++;
-                LT_TRY(process((uint32_t)c, false, counts[c], 2));
+                record((uint32_t)c, 2, -1, -1);
+                LT_TRY(process((uint32_t)c, false, counts[c]));
                 LT_HIP(hipStreamSynchronize(s_compute));
             }
+            // the pipelined work changed the queues; p's count is re-read below
+            LT_TRY(read_meta());
         }
         LT_HIP(hipStreamSynchronize(s_copy));
         resident[p] = 1;
         LT_TRY(set_table(p, true));
+    } else {
+        record(p, 0, -1, -1);
     }
     prev_loaded = (int32_t)p;
-    if (cfg.record_trace) trace.push_back(lt_trace_rec{(int32_t)p, 3, loaded, evicted, n});
     st.computing_calls++;
-    return process(p, false, n, 0);
+    return process(p, false, counts[p]);
 }
 
 // Scheduler::start (src/Scheduler.h:425-438).
@@ -560,10 +636,116 @@ lt_status Engine::scheduler_loop() {
     return LT_OK;
 }
 
+// Learned scheduling: the partition to walk is still chosen by the plain
+// rule, but while it is walked the scorer's best guess for the next partition
+// that needs host data is copied into a free slot (prefetch), and every
+// eviction drops the resident partition with the lowest predicted future
+// traffic.  Evictions happen only while the compute stream is idle, and a
+// prefetched slot joins the resident table only after its copy completed.
+lt_status Engine::scheduler_loop_learned() {
+    int32_t prev = -1;
+    bool sync = false;
+    int32_t incoming = -1;
+    std::vector<uint8_t> prefetched(P, 0);  // resident because of a prefetch, not yet picked
+    sched.begin_run();
+    auto finish_incoming = [&](bool wait) -> lt_status {
+        if (incoming < 0) return LT_OK;
+        if (!wait && hipStreamQuery(s_copy) != hipSuccess) return LT_OK;
+        LT_HIP(hipStreamSynchronize(s_copy));
+        resident[incoming] = 1;
+        LT_TRY(set_table((uint32_t)incoming, true));
+        incoming = -1;
+        return LT_OK;
+    };
+    while (true) {
+        LT_TRY(read_meta());
+        LT_TRY(finish_incoming(false));
+        const int32_t c = sched.next(counts.data(), resident.data(), prev, sync);
+        if (c < 0) {
+            if (sync) break;
+            sync = true;
+            continue;
+        }
+        sync = false;
+        prev = c;
+        st.iterations++;
+        scores_valid = false;
+        bool zc = false;
+        if (c == incoming) LT_TRY(finish_incoming(true));
+        if (resident[c]) {
+            if (prefetched[c]) st.prefetch_hits++;
+            prefetched[c] = 0;
+            record((uint32_t)c, 0, -1, -1);
+        } else if (counts[c] < zc_threshold) {
+            zc = true;
+            st.zerocopy_calls++;
+            record((uint32_t)c, 1, -1, -1);
+        } else {
+            LT_TRY(finish_incoming(true));  // the copy engine is in-order anyway
+            int32_t evicted = -1;
+            if (free_slots.empty()) {
+                LT_TRY(infer_scores(c));
+                evicted = learned_victim(c, -1);
+                if (evicted < 0) {
+                    set_error("no partition to evict");
+                    return LT_E_INTERNAL;
+                }
+                LT_TRY(evict((uint32_t)evicted));
+            }
+            record((uint32_t)c, 3, c, evicted);
+            LT_TRY(load_async((uint32_t)c));
+            st.graph_loading_calls++;
+            sched.on_load((uint32_t)c);
+            LT_HIP(hipStreamSynchronize(s_copy));
+            resident[c] = 1;
+            LT_TRY(set_table((uint32_t)c, true));
+        }
+        if (!zc) st.explicit_iterations++;
+        // prefetch the predicted next load while c is walked
+        if (incoming < 0 && scorer) {
+            if (!scores_valid) LT_TRY(infer_scores(c));
+            int32_t q = -1;
+            if (scores_valid) {
+                float best = 0.f;
+                for (uint32_t i = 0; i < P; i++) {
+                    if (resident[i] || (int32_t)i == c || counts[i] < zc_threshold) continue;
+                    const float v = scores[(size_t)i * kOut];
+                    if (q < 0 || v > best) {
+                        best = v;
+                        q = (int32_t)i;
+                    }
+                }
+            }
+            if (q >= 0) {
+                if (free_slots.empty()) {
+                    const int32_t v = learned_victim(c, q);
+                    if (v >= 0) LT_TRY(evict((uint32_t)v));
+                }
+                if (!free_slots.empty()) {
+                    LT_TRY(load_async((uint32_t)q));
+                    st.graph_loading_calls++;
+                    sched.on_load((uint32_t)q);
+                    st.prefetch_issued++;
+                    prefetched[q] = 1;
+                    incoming = q;
+                }
+            }
+        }
+        st.computing_calls++;
+        LT_TRY(process((uint32_t)c, zc, counts[c]));
+    }
+    LT_TRY(finish_incoming(true));
+    LT_HIP(hipDeviceSynchronize());
+    return LT_OK;
+}
+
 lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoints_out, uint32_t *moves_out) {
     LT_HIP(hipSetDevice(device));
     memset(&st, 0, sizeof(st));
     trace.clear();
+    trace_counts.clear();
+    trace_resident.clear();
+    trace_recent.clear();
     t_load.reset();
     t_compute.reset();
     t_zc.reset();
@@ -632,7 +814,8 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         LT_HIP(hipGetLastError());
         LT_HIP(hipStreamSynchronize(s_compute));
         const auto t0 = std::chrono::steady_clock::now();
-        LT_TRY(scheduler_loop());
+        w_run = id1 - id0;
+        LT_TRY(cfg.policy == LT_SCHED_LEARNED && scorer ? scheduler_loop_learned() : scheduler_loop());
         const auto t1 = std::chrono::steady_clock::now();
         running_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
@@ -762,6 +945,85 @@ lt_status lt_engine_stats(const lt_engine *e, lt_stats *out) {
 }
 
 uint64_t lt_engine_trace_len(const lt_engine *e) { return e ? e->trace.size() : 0; }
+
+lt_status lt_engine_trace_state(const lt_engine *e, uint32_t *counts_out, uint8_t *resident_out, int32_t *recent_out,
+                                uint64_t cap) {
+    if (!e) return LT_E_ARG;
+    const uint64_t n = std::min<uint64_t>(cap, e->trace.size());
+    const size_t P = e->P;
+    if (counts_out && n) memcpy(counts_out, e->trace_counts.data(), n * P * sizeof(uint32_t));
+    if (resident_out && n) memcpy(resident_out, e->trace_resident.data(), n * P);
+    if (recent_out && n) memcpy(recent_out, e->trace_recent.data(), n * kRecent * sizeof(int32_t));
+    return LT_OK;
+}
+
+// Training samples from a traced run.  Decisions are the records of kind 0,
+// 1 and 3 (kind 2 = pipelined work inside a load).  Output 0's label is the
+// next decision's partition when that decision needed host data (kind 1 or
+// 3); output 1's label is each partition's share of the walkers processed by
+// all later records, times P.
+lt_status lt_scorer_add_trace(lt_scorer *s_, const lt_engine *e, int32_t use_next, int32_t use_keep,
+                              uint64_t *added) {
+    if (!s_ || !e) return LT_E_ARG;
+    lt::Scorer *s = (lt::Scorer *)s_;
+    const uint32_t P = e->P;
+    if (scorer_partitions(s) != P) {
+        set_error("scorer built for %u partitions, engine has %u", scorer_partitions(s), P);
+        return LT_E_ARG;
+    }
+    const auto &tr = e->trace;
+    const size_t n = tr.size();
+    if (n == 0) {
+        set_error("the engine holds no trace (set record_trace)");
+        return LT_E_ARG;
+    }
+    // suffix traffic per partition, built from the back
+    std::vector<double> future(P, 0.0);
+    std::vector<std::vector<float>> keep_at(n);
+    double total = 0;
+    int32_t next_need = -1;
+    std::vector<int32_t> next_of(n, -1);
+    for (size_t j = n; j-- > 0;) {
+        const auto &r = tr[j];
+        if (r.kind != 2) {
+            next_of[j] = next_need;
+            keep_at[j].resize(P);
+            for (uint32_t q = 0; q < P; q++) keep_at[j][q] = total > 0 ? (float)(future[q] / total * P) : 0.f;
+            next_need = (r.kind == 1 || r.kind == 3) ? r.partition : -1;
+        }
+        future[r.partition] += (double)r.walkers;
+        total += (double)r.walkers;
+    }
+    Samples &d = scorer_samples(s);
+    uint64_t cnt = 0;
+    for (size_t j = 0; j < n; j++) {
+        const auto &r = tr[j];
+        if (r.kind == 2) continue;
+        const bool has_next = use_next && next_of[j] >= 0;
+        const bool has_keep = use_keep != 0;
+        if (!has_next && !has_keep) continue;
+        d.counts.insert(d.counts.end(), e->trace_counts.begin() + j * P, e->trace_counts.begin() + (j + 1) * P);
+        d.resident.insert(d.resident.end(), e->trace_resident.begin() + j * P, e->trace_resident.begin() + (j + 1) * P);
+        d.recent.insert(d.recent.end(), e->trace_recent.begin() + j * kRecent,
+                        e->trace_recent.begin() + (j + 1) * kRecent);
+        d.cur.push_back(r.partition);
+        d.next.push_back(has_next ? next_of[j] : -1);
+        d.w_total.push_back((float)e->w_run);
+        d.keep.insert(d.keep.end(), keep_at[j].begin(), keep_at[j].end());
+        d.wa.push_back(has_next ? 1.f : 0.f);
+        d.wb.push_back(has_keep ? 1.f : 0.f);
+        d.n++;
+        cnt++;
+    }
+    if (added) *added = cnt;
+    return LT_OK;
+}
+
+lt_status lt_engine_set_scorer(lt_engine *e, lt_scorer *s) {
+    if (!e) return LT_E_ARG;
+    e->scorer = (lt::Scorer *)s;
+    return LT_OK;
+}
 
 lt_status lt_engine_trace(const lt_engine *e, lt_trace_rec *out, uint64_t cap) {
     if (!e || !out) return LT_E_ARG;

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <stdlib.h>
+
 #include <memory>
 
 #include "lt_internal.h"
@@ -153,15 +155,30 @@ static lt_status generate(uint32_t scale, uint32_t ef, double a, double b, doubl
     uint64_t *sorted = db.Current();
     uint64_t *uniq = (sorted == keys.p) ? alt.p : keys.p;
 
+    // Deduplicate in pieces of at most 2^30 keys.  Piece j > 0 starts one key
+    // early: that key equals the last one already kept, so its copy lands on
+    // the same slot and the piece adds nsel-1 new keys.
     DevBuf<uint64_t> d_nsel;
     RM_HIP(d_nsel.alloc(1));
-    tmp_bytes = 0;
-    RM_HIP(hipcub::DeviceSelect::Unique(nullptr, tmp_bytes, sorted, uniq, d_nsel.p, nk, s));
-    RM_HIP(tmp.alloc(tmp_bytes));
-    RM_HIP(hipcub::DeviceSelect::Unique(tmp.p, tmp_bytes, sorted, uniq, d_nsel.p, nk, s));
+    uint64_t piece = 1ull << 30;
+    if (const char *env = getenv("LT_RMAT_DEDUPE_PIECE")) {  // smaller pieces exercise the seams in tests
+        const uint64_t v = strtoull(env, nullptr, 10);
+        if (v >= 2) piece = v;
+    }
     uint64_t nsel = 0, last = 0;
-    RM_HIP(hipMemcpyAsync(&nsel, d_nsel.p, 8, hipMemcpyDeviceToHost, s));
-    RM_HIP(hipStreamSynchronize(s));
+    for (uint64_t lo = 0; lo < nk; lo += piece) {
+        const uint64_t hi = nk - lo < piece ? nk : lo + piece;
+        const uint64_t first = lo == 0 ? 0 : lo - 1;
+        uint64_t *dst_ptr = uniq + (lo == 0 ? 0 : nsel - 1);
+        tmp_bytes = 0;
+        RM_HIP(hipcub::DeviceSelect::Unique(nullptr, tmp_bytes, sorted + first, dst_ptr, d_nsel.p, hi - first, s));
+        RM_HIP(tmp.alloc(tmp_bytes));
+        RM_HIP(hipcub::DeviceSelect::Unique(tmp.p, tmp_bytes, sorted + first, dst_ptr, d_nsel.p, hi - first, s));
+        uint64_t got = 0;
+        RM_HIP(hipMemcpyAsync(&got, d_nsel.p, 8, hipMemcpyDeviceToHost, s));
+        RM_HIP(hipStreamSynchronize(s));
+        nsel += lo == 0 ? got : got - 1;
+    }
     const uint64_t sentinel = (1ull << (2 * scale)) - 1;
     uint64_t E = nsel;
     if (E) {

@@ -189,3 +189,13 @@ def test_s20_genericwalk_subsample():
         assert np.array_equal(ends[lo:lo + 5000], re) and np.array_equal(moves[lo:lo + 5000], rm)
     st = eng.stats()
     assert st["moves"] == int(moves.sum())
+
+
+def test_generator_dedupe_seams(monkeypatch):
+    """Deduplication in small pieces gives the same graph (pieces meet inside runs of equal keys)."""
+    log("generator dedupe pieces")
+    rowptr, col = oracle.rmat_graph(12, 16)
+    for piece in ("1000", "4096", "7"):
+        monkeypatch.setenv("LT_RMAT_DEDUPE_PIECE", piece)
+        g = lt.Graph.rmat(12, 16)
+        assert np.array_equal(g.rowptr(), rowptr) and np.array_equal(g.col(), col)

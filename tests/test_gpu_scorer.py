@@ -1,0 +1,158 @@
+"""GPU tests of the learned scheduler: HIP MLP kernels against the FP64
+oracle (oracle/mlp_ref.py), training, the weights file, and the learned
+policy inside the engine (results bit-exact against the walk oracle, since the
+policy only changes the order of work and the cache contents)."""
+import numpy as np
+import pytest
+
+import lighttraffic_amd as lt
+from oracle import ffi as oracle
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def _part(P, rng):
+    node = np.concatenate([[0], np.cumsum(rng.integers(1, 50, P))]).astype(np.uint32)
+    edge = np.concatenate([[0], np.cumsum(rng.integers(0, 400, P))]).astype(np.uint64)
+    return node, edge
+
+
+def _partition_handle(node, edge):
+    h = lt._vp()
+    lt._check(lt.lib().lt_partition_from_offsets(len(node) - 1, 1 << 20, node.ctypes.data, edge.ctypes.data,
+                                                 lt.ctypes.byref(h)), "lt_partition_from_offsets")
+    return lt.Partition(h)
+
+
+def _sample(rng, P, with_next=True):
+    counts = (rng.integers(0, 5, P) * rng.integers(0, 100000, P)).astype(np.uint32)
+    return {"counts": counts, "resident": (rng.random(P) < 0.5).astype(np.uint8),
+            "recent": rng.integers(-1, P, 8).astype(np.int32), "cur": int(rng.integers(0, P)),
+            "w_total": float(counts.sum() * 3 + 1), "next": int(rng.integers(0, P)) if with_next else -1,
+            "keep": (rng.random(P) * 2).astype(np.float32), "wa": 1.0, "wb": 1.0}
+
+
+@pytest.mark.parametrize("P", [5, 37, 429])
+def test_loss_and_gradient_match_fp64_oracle(P):
+    log(f"scorer gradient P={P}")
+    rng = np.random.default_rng(P)
+    node, edge = _part(P, rng)
+    part = _partition_handle(node, edge)
+    sc = lt.Scorer(part, seed=3)
+    params = sc.get_params()
+    samples = [_sample(rng, P, with_next=(i % 3 != 2)) for i in range(6)]
+    loss, grad = sc.loss_grad(samples)
+    vsh, esh = M.shares(node, edge)
+    rloss, rgrad = M.loss_and_grad(params.astype(np.float64), samples, vsh, esh)
+    assert np.allclose(loss, rloss, rtol=2e-3, atol=1e-5), (loss, rloss)
+    scale = np.abs(rgrad).max()
+    assert np.abs(grad - rgrad).max() <= 2e-3 * scale, (np.abs(grad - rgrad).max(), scale)
+    # per block too, so a wrong small block cannot hide behind a large one
+    off = 0
+    for name, shape in M.SIZES:
+        n = int(np.prod(shape))
+        g, r = grad[off:off + n], rgrad[off:off + n]
+        assert np.abs(g - r).max() <= 2e-3 * max(np.abs(r).max(), 1e-6) + 1e-7, name
+        off += n
+
+
+def test_inference_matches_oracle_and_weights_file(tmp_path):
+    log("scorer inference + file")
+    rng = np.random.default_rng(11)
+    P = 123
+    node, edge = _part(P, rng)
+    part = _partition_handle(node, edge)
+    sc = lt.Scorer(part, seed=5)
+    s = _sample(rng, P)
+    out = sc.infer(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"])
+    Wb = {k: M.bf16_round(v) for k, v in M.unpack(sc.get_params().astype(np.float64)).items()}
+    vsh, esh = M.shares(node, edge)
+    _, _, Y = M.forward(Wb, M.features(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"], vsh, esh))
+    assert np.allclose(out, Y, rtol=1e-3, atol=1e-4 * np.abs(Y).max())
+    path = str(tmp_path / "w.ltmlp")
+    sc.save(path)
+    sc2 = lt.Scorer(part, seed=99)
+    sc2.load(path)
+    assert np.array_equal(sc2.get_params(), M.bf16_round(sc.get_params()).astype(np.float32))
+    assert np.array_equal(sc2.infer(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"]), out)
+
+
+def test_training_reduces_loss():
+    """A learnable rule: the next pick is the partition with the most walkers
+    among the non-resident ones; traffic share equals the vertex share."""
+    log("scorer training")
+    rng = np.random.default_rng(2)
+    P = 64
+    node, edge = _part(P, rng)
+    part = _partition_handle(node, edge)
+    vsh, _ = M.shares(node, edge)
+    samples = []
+    for i in range(512):
+        s = _sample(rng, P)
+        cand = np.where(s["resident"] == 0, s["counts"].astype(np.int64), -1)
+        s["next"] = int(np.argmax(cand)) if cand.max() > 0 else -1
+        s["keep"] = vsh.astype(np.float32)
+        samples.append(s)
+    sc = lt.Scorer(part, seed=1)
+    before, _ = sc.loss_grad(samples[:64])
+    sc.train(samples, epochs=30, batch=32, lr=2e-3)
+    after, _ = sc.loss_grad(samples[:64])
+    log(f"losses before {before} after {after}")
+    assert after[0] < 0.5 * before[0] and after[1] < 0.5 * before[1]
+    hits = 0
+    for s in samples[:64]:
+        y = sc.infer(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"])
+        cand = (s["resident"] == 0) & (s["counts"] > 0)
+        if cand.any():
+            hits += int(np.argmax(np.where(cand, y[:, 0], -np.inf)) == s["next"])
+    assert hits >= 40
+
+
+def test_learned_policy_keeps_results_bit_exact():
+    log("learned policy in the engine")
+    g = lt.Graph.rmat(14, 16)
+    part = lt.Partition.build(g, 32 << 10)
+    slots = max(3, part.count // 3)
+    w = 2 * g.num_node
+    plain = lt.Engine(g, part, num_device_partitions=slots, record_trace=1, zerocopy_threshold=500)
+    c0 = plain.pagerank(w, 10)
+    c0 = plain.pagerank(w, 10)
+    sc = lt.Scorer(part, seed=4)
+    n = sc.add_trace(plain)
+    assert n > 10
+    loss = sc.fit(epochs=10, batch=16, lr=1e-3)
+    assert np.all(np.isfinite(loss))
+    for span in ("partition", "resident"):
+        eng = lt.Engine(g, part, num_device_partitions=slots, policy="learned", span=span, zerocopy_threshold=500)
+        eng.set_scorer(sc)
+        counts, ends = eng.pagerank(w, 10, endpoints=True)
+        rc, re, _ = oracle.pagerank(g.rowptr(), g.col(), w, 10, endpoints=True)
+        assert np.array_equal(counts, rc) and np.array_equal(ends, re)
+        st = eng.stats()
+        log(f"{span}: prefetch issued {st['prefetch_issued']} hits {st['prefetch_hits']} "
+            f"learned evictions {st['learned_decisions']} loads {st['graph_loading_calls']}")
+        assert st["prefetch_issued"] > 0 and st["learned_decisions"] > 0
+        ends2, _ = eng.genericwalk(w, 8)
+        re2, _, _ = oracle.genericwalk(g.rowptr(), g.col(), w, 8)
+        assert np.array_equal(ends2, re2)
+    assert np.array_equal(c0, rc)
+
+
+def test_learned_policy_without_trained_scorer_falls_back():
+    log("learned policy fallback")
+    g = lt.Graph.rmat(12, 8)
+    part = lt.Partition.build(g, 16 << 10)
+    eng = lt.Engine(g, part, num_device_partitions=3, policy="learned")
+    sc = lt.Scorer(part)  # untrained: no scores, plain rules
+    eng.set_scorer(sc)
+    w = 2 * g.num_node
+    counts = eng.pagerank(w, 10)
+    rc, _, _ = oracle.pagerank(g.rowptr(), g.col(), w, 10)
+    assert np.array_equal(counts, rc)
+    st = eng.stats()
+    assert st["prefetch_issued"] == 0 and st["learned_decisions"] == 0 and st["fallback_decisions"] > 0

@@ -1,0 +1,203 @@
+"""Host-side tests of liblighttraffic.so (no GPU needed).
+
+- the library loads and exports every function include/lighttraffic.h declares;
+- bcsr and .part files: byte layout (src/GraphLoader.h:30-48,
+  src/partition/PartitionStrategy.h:76-97, 325-337) and round trips;
+- the greedy partitioner against the oracle and its invariants
+  (src/partition/PartitionStrategy.h:189-235);
+- the pure scheduler rules against the oracle (src/Scheduler.h:84-171, 368-388);
+- error paths: missing/truncated files, oversize vertex, no device.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import lighttraffic_amd as lt
+from oracle import ffi as oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLDEN = os.path.join(HERE, "golden")
+HEADER = os.path.join(ROOT, "include", "lighttraffic.h")
+
+
+def _no_device():
+    try:
+        lt.device_info()
+        return False
+    except lt.LTError:
+        return True
+
+
+def test_every_declared_symbol_is_exported():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(lt_[a-z0-9_]+)\s*\(", text))
+    assert len(names) > 30
+    out = subprocess.run(["nm", "-D", "--defined-only", lt.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (lt_\w+)", out))
+    assert names <= exported, sorted(names - exported)
+
+
+def test_status_strings_and_defaults():
+    L = lt.lib()
+    for code in range(9):
+        assert L.lt_status_str(code)
+    c = lt.default_config()
+    assert c.zerocopy_threshold == 2 ** 64 - 1 and c.pipeline == 1 and c.policy == 0
+
+
+@pytest.mark.parametrize("name", ["hand", "rmat_s8_ef4", "rmat_s10_ef8"])
+def test_bcsr_open_and_roundtrip(name, tmp_path):
+    path = os.path.join(GOLDEN, name + ".bcsr")
+    g = lt.Graph.open(path)
+    rowptr, col = oracle.read_bcsr(path)
+    assert g.num_node == len(rowptr) - 1 and g.num_edge == len(col)
+    assert np.array_equal(g.rowptr(), rowptr) and np.array_equal(g.col(), col)
+    out = tmp_path / "x.bcsr"
+    g.save(str(out))
+    assert out.read_bytes() == open(path, "rb").read()
+    # header: u32 V, u64 E, no padding
+    raw = out.read_bytes()
+    assert len(raw) == 12 + 8 * (g.num_node + 1) + 4 * g.num_edge
+
+
+def test_bcsr_errors(tmp_path):
+    with pytest.raises(lt.LTError) as e:
+        lt.Graph.open(str(tmp_path / "missing.bcsr"))
+    assert e.value.status == 1
+    raw = open(os.path.join(GOLDEN, "hand.bcsr"), "rb").read()
+    (tmp_path / "short.bcsr").write_bytes(raw[:-3])
+    with pytest.raises(lt.LTError) as e:
+        lt.Graph.open(str(tmp_path / "short.bcsr"))
+    assert e.value.status == 2
+    bad = bytearray(raw)
+    bad[-1] = 0xFF  # a column id >= V
+    (tmp_path / "bad.bcsr").write_bytes(bytes(bad))
+    with pytest.raises(lt.LTError) as e:
+        lt.Graph.open(str(tmp_path / "bad.bcsr"))
+    assert e.value.status == 2
+
+
+def _random_csr(rng, V, maxdeg):
+    deg = rng.integers(0, maxdeg, V)
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    col = rng.integers(0, V, int(rowptr[-1])).astype(np.uint32)
+    return rowptr, col
+
+
+def test_partition_matches_oracle():
+    rng = np.random.default_rng(11)
+    for trial in range(25):
+        V = int(rng.integers(1, 3000))
+        rowptr, col = _random_csr(rng, V, 40)
+        g = lt.Graph.from_csr(rowptr, col)
+        S = int(rng.integers(8 * 2 + 4 * 40, 20000))
+        p = lt.Partition.build(g, S)
+        no, eo = oracle.partition(rowptr, S)
+        assert np.array_equal(p.node_offsets(), no) and np.array_equal(p.edge_offsets(), eo)
+        for v in rng.integers(0, V, 50):
+            assert p.find(int(v)) == oracle.find_partition(no, int(v))
+
+
+def test_partition_rmat_golden_and_counts():
+    g = lt.Graph.open(os.path.join(GOLDEN, "rmat_s10_ef8.bcsr"))
+    total = 8 * (g.num_node + 1) + 4 * g.num_edge
+    for S in (4096, 10000, 1 << 16, 1 << 25):
+        p = lt.Partition.build(g, S)
+        no, eo = oracle.partition(g.rowptr(), S)
+        assert np.array_equal(p.node_offsets(), no)
+        # greedy count is at least ceil(total/S) (each part <= S, parts share no rows)
+        assert p.count >= -(-total // S)
+    assert lt.Partition.build(g, 1 << 25).count == 1
+
+
+def test_partition_file_layout_and_roundtrip(tmp_path):
+    src = os.path.join(GOLDEN, "rmat_s10_ef8.bcsr")
+    gpath = str(tmp_path / "g.bcsr")
+    open(gpath, "wb").write(open(src, "rb").read())
+    g = lt.Graph.open(gpath)
+    p = lt.Partition.build(g, 8192)
+    p.save(gpath)
+    raw = open(gpath + ".part", "rb").read()
+    P = p.count
+    assert len(raw) == 2 + 8 + 4 * (P + 1) + 8 * (P + 1)  # u16 P | u64 S | u32[P+1] | u64[P+1]
+    assert int(np.frombuffer(raw[:2], np.uint16)[0]) == P
+    assert int(np.frombuffer(raw[2:10], np.uint64)[0]) == 8192
+    assert np.array_equal(np.frombuffer(raw[10:10 + 4 * (P + 1)], np.uint32), p.node_offsets())
+    assert np.array_equal(np.frombuffer(raw[10 + 4 * (P + 1):], np.uint64), p.edge_offsets())
+    q = lt.Partition.load(gpath)
+    assert q.count == P and q.bytes == 8192
+    assert np.array_equal(q.node_offsets(), p.node_offsets())
+    assert np.array_equal(q.edge_offsets(), p.edge_offsets())
+
+
+def test_partition_errors(tmp_path):
+    rowptr = np.array([0, 3, 1003, 1005], np.uint64)
+    col = np.zeros(1005, np.uint32)
+    g = lt.Graph.from_csr(rowptr, col)
+    with pytest.raises(lt.LTError) as e:
+        lt.Partition.build(g, 1000)
+    assert e.value.status == 3  # LT_E_OVERSIZE_VERTEX
+    with pytest.raises(lt.LTError) as e:
+        lt.Partition.load(str(tmp_path / "nothing.bcsr"))
+    assert e.value.status == 1
+
+
+def test_scheduler_rules_match_oracle():
+    rng = np.random.default_rng(2)
+    for trial in range(300):
+        P = int(rng.integers(1, 40))
+        counts = rng.integers(0, 5, P) * rng.integers(0, 300000, P)
+        if trial % 5 == 0:
+            counts[:] = counts[0]  # ties
+        resident = (rng.random(P) < 0.5).astype(np.uint8)
+        prev = int(rng.integers(-1, P))
+        for sync in (0, 1):
+            assert lt.sched_greedy_next(counts, prev, sync) == oracle.greedy_next(counts, prev, sync)
+        assert lt.sched_evict_min(counts, resident, prev) == oracle.evict_min(counts, resident, prev)
+        page = int(rng.choice([1, 1000, 167936]))
+        assert (lt.sched_computed_partition(counts, resident, prev, page) ==
+                oracle.computed_partition(counts, resident, prev, page))
+
+
+def test_scheduler_rules_known_cases():
+    # greedy skips the previously computed partition unless synchronised
+    assert lt.sched_greedy_next([5, 9, 9, 1], 1, 0) == 2
+    assert lt.sched_greedy_next([5, 9, 9, 1], 1, 1) == 1
+    assert lt.sched_greedy_next([0, 0, 0], -1, 1) == -1
+    # eviction: fewest walkers among resident partitions except the last loaded
+    assert lt.sched_evict_min([7, 1, 3, 0], [1, 1, 1, 0], 1) == 2
+    # computed partition: a "full" queue (>= page) with the fewest walkers wins
+    assert lt.sched_computed_partition([10, 500, 300, 50], [1, 1, 1, 1], 0, 200) == 2
+    assert lt.sched_computed_partition([10, 100, 30, 50], [1, 1, 1, 1], 0, 200) == 1
+
+
+@pytest.mark.skipif(not _no_device(), reason="a HIP device is visible")
+def test_engine_without_device_fails_loudly():
+    g = lt.Graph.open(os.path.join(GOLDEN, "rmat_s8_ef4.bcsr"))
+    p = lt.Partition.build(g, 1 << 20)
+    with pytest.raises(lt.LTError) as e:
+        lt.Engine(g, p)
+    assert e.value.status == 7  # LT_E_NODEVICE
+
+
+@pytest.mark.skipif(not _no_device(), reason="a HIP device is visible")
+def test_cli_materialises_partitions_then_needs_a_device(tmp_path):
+    gpath = str(tmp_path / "g.bcsr")
+    open(gpath, "wb").write(open(os.path.join(GOLDEN, "rmat_s10_ef8.bcsr"), "rb").read())
+    r = subprocess.run([os.path.join(lt.BIN_DIR, "pagerank"), "-f", gpath, "-m", "-s", "1"], capture_output=True,
+                       text=True)
+    assert r.returncode == 1
+    assert "node with max degree:" in r.stdout and "num nodes:" in r.stdout
+    assert "no HIP device" in r.stderr
+    p = lt.Partition.load(gpath)
+    q = lt.Partition.build(lt.Graph.open(gpath), 1 << 20)
+    assert np.array_equal(p.node_offsets(), q.node_offsets())
+    r = subprocess.run([os.path.join(lt.BIN_DIR, "ppr"), "-f", str(tmp_path / "none.bcsr")], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "open graph" in r.stderr
