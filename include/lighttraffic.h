@@ -67,7 +67,10 @@ typedef struct {
     int32_t  span;                  /* lt_span                                    */
     int32_t  event_timers;          /* 1: per-phase HIP event timers              */
     int32_t  record_trace;          /* 1: keep the scheduler decision trace       */
-    int32_t  reserved[7];
+    int32_t  prefetch_depth;        /* LEARNED: partitions copied ahead (0 = off)  */
+    int32_t  prefetch_stick_pct;    /* LEARNED: walk a prefetched partition first
+                                       when its queue is >= this % of the largest */
+    int32_t  reserved[5];
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines

@@ -49,7 +49,8 @@ class Config(ctypes.Structure):
     """lt_config (include/lighttraffic.h)."""
     _fields_ = [("num_device_partitions", _u32), ("num_device_walkers", _u64), ("zerocopy_threshold", _u64),
                 ("seed", _u64), ("policy", _i32), ("pipeline", _i32), ("span", _i32), ("event_timers", _i32),
-                ("record_trace", _i32), ("reserved", _i32 * 7)]
+                ("record_trace", _i32), ("prefetch_depth", _i32), ("prefetch_stick_pct", _i32),
+                ("reserved", _i32 * 5)]
 
 
 class Stats(ctypes.Structure):

@@ -178,6 +178,7 @@ struct Engine {
     std::vector<uint8_t> trace_resident; // residency seen by each traced decision
     std::vector<int32_t> trace_recent;   // last kRecent loads at each traced decision
     Scorer *scorer = nullptr;             // learned policy (not owned)
+    std::vector<hipEvent_t> pool_events;  // reusable prefetch-completion events
     std::vector<float> scores;            // scorer output of the current decision
     bool scores_valid = false;
     uint64_t w_run = 0;                   // walkers of the current wave
@@ -216,6 +217,7 @@ struct Engine {
 Engine::~Engine() {
     if (s_compute) (void)hipStreamSynchronize(s_compute);
     if (s_copy) (void)hipStreamSynchronize(s_copy);
+    for (auto ev : pool_events) (void)hipEventDestroy(ev);
     dfree(d_node_off);
     dfree(d_edge_off);
     dfree(d_lut);
@@ -538,17 +540,19 @@ lt_status Engine::infer_scores(int32_t cur) {
     return LT_OK;
 }
 
-// Resident partition (other than a and b) with the lowest keep score; falls
-// back to the plain rule without valid scores.
+// Resident partition (other than a and b) with the fewest queued walkers, ties
+// broken by the lowest keep score; without valid scores, the plain rule.
 int32_t Engine::learned_victim(int32_t a, int32_t b) {
     if (scores_valid) {
         int32_t v = -1;
-        float best = 0.f;
+        uint64_t fewest = 0;
+        float keep = 0.f;
         for (uint32_t i = 0; i < P; i++) {
             if (!resident[i] || (int32_t)i == a || (int32_t)i == b) continue;
             const float k = scores[(size_t)i * kOut + 1];
-            if (v < 0 || k < best) {
-                best = k;
+            if (v < 0 || counts[i] < fewest || (counts[i] == fewest && k < keep)) {
+                fewest = counts[i];
+                keep = k;
                 v = (int32_t)i;
             }
         }
@@ -636,105 +640,160 @@ lt_status Engine::scheduler_loop() {
     return LT_OK;
 }
 
-// Learned scheduling: the partition to walk is still chosen by the plain
-// rule, but while it is walked the scorer's best guess for the next partition
-// that needs host data is copied into a free slot (prefetch), and every
-// eviction drops the resident partition with the lowest predicted future
-// traffic.  Evictions happen only while the compute stream is idle, and a
-// prefetched slot joins the resident table only after its copy completed.
+// Learned scheduling.  Decisions follow the plain scheduler (choice, zero-copy
+// rule, pipelined compute during a load), with two changes: while the chosen
+// partition is walked, the scorer's best guess for the next partition that
+// will need host data is copied into a slot ahead of time (prefetch), and a
+// victim is the resident partition with the fewest queued walkers, ties
+// broken by the lowest predicted future traffic (the plain rule breaks them by
+// partition id).  Evictions happen only while the compute stream is idle, and
+// a prefetched slot joins the resident table only after its copy completed.
 lt_status Engine::scheduler_loop_learned() {
-    int32_t prev = -1;
+    int32_t prev = -1, prev_loaded = -1;
     bool sync = false;
-    int32_t incoming = -1;
+    struct Inflight {
+        int32_t part;
+        hipEvent_t done;
+    };
+    std::vector<Inflight> inflight;         // prefetch copies in copy-stream order
     std::vector<uint8_t> prefetched(P, 0);  // resident because of a prefetch, not yet picked
+    std::vector<uint8_t> pending(P, 0);     // being copied by a prefetch
+    const uint32_t depth = (uint32_t)std::max(0, cfg.prefetch_depth);
     sched.begin_run();
-    auto finish_incoming = [&](bool wait) -> lt_status {
-        if (incoming < 0) return LT_OK;
-        if (!wait && hipStreamQuery(s_copy) != hipSuccess) return LT_OK;
-        LT_HIP(hipStreamSynchronize(s_copy));
-        resident[incoming] = 1;
-        LT_TRY(set_table((uint32_t)incoming, true));
-        incoming = -1;
+    // make completed prefetches resident, in order; with `upto`, wait for it
+    auto retire = [&](int32_t upto) -> lt_status {
+        while (!inflight.empty()) {
+            Inflight f = inflight.front();
+            const bool must = upto >= 0;
+            if (must) LT_HIP(hipEventSynchronize(f.done));
+            else if (hipEventQuery(f.done) != hipSuccess) break;
+            inflight.erase(inflight.begin());
+            pool_events.push_back(f.done);
+            pending[f.part] = 0;
+            resident[f.part] = 1;
+            LT_TRY(set_table((uint32_t)f.part, true));
+            if (f.part == upto) break;
+        }
         return LT_OK;
+    };
+    // reference pipelining: walk resident partitions while the copy engine works
+    auto pipeline_while_copying = [&](int32_t except) -> lt_status {
+        if (!cfg.pipeline) return LT_OK;
+        while (hipStreamQuery(s_copy) == hipErrorNotReady) {
+            LT_TRY(read_meta());
+            const int32_t p2 = sched.computed_choice(counts.data(), resident.data(), except, kRefPage);
+            if (p2 < 0) break;
+            st.computing_calls++;
+            record((uint32_t)p2, 2, -1, -1);
+            LT_TRY(process((uint32_t)p2, false, counts[p2]));
+            LT_HIP(hipStreamSynchronize(s_compute));
+        }
+        return read_meta();
     };
     while (true) {
         LT_TRY(read_meta());
-        LT_TRY(finish_incoming(false));
-        const int32_t c = sched.next(counts.data(), resident.data(), prev, sync);
+        LT_TRY(retire(-1));
+        int32_t c = sched.next(counts.data(), resident.data(), prev, sync);
         if (c < 0) {
             if (sync) break;
             sync = true;
             continue;
+        }
+        // Stick to a prefetched partition (copied or in flight) whose queue is
+        // close to the largest: its data is already on the way.
+        if (cfg.prefetch_stick_pct > 0 && !resident[c] && !pending[c]) {
+            int32_t s2 = -1;
+            for (uint32_t i = 0; i < P; i++)
+                if ((pending[i] || prefetched[i]) && counts[i] >= zc_threshold && (s2 < 0 || counts[i] > counts[s2]))
+                    s2 = (int32_t)i;
+            if (s2 >= 0 && counts[s2] * 100 >= counts[c] * (uint64_t)cfg.prefetch_stick_pct) c = s2;
         }
         sync = false;
         prev = c;
         st.iterations++;
         scores_valid = false;
         bool zc = false;
-        if (c == incoming) LT_TRY(finish_incoming(true));
-        if (resident[c]) {
-            if (prefetched[c]) st.prefetch_hits++;
-            prefetched[c] = 0;
-            record((uint32_t)c, 0, -1, -1);
-        } else if (counts[c] < zc_threshold) {
+        if (!resident[c] && !pending[c] && counts[c] < zc_threshold) {
             zc = true;
             st.zerocopy_calls++;
             record((uint32_t)c, 1, -1, -1);
         } else {
-            LT_TRY(finish_incoming(true));  // the copy engine is in-order anyway
-            int32_t evicted = -1;
-            if (free_slots.empty()) {
-                LT_TRY(infer_scores(c));
-                evicted = learned_victim(c, -1);
-                if (evicted < 0) {
-                    set_error("no partition to evict");
-                    return LT_E_INTERNAL;
-                }
-                LT_TRY(evict((uint32_t)evicted));
+            st.explicit_iterations++;
+            if (pending[c]) {
+                if (hipEventQuery(inflight.front().done) != hipSuccess && inflight.front().part == c)
+                    LT_TRY(pipeline_while_copying(c));
+                LT_TRY(retire(c));
             }
-            record((uint32_t)c, 3, c, evicted);
-            LT_TRY(load_async((uint32_t)c));
-            st.graph_loading_calls++;
-            sched.on_load((uint32_t)c);
-            LT_HIP(hipStreamSynchronize(s_copy));
-            resident[c] = 1;
-            LT_TRY(set_table((uint32_t)c, true));
+            if (resident[c]) {
+                if (prefetched[c]) st.prefetch_hits++;
+                prefetched[c] = 0;
+                record((uint32_t)c, 0, -1, -1);
+            } else {
+                LT_TRY(retire(inflight.empty() ? -1 : inflight.back().part));  // the copy engine is in-order
+                int32_t evicted = -1;
+                if (free_slots.empty()) {
+                    LT_TRY(infer_scores(c));
+                    evicted = learned_victim(c, prev_loaded);
+                    if (evicted < 0) {
+                        set_error("no partition to evict");
+                        return LT_E_INTERNAL;
+                    }
+                    LT_TRY(evict((uint32_t)evicted));
+                    prefetched[evicted] = 0;
+                }
+                record((uint32_t)c, 3, c, evicted);
+                LT_TRY(load_async((uint32_t)c));
+                st.graph_loading_calls++;
+                sched.on_load((uint32_t)c);
+                LT_TRY(pipeline_while_copying(c));
+                LT_HIP(hipStreamSynchronize(s_copy));
+                resident[c] = 1;
+                LT_TRY(set_table((uint32_t)c, true));
+            }
+            prev_loaded = c;
         }
-        if (!zc) st.explicit_iterations++;
-        // prefetch the predicted next load while c is walked
-        if (incoming < 0 && scorer) {
+        // prefetch the predicted next loads while c is walked (compute is idle here)
+        if (scorer && inflight.size() < depth) {
             if (!scores_valid) LT_TRY(infer_scores(c));
-            int32_t q = -1;
-            if (scores_valid) {
+            while (scores_valid && inflight.size() < depth) {
+                int32_t q = -1;
                 float best = 0.f;
                 for (uint32_t i = 0; i < P; i++) {
-                    if (resident[i] || (int32_t)i == c || counts[i] < zc_threshold) continue;
+                    if (resident[i] || pending[i] || (int32_t)i == c || counts[i] < zc_threshold) continue;
                     const float v = scores[(size_t)i * kOut];
                     if (q < 0 || v > best) {
                         best = v;
                         q = (int32_t)i;
                     }
                 }
-            }
-            if (q >= 0) {
+                if (q < 0) break;
                 if (free_slots.empty()) {
                     const int32_t v = learned_victim(c, q);
-                    if (v >= 0) LT_TRY(evict((uint32_t)v));
+                    if (v < 0) break;
+                    LT_TRY(evict((uint32_t)v));
+                    prefetched[v] = 0;
                 }
-                if (!free_slots.empty()) {
-                    LT_TRY(load_async((uint32_t)q));
-                    st.graph_loading_calls++;
-                    sched.on_load((uint32_t)q);
-                    st.prefetch_issued++;
-                    prefetched[q] = 1;
-                    incoming = q;
+                LT_TRY(load_async((uint32_t)q));
+                hipEvent_t ev;
+                if (!pool_events.empty()) {
+                    ev = pool_events.back();
+                    pool_events.pop_back();
+                } else {
+                    LT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
                 }
+                LT_HIP(hipEventRecord(ev, s_copy));
+                inflight.push_back({q, ev});
+                pending[q] = 1;
+                st.graph_loading_calls++;
+                sched.on_load((uint32_t)q);
+                st.prefetch_issued++;
+                prefetched[q] = 1;
             }
         }
         st.computing_calls++;
         LT_TRY(process((uint32_t)c, zc, counts[c]));
     }
-    LT_TRY(finish_incoming(true));
+    LT_TRY(retire(inflight.empty() ? -1 : inflight.back().part));
     LT_HIP(hipDeviceSynchronize());
     return LT_OK;
 }

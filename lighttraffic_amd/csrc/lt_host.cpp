@@ -113,6 +113,8 @@ void lt_config_default(lt_config *c) {
     c->span = LT_SPAN_PARTITION;
     c->event_timers = 0;
     c->record_trace = 0;
+    c->prefetch_depth = 2;
+    c->prefetch_stick_pct = 50;
 }
 
 // ---- bcsr (src/GraphLoader.h:30-48; tools/convert_from_txt.cc:211-218) ----

@@ -108,22 +108,28 @@ __global__ __launch_bounds__(256) void feat_kernel(uint32_t P, const uint32_t *c
 // EPI 0: store; 1: relu(acc + bias[n]); 2: acc * (mask[m][n] > 0); 3: acc + bias[n].
 enum { EPI_STORE = 0, EPI_BIAS_RELU = 1, EPI_RELU_GRAD = 2, EPI_BIAS = 3 };
 
+// With gridDim.z > 1 the K range is split: slice z covers [z*kchunk, (z+1)*kchunk)
+// and stores its partial product at C + z*M*ldc (summed by splitk_reduce_kernel).
 template <bool TA, bool TB, typename BT, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const float *A, int lda, const BT *B, int ldb,
-                                                   float *C, int ldc, const BT *bias, const float *mask, int ldm) {
+                                                   float *C, int ldc, const BT *bias, const float *mask, int ldm,
+                                                   int kchunk) {
     constexpr int TM = 64, TN = 64, TK = 16;
     __shared__ float As[TK][TM + 4];
     __shared__ float Bs[TK][TN + 4];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kb = blockIdx.z * kchunk;
+    const int ke = min(K, kb + kchunk);
+    C += (size_t)blockIdx.z * M * ldc;
     float acc[4][4] = {};
-    for (int k0 = 0; k0 < K; k0 += TK) {
+    for (int k0 = kb; k0 < ke; k0 += TK) {
         for (int i = threadIdx.x; i < TM * TK; i += 256) {
             int mm, kk;
             if (TA) { mm = i % TM; kk = i / TM; } else { kk = i % TK; mm = i / TK; }
             const int m = m0 + mm, k = k0 + kk;
             float v = 0.f;
-            if (m < M && k < K) v = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+            if (m < M && k < ke) v = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
             As[kk][mm] = v;
         }
         for (int i = threadIdx.x; i < TN * TK; i += 256) {
@@ -131,7 +137,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const fl
             if (TB) { kk = i % TK; nn = i / TK; } else { nn = i % TN; kk = i / TN; }
             const int n = n0 + nn, k = k0 + kk;
             float v = 0.f;
-            if (n < N && k < K) v = to_f32(TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n]);
+            if (n < N && k < ke) v = to_f32(TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n]);
             Bs[kk][nn] = v;
         }
         __syncthreads();
@@ -169,8 +175,30 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const fl
 template <bool TA, bool TB, typename BT, int EPI>
 static void gemm(hipStream_t s, int M, int N, int K, const float *A, int lda, const BT *B, int ldb, float *C, int ldc,
                  const BT *bias = nullptr, const float *mask = nullptr, int ldm = 0) {
-    dim3 grid((N + 63) / 64, (M + 63) / 64);
-    gemm_kernel<TA, TB, BT, EPI><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, bias, mask, ldm);
+    dim3 grid((N + 63) / 64, (M + 63) / 64, 1);
+    gemm_kernel<TA, TB, BT, EPI><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, bias, mask, ldm, K);
+}
+
+// out[i] = sum_z part[z*n + i], fixed order (deterministic).
+__global__ void splitk_reduce_kernel(size_t n, int splits, const float *part, float *out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < splits; z++) s += part[(size_t)z * n + i];
+    out[i] = s;
+}
+
+// Weight gradient C[M x N] = A^T B with a long K (rows of the batch): K is cut
+// into slices of kSplitK, each slice writes a partial product, then a fixed-order sum.
+constexpr int kSplitK = 1024;
+static void gemm_wgrad(hipStream_t s, int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
+                       float *partial) {
+    const int splits = (K + kSplitK - 1) / kSplitK;
+    dim3 grid((N + 63) / 64, (M + 63) / 64, splits);
+    gemm_kernel<true, false, float, EPI_STORE><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, partial, N, nullptr,
+                                                                   nullptr, 0, kSplitK);
+    const size_t n = (size_t)M * N;
+    splitk_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, splits, partial, C);
 }
 
 // ---- losses -----------------------------------------------------------------
@@ -234,13 +262,21 @@ __global__ __launch_bounds__(256) void loss_kernel(uint32_t P, uint32_t B, const
     }
 }
 
-// db[n] = sum_m dZ[m][n]
-__global__ void colsum_kernel(int M, int N, const float *dZ, float *db) {
+// db[n] = sum_m dZ[m][n]: slice z of the rows sums into part[z*N + n]
+// (reduced afterwards in fixed order).
+__global__ void colsum_kernel(int M, int N, const float *dZ, float *part, int rows_per_slice) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    const int m0 = blockIdx.y * rows_per_slice, m1 = min(M, m0 + rows_per_slice);
     float s = 0.f;
-    for (int m = 0; m < M; m++) s += dZ[(size_t)m * N + n];
-    db[n] = s;
+    for (int m = m0; m < m1; m++) s += dZ[(size_t)m * N + n];
+    part[(size_t)blockIdx.y * N + n] = s;
+}
+
+static void colsum(hipStream_t s, int M, int N, const float *dZ, float *out, float *partial) {
+    const int splits = (M + kSplitK - 1) / kSplitK;
+    colsum_kernel<<<dim3((N + 255) / 256, splits), 256, 0, s>>>(M, N, dZ, partial, kSplitK);
+    splitk_reduce_kernel<<<(N + 255) / 256, 256, 0, s>>>((size_t)N, splits, partial, out);
 }
 
 __global__ void adam_kernel(size_t n, float *W, uint16_t *Wb, const float *G, float *Mo, float *Vo, float lr,
@@ -263,16 +299,60 @@ __global__ void round_kernel(size_t n, const float *W, uint16_t *Wb) {
 }
 
 // ---- fused inference: one decision, 8 partitions per workgroup --------------
-__global__ __launch_bounds__(256) void infer_kernel(uint32_t P, const float *X, const uint16_t *Wb, float *Y) {
+// The decision state arrives packed: u32 counts[P] | u8 resident[P] | pad to 8 |
+// i32 recent[8] | i32 cur | f32 w_total.  Every workgroup recomputes the max
+// and sum of the counts (P is small), builds the features of its 8 rows and
+// runs the three layers with the BF16 weights.
+__global__ __launch_bounds__(256) void infer_kernel(uint32_t P, const uint8_t *state, const float *vsh,
+                                                    const float *esh, const uint16_t *Wb, float *Y) {
     constexpr int R = 8;
     __shared__ float s_x[R][kFeat];
     __shared__ float s_h1[R][kHidden];
     __shared__ float s_h2[R][kHidden];
+    __shared__ float s_max[256], s_sum[256];
+    const uint32_t *counts = (const uint32_t *)state;
+    const uint8_t *res = state + 4 * (size_t)P;
+    const int32_t *meta = (const int32_t *)(state + (((size_t)P * 5 + 7) & ~(size_t)7));
     const uint32_t r0 = blockIdx.x * R;
     const int j = threadIdx.x;
-    for (int i = j; i < R * kFeat; i += 256) {
-        const uint32_t row = r0 + i / kFeat;
-        s_x[i / kFeat][i % kFeat] = row < P ? X[(size_t)row * kFeat + i % kFeat] : 0.f;
+    float mx = 0.f, sm = 0.f;
+    for (uint32_t q = j; q < P; q += 256) {
+        const float v = (float)counts[q];
+        mx = fmaxf(mx, v);
+        sm += v;
+    }
+    s_max[j] = mx;
+    s_sum[j] = sm;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (j < off) {
+            s_max[j] = fmaxf(s_max[j], s_max[j + off]);
+            s_sum[j] += s_sum[j + off];
+        }
+        __syncthreads();
+    }
+    const float cmax = fmaxf(s_max[0], 1.f), csum = s_sum[0];
+    const float W = fmaxf(__int_as_float(meta[kRecent + 1]), 1.f);
+    const float mean = fmaxf(csum / (float)P, 1.f);
+    const float lw = log1pf(W);
+    const int32_t cur = meta[kRecent];
+    if (j < R) {
+        const uint32_t q = r0 + j;
+        float *x = s_x[j];
+        if (q < P) {
+            const float v = (float)counts[q];
+            x[0] = log1pf(v) / lw;
+            x[1] = v / cmax;
+            x[2] = log1pf(v / mean);
+            x[3] = res[q] ? 1.f : 0.f;
+            x[4] = cur == (int32_t)q ? 1.f : 0.f;
+            for (int k = 0; k < kRecent; k++) x[5 + k] = meta[k] == (int32_t)q ? 1.f : 0.f;
+            x[13] = vsh[q];
+            x[14] = esh[q];
+            x[15] = csum / W;
+        } else {
+            for (int k = 0; k < kFeat; k++) x[k] = 0.f;
+        }
     }
     __syncthreads();
     const uint16_t *W1 = Wb + kOffW1, *B1 = Wb + kOffB1, *W2 = Wb + kOffW2, *B2 = Wb + kOffB2;
@@ -298,7 +378,6 @@ __global__ __launch_bounds__(256) void infer_kernel(uint32_t P, const float *X, 
 #pragma unroll
     for (int r = 0; r < R; r++) s_h2[r][j] = fmaxf(a[r] + bf16_to_f32(B2[j]), 0.f);
     __syncthreads();
-    // output layer: 16 rows x 2 outputs = 16 dot products of length 256, one per wave quarter
     const int lane = j & 63, wave = j >> 6;
     for (int t = wave; t < R * kOut; t += 4) {
         const int r = t / kOut, o = t % kOut;
@@ -337,12 +416,11 @@ struct Scorer {
     int32_t *d_recent = nullptr, *d_cur = nullptr, *d_next = nullptr;
     float *d_wtot = nullptr, *d_keep = nullptr, *d_wa = nullptr, *d_wb = nullptr;
     float *X = nullptr, *A1 = nullptr, *A2 = nullptr, *Y = nullptr, *dY = nullptr, *dA1 = nullptr, *dA2 = nullptr;
+    float *partial = nullptr;  // split-K partial products
     float *d_loss = nullptr;
     // inference buffers
-    uint32_t *i_counts = nullptr;
-    uint8_t *i_res = nullptr;
-    int32_t *i_recent = nullptr, *i_cur = nullptr;
-    float *i_wtot = nullptr, *i_X = nullptr, *i_Y = nullptr;
+    uint8_t *i_state = nullptr;  // packed decision state (see infer_kernel)
+    float *i_Y = nullptr;
     uint8_t *h_stage = nullptr;  // pinned staging for one decision's state
     float *h_Y = nullptr;
     Samples data;
@@ -356,7 +434,7 @@ struct Scorer {
 Scorer::~Scorer() {
     if (stream) (void)hipStreamSynchronize(stream);
     void *ptrs[] = {d_vsh, d_esh, W, G, Mo, Vo, Wb, d_counts, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa,
-                    d_wb, X, A1, A2, Y, dY, dA1, dA2, d_loss, i_counts, i_res, i_recent, i_cur, i_wtot, i_X, i_Y};
+                    d_wb, X, A1, A2, Y, dY, dA1, dA2, partial, d_loss, i_state, i_Y};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h_stage) (void)hipHostFree(h_stage);
@@ -397,10 +475,7 @@ lt_status Scorer::init(uint32_t P_, const uint32_t *node_off, const uint64_t *ed
     MLP_HIP(hipMemset(Vo, 0, 4 * kParams));
     round_kernel<<<(kParams + 255) / 256, 256, 0, stream>>>(kParams, W, Wb);
     MLP_HIP(hipGetLastError());
-    if ((st = dalloc(&i_counts, P)) || (st = dalloc(&i_res, P)) || (st = dalloc(&i_recent, kRecent)) ||
-        (st = dalloc(&i_cur, 1)) || (st = dalloc(&i_wtot, 1)) || (st = dalloc(&i_X, (size_t)P * kFeat)) ||
-        (st = dalloc(&i_Y, (size_t)P * kOut)))
-        return st;
+    if ((st = dalloc(&i_state, (size_t)P * 5 + 64)) || (st = dalloc(&i_Y, (size_t)P * kOut))) return st;
     MLP_HIP(hipHostMalloc((void **)&h_stage, (size_t)P * 5 + 64, hipHostMallocDefault));
     MLP_HIP(hipHostMalloc((void **)&h_Y, sizeof(float) * P * kOut, hipHostMallocDefault));
     MLP_HIP(hipStreamSynchronize(stream));
@@ -409,7 +484,8 @@ lt_status Scorer::init(uint32_t P_, const uint32_t *node_off, const uint64_t *ed
 
 lt_status Scorer::ensure_cap(uint32_t n) {
     if (n <= cap) return LT_OK;
-    void *ptrs[] = {d_counts, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa, d_wb, X, A1, A2, Y, dY, dA1, dA2};
+    void *ptrs[] = {d_counts, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa, d_wb, X, A1, A2, Y, dY, dA1, dA2,
+                    partial};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     const size_t rows = (size_t)n * P;
@@ -419,7 +495,8 @@ lt_status Scorer::ensure_cap(uint32_t n) {
         (st = dalloc(&d_keep, rows)) || (st = dalloc(&d_wa, n)) || (st = dalloc(&d_wb, n)) ||
         (st = dalloc(&X, rows * kFeat)) || (st = dalloc(&A1, rows * kHidden)) || (st = dalloc(&A2, rows * kHidden)) ||
         (st = dalloc(&Y, rows * kOut)) || (st = dalloc(&dY, rows * kOut)) || (st = dalloc(&dA1, rows * kHidden)) ||
-        (st = dalloc(&dA2, rows * kHidden)))
+        (st = dalloc(&dA2, rows * kHidden)) ||
+        (st = dalloc(&partial, ((rows + kSplitK - 1) / kSplitK) * (size_t)kHidden * kHidden)))
         return st;
     cap = n;
     return LT_OK;
@@ -439,14 +516,14 @@ lt_status Scorer::forward_backward(uint32_t nb, bool update, float lr) {
     gemm<false, false, uint16_t, EPI_BIAS>(s, M, kOut, H, A2, H, Wb + kOffW3, kOut, Y, kOut, Wb + kOffB3);
     loss_kernel<<<nb, 256, 0, s>>>(P, nb, Y, d_counts, d_next, d_keep, d_wa, d_wb, dY, d_loss);
     // backward
-    gemm<true, false, float, EPI_STORE>(s, H, kOut, M, A2, H, dY, kOut, G + kOffW3, kOut);
-    colsum_kernel<<<1, 64, 0, s>>>(M, kOut, dY, G + kOffB3);
+    gemm_wgrad(s, H, kOut, M, A2, H, dY, kOut, G + kOffW3, partial);
+    colsum(s, M, kOut, dY, G + kOffB3, partial);
     gemm<false, true, uint16_t, EPI_RELU_GRAD>(s, M, H, kOut, dY, kOut, Wb + kOffW3, kOut, dA2, H, nullptr, A2, H);
-    gemm<true, false, float, EPI_STORE>(s, H, H, M, A1, H, dA2, H, G + kOffW2, H);
-    colsum_kernel<<<(H + 255) / 256, 256, 0, s>>>(M, H, dA2, G + kOffB2);
+    gemm_wgrad(s, H, H, M, A1, H, dA2, H, G + kOffW2, partial);
+    colsum(s, M, H, dA2, G + kOffB2, partial);
     gemm<false, true, uint16_t, EPI_RELU_GRAD>(s, M, H, H, dA2, H, Wb + kOffW2, H, dA1, H, nullptr, A1, H);
-    gemm<true, false, float, EPI_STORE>(s, kFeat, H, M, X, kFeat, dA1, H, G + kOffW1, H);
-    colsum_kernel<<<(H + 255) / 256, 256, 0, s>>>(M, H, dA1, G + kOffB1);
+    gemm_wgrad(s, kFeat, H, M, X, kFeat, dA1, H, G + kOffW1, partial);
+    colsum(s, M, H, dA1, G + kOffB1, partial);
     MLP_HIP(hipGetLastError());
     if (update) {
         adam_t++;
@@ -469,27 +546,23 @@ Samples &scorer_samples(Scorer *s) { return s->data; }
 uint32_t scorer_partitions(const Scorer *s) { return s->P; }
 bool scorer_trained(const Scorer *s) { return s && s->trained; }
 
-// Stage one decision's state (counts, residency, recent loads, choice) into
-// pinned memory, copy it over, build features and run the fused MLP.
+// Stage one decision's state into pinned memory in the packed layout of
+// infer_kernel, copy it over in one transfer, run the fused MLP, read back.
 lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float *out) {
     hipStream_t st = stream_v ? (hipStream_t)stream_v : s->stream;
     const uint32_t P = s->P;
     uint8_t *h = s->h_stage;
     memcpy(h, d.counts, 4 * (size_t)P);
     memcpy(h + 4 * (size_t)P, d.resident, P);
-    int32_t *meta = (int32_t *)(h + (((size_t)P * 5 + 7) & ~(size_t)7));
+    const size_t moff = ((size_t)P * 5 + 7) & ~(size_t)7;
+    int32_t *meta = (int32_t *)(h + moff);
     for (int k = 0; k < kRecent; k++) meta[k] = d.recent[k];
     meta[kRecent] = d.cur;
     float wt = d.w_total;
     memcpy(&meta[kRecent + 1], &wt, 4);
-    MLP_HIP(hipMemcpyAsync(s->i_counts, h, 4 * (size_t)P, hipMemcpyHostToDevice, st));
-    MLP_HIP(hipMemcpyAsync(s->i_res, h + 4 * (size_t)P, P, hipMemcpyHostToDevice, st));
-    MLP_HIP(hipMemcpyAsync(s->i_recent, meta, 4 * kRecent, hipMemcpyHostToDevice, st));
-    MLP_HIP(hipMemcpyAsync(s->i_cur, &meta[kRecent], 4, hipMemcpyHostToDevice, st));
-    MLP_HIP(hipMemcpyAsync(s->i_wtot, &meta[kRecent + 1], 4, hipMemcpyHostToDevice, st));
-    feat_kernel<<<1, 256, 0, st>>>(P, s->i_counts, s->i_res, s->i_recent, s->i_cur, s->i_wtot, s->d_vsh, s->d_esh,
-                                   s->i_X);
-    infer_kernel<<<(P + 7) / 8, 256, 0, st>>>(P, s->i_X, s->Wb, s->i_Y);
+    const size_t bytes = moff + 4 * (kRecent + 2);
+    MLP_HIP(hipMemcpyAsync(s->i_state, h, bytes, hipMemcpyHostToDevice, st));
+    infer_kernel<<<(P + 7) / 8, 256, 0, st>>>(P, s->i_state, s->d_vsh, s->d_esh, s->Wb, s->i_Y);
     MLP_HIP(hipGetLastError());
     MLP_HIP(hipMemcpyAsync(s->h_Y, s->i_Y, sizeof(float) * P * kOut, hipMemcpyDeviceToHost, st));
     MLP_HIP(hipStreamSynchronize(st));
