@@ -8,15 +8,18 @@ results back to the host are outside it, as in the reference.
 
 Workloads (R-MAT a,b,c,d = .57/.19/.19/.05, seeded, deduped, symmetrised,
 self loops and isolated vertices removed):
+  c5  (default) scale 26 ef 27, 32 MB partitions, 8 GB partition cache
+      (256 slots), pagerank w=2|V| l=10: plain scheduler vs learned scheduler
   c1  scale 16 ef 16, one 32 MB partition, pagerank w=2|V| l=10
   c2  scale 22 ef 16, 32 MB partitions all HBM-resident, genericwalk w=2|V| l=10
-  c3  scale 25 ef 44, 4 GB partition cache, ppr from vertex 0 w=2|V| p=.15
-  c5  scale 26 ef 27, 8 GB partition cache, pagerank w=2|V| l=10
+  c3  scale 25 ef 44, 4 GB partition cache, 2^26 walkers at once, ppr from vertex 0
+For c5 the learned scorer is trained here, on the GPU, from the traces of the
+plain scheduler's warm-up runs; `value` is the learned scheduler's steps/s and
+the plain scheduler's figures are reported beside it.
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -54,6 +57,44 @@ def run_job(eng, wl, w):
     else:
         eng.ppr(w, source=0, prob=0.15)
     return eng.stats()
+
+
+def measure(eng, wl, w, warmup, steps, label, timed_engine=None):
+    """Warm-up jobs, then `steps` timed jobs on an engine without event
+    timers; the walk-kernel time for the roofline comes from one more job on
+    `timed_engine` (same configuration, HIP event timers on), so the timers'
+    cost stays out of the steps/s figures."""
+    for i in range(warmup):
+        st = run_job(eng, wl, w)
+        log(f"{label} warmup {i}: {st['running_ms']:.1f} ms")
+    stats = []
+    for i in range(steps):
+        stats.append(run_job(eng, wl, w))
+        s = stats[-1]
+        log(f"{label} step {i}: {s['running_ms']:.1f} ms, loads {s['graph_loading_calls']}, "
+            f"H2D {(s['h2d_bytes_explicit'] + s['h2d_bytes_zerocopy']) / 1e9:.2f} GB")
+    ms = float(np.mean([s["running_ms"] for s in stats]))
+    nominal = stats[0]["nominal_steps"]
+    keys = ("moves", "visits", "residencies", "kernel_launches", "h2d_bytes_explicit", "h2d_bytes_zerocopy",
+            "graph_loading_calls", "zerocopy_calls", "iterations", "prefetch_issued", "prefetch_hits")
+    tot = {k: sum(s[k] for s in stats) for k in keys}
+    h2d = (tot["h2d_bytes_explicit"] + tot["h2d_bytes_zerocopy"]) / len(stats)
+    res = {"ms_per_step": ms, "steps_per_s": nominal / (ms * 1e-3), "h2d_bytes_per_step": h2d / nominal,
+           "h2d_GB_per_job": h2d / 1e9, "pcie_GBps": h2d / (ms * 1e-3) / 1e9,
+           "per_job": {k: v / len(stats) for k, v in tot.items()}, "nominal": nominal}
+    if timed_engine is not None:
+        tst = run_job(timed_engine, wl, w)
+        res.update({"kernel_ms": tst["walk_kernel_ms"], "launches": tst["kernel_launches"],
+                    "alg_bytes": algorithmic_bytes(tst), "timed_job_ms": tst["running_ms"]})
+    return res
+
+
+def roofline(m):
+    achieved = m["alg_bytes"] / (m["kernel_ms"] * 1e-3) / 1e9 if m["kernel_ms"] > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "walk_kernel",
+            "launches": m["launches"], "avg_launch_us": m["kernel_ms"] * 1e3 / max(m["launches"], 1),
+            "algorithmic_bytes_per_launch": m["alg_bytes"] / max(m["launches"], 1)}
 
 
 def cpu_baseline(g, wl, w, budget_s=20.0):
@@ -97,10 +138,12 @@ def _cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--span", default="partition", choices=["partition", "resident"])
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
+    ap.add_argument("--span", default=None, choices=["partition", "resident"],
+                    help="single-policy workloads only (default partition)")
+    ap.add_argument("--epochs", type=int, default=20, help="scorer training epochs (c5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -117,52 +160,80 @@ def main():
     log(f"graph s{wl['scale']} ef{wl['ef']}: V={g.num_node} E={g.num_edge} in {time.perf_counter() - t0:.1f} s")
     part = lt.Partition.build(g, wl["mb"] << 20)
     w = 2 * g.num_node
-    cfg = dict(num_device_partitions=wl["slots"], event_timers=1, span=args.span)
+    base = dict(num_device_partitions=wl["slots"])
     if "wdev" in wl:
-        cfg["num_device_walkers"] = wl["wdev"]
-    eng = lt.Engine(g, part, **cfg)
-    for i in range(args.warmup):
-        st = run_job(eng, wl, w)
-        log(f"warmup {i}: {st['running_ms']:.2f} ms")
-    stats = []
-    for i in range(args.steps):
-        stats.append(run_job(eng, wl, w))
-        log(f"step {i}: {stats[-1]['running_ms']:.2f} ms, launches {stats[-1]['kernel_launches']}")
-    ms = float(np.mean([s["running_ms"] for s in stats]))
-    nominal = stats[0]["nominal_steps"]
-    tot = {k: sum(s[k] for s in stats) for k in ("moves", "visits", "residencies", "kernel_launches",
-                                                  "h2d_bytes_explicit", "h2d_bytes_zerocopy")}
-    kernel_ms = sum(s["walk_kernel_ms"] for s in stats)
-    launches = tot["kernel_launches"]
-    alg_bytes = algorithmic_bytes(tot)
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
-    out = {
-        "metric": METRIC,
-        "value": nominal / (ms * 1e-3),
-        "unit": "walk steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic R-MAT (a,b,c,d=.57/.19/.19/.05), generated on the GPU",
-        "config": {"workload": f"{args.workload}: R-MAT scale {wl['scale']} ef {wl['ef']}, {wl['app']} "
-                               f"w=2|V|={w}" + (f" l={wl['l']}" if wl['l'] else " p=0.15") +
-                               f", {wl['mb']} MB partitions, cache "
-                               + ("all" if not wl["slots"] else f"{wl['slots']} slots"),
-                   "V": g.num_node, "E": g.num_edge, "partitions": part.count, "scheduler": "greedy",
-                   "span": args.span, "device": info["name"]},
-        "h2d_bytes_per_step": (tot["h2d_bytes_explicit"] + tot["h2d_bytes_zerocopy"]) / (nominal * len(stats)),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "walk_kernel", "launches": launches,
-                     "avg_launch_us": kernel_ms * 1e3 / max(launches, 1),
-                     "algorithmic_bytes_per_launch": alg_bytes / max(launches, 1)},
-        "stats_per_step": {k: v / len(stats) for k, v in tot.items()},
-    }
+        base["num_device_walkers"] = wl["wdev"]
+    desc = (f"{args.workload}: R-MAT scale {wl['scale']} ef {wl['ef']} (V={g.num_node}, E={g.num_edge}), "
+            f"{wl['app']} w=2|V|={w}" + (f" l={wl['l']}" if wl['l'] else " p=0.15") +
+            f", {wl['mb']} MB partitions (P={part.count}), cache " +
+            ("all partitions" if not wl["slots"] else f"{wl['slots']} slots = {wl['slots'] * wl['mb'] // 1024} GB"))
+    out = {"metric": METRIC, "unit": "walk steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+           "data": "synthetic R-MAT (a,b,c,d=.57/.19/.19/.05), generated on the GPU, Philox walks"}
+
+    if args.workload == "c5":
+        plain = {}
+        engines = {}
+        for span in ("partition", "resident"):
+            eng = lt.Engine(g, part, span=span, record_trace=1, **base)
+            plain[span] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}")
+            engines[span] = eng
+        t1 = time.perf_counter()
+        sc = lt.Scorer(part, seed=1)
+        n_next = sc.add_trace(engines["partition"], use_next=True, use_keep=True)
+        n_res = sc.add_trace(engines["resident"], use_next=True, use_keep=False)
+        loss = sc.fit(epochs=args.epochs, batch=32, lr=1e-3)
+        train_s = time.perf_counter() - t1
+        log(f"scorer trained on {n_next + n_res} decisions in {train_s:.1f} s, losses {loss.tolist()}")
+        ref = engines["resident"].pagerank(w, wl["l"])
+        for e in engines.values():
+            e.close()
+        eng = lt.Engine(g, part, span="resident", policy="learned", **base)
+        eng.set_scorer(sc)
+        teng = lt.Engine(g, part, span="resident", policy="learned", event_timers=1, **base)
+        teng.set_scorer(sc)
+        run_job(teng, wl, w)
+        learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident", timed_engine=teng)
+        teng.close()
+        same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
+        best_plain = min(plain.values(), key=lambda m: m["ms_per_step"])
+        ref_plain = plain["partition"]
+        out.update({
+            "value": learned["steps_per_s"],
+            "ms_per_step": learned["ms_per_step"],
+            "config": {"workload": desc, "scheduler": "learned (prefetch depth 2, span resident)",
+                       "device": info["name"]},
+            "h2d_bytes_per_step": learned["h2d_bytes_per_step"],
+            "plain": {"span_partition": {k: ref_plain[k] for k in ("steps_per_s", "ms_per_step",
+                                                                    "h2d_bytes_per_step", "h2d_GB_per_job",
+                                                                    "pcie_GBps")},
+                      "span_resident": {k: plain["resident"][k] for k in ("steps_per_s", "ms_per_step",
+                                                                         "h2d_bytes_per_step", "h2d_GB_per_job",
+                                                                         "pcie_GBps")}},
+            "learned": {k: learned[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
+                                                "h2d_GB_per_job", "pcie_GBps")},
+            "learned_vs_plain": {
+                "speedup_vs_reference_plain": learned["steps_per_s"] / ref_plain["steps_per_s"],
+                "h2d_bytes_ratio_vs_reference_plain": learned["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"],
+                "speedup_vs_best_plain": learned["steps_per_s"] / best_plain["steps_per_s"],
+                "h2d_bytes_ratio_vs_best_plain": learned["h2d_bytes_per_step"] / best_plain["h2d_bytes_per_step"],
+                "visit_counts_identical": same,
+                "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
+                "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],
+                "scorer_training_s": train_s, "training_decisions": n_next + n_res},
+            "roofline": roofline(learned),
+            "stats_per_step": learned["per_job"],
+        })
+    else:
+        eng = lt.Engine(g, part, span=args.span or "partition", **base)
+        teng = lt.Engine(g, part, span=args.span or "partition", event_timers=1, **base)
+        run_job(teng, wl, w)
+        m = measure(eng, wl, w, args.warmup, args.steps, args.workload, timed_engine=teng)
+        out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
+                    "config": {"workload": desc, "scheduler": "greedy", "span": args.span or "partition",
+                               "device": info["name"]},
+                    "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m),
+                    "stats_per_step": m["per_job"]})
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, wl, w)
     print(json.dumps(out), flush=True)
