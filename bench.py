@@ -13,6 +13,9 @@ self loops and isolated vertices removed):
   c1  scale 16 ef 16, one 32 MB partition, pagerank w=2|V| l=10
   c2  scale 22 ef 16, 32 MB partitions all HBM-resident, genericwalk w=2|V| l=10
   c3  scale 25 ef 44, 4 GB partition cache, 2^26 walkers at once, ppr from vertex 0
+  c4  same graph, 4,096 PPR queries (sources proportional to degree, walk counts
+      log-uniform in [1e3, 1e6]), 8 GB cache; configuration chosen by the
+      autotuner (partition size 16/32/64 MB, span, host or device scheduling)
 For c5 the learned scorer is trained here, on the GPU, from the traces of the
 plain scheduler's warm-up runs; `value` is the learned scheduler's steps/s and
 the plain scheduler's figures are reported beside it.
@@ -35,6 +38,7 @@ WORKLOADS = {
     "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10),
     "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10),
     "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26),
+    "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
     "c5": dict(scale=26, ef=27, mb=32, app="pagerank", slots=256, l=10),
 }
 
@@ -49,8 +53,14 @@ def algorithmic_bytes(st):
     return 20 * st["moves"] + 8 * st["visits"] + 32 * st["residencies"]
 
 
+QUERIES = {}
+
+
 def run_job(eng, wl, w):
-    if wl["app"] == "pagerank":
+    if wl["app"] == "ppr_batch":
+        src, cnt = QUERIES["table"]
+        eng.ppr_batch(src, cnt, prob=0.15, endpoints=False)
+    elif wl["app"] == "pagerank":
         eng.pagerank(w, wl["l"])
     elif wl["app"] == "genericwalk":
         eng.genericwalk(w, wl["l"], endpoints=True)
@@ -89,10 +99,25 @@ def measure(eng, wl, w, warmup, steps, label, timed_engine=None):
     return res
 
 
-def roofline(m):
+def measured_traffic(workload, span, alg_bytes_per_launch):
+    """HBM bytes per walk-kernel launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
+    kernel of known size), as the ratio traffic / algorithmic bytes measured
+    there times this run's algorithmic bytes per launch; None if not profiled."""
+    path = os.path.join(ROOT, "profiles", f"walk_kernel_{workload}_{span}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d["traffic_per_algorithmic_byte"] * alg_bytes_per_launch
+
+
+def roofline(m, workload=None, span=None):
     achieved = m["alg_bytes"] / (m["kernel_ms"] * 1e-3) / 1e9 if m["kernel_ms"] > 0 else 0.0
+    per_launch = m["alg_bytes"] / max(m["launches"], 1)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "walk_kernel",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, span, per_launch),
+            "kernel": "walk_kernel",
             "launches": m["launches"], "avg_launch_us": m["kernel_ms"] * 1e3 / max(m["launches"], 1),
             "algorithmic_bytes_per_launch": m["alg_bytes"] / max(m["launches"], 1)}
 
@@ -106,7 +131,13 @@ def cpu_baseline(g, wl, w, budget_s=20.0):
     n = min(w, 200000)
     while True:
         t0 = time.perf_counter()
-        if wl["app"] == "pagerank":
+        if wl["app"] == "ppr_batch":
+            src, cnt = QUERIES["table"]
+            k = int(np.searchsorted(np.cumsum(cnt), n)) + 1
+            oracle.ppr_batch(rowptr, col, src[:k], cnt[:k], threads=threads)
+            steps = float(cnt[:k].sum()) / 0.15
+            n = int(cnt[:k].sum())
+        elif wl["app"] == "pagerank":
             oracle.pagerank(rowptr, col, w, wl["l"], id_begin=0, id_end=n, threads=threads)
             steps = n * wl["l"]
         elif wl["app"] == "genericwalk":
@@ -119,6 +150,8 @@ def cpu_baseline(g, wl, w, budget_s=20.0):
         if dt > budget_s / 4 or n >= w:
             break
         n = min(w, int(n * max(2.0, (budget_s / 2) / max(dt, 1e-3))))
+    if wl["app"] == "ppr_batch":
+        w = int(QUERIES["table"][1].sum())
     return {"value": steps / dt, "unit": "walk steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle {wl['app']} over walker ids [0, {n}) of {w} on the same graph, "
                       f"{dt:.2f} s on {threads} threads ({_cpu_model()})"}
@@ -145,6 +178,9 @@ def main():
                     help="single-policy workloads only (default partition)")
     ap.add_argument("--epochs", type=int, default=20, help="scorer training epochs (c5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a device-to-device copy of MIB MiB (a kernel of known traffic for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only",
@@ -155,6 +191,15 @@ def main():
 
     wl = WORKLOADS[args.workload]
     info = lt.device_info()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda")
+        a.fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     t0 = time.perf_counter()
     g = lt.Graph.rmat(wl["scale"], wl["ef"])
     log(f"graph s{wl['scale']} ef{wl['ef']}: V={g.num_node} E={g.num_edge} in {time.perf_counter() - t0:.1f} s")
@@ -221,18 +266,57 @@ def main():
                 "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
                 "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],
                 "scorer_training_s": train_s, "training_decisions": n_next + n_res},
-            "roofline": roofline(learned),
+            "roofline": roofline(learned, "c5", "resident"),
             "stats_per_step": learned["per_job"],
         })
+    elif args.workload == "c4":
+        from lighttraffic_amd.tune import tune_ppr_batch
+        src, cnt = g.ppr_queries(wl["nq"])
+        QUERIES["table"] = (src, cnt)
+        t1 = time.perf_counter()
+        best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=wl["slots"] * (wl["mb"] << 20), log=log)
+        tune_s = time.perf_counter() - t1
+        log(f"autotuner picked {best} in {tune_s:.1f} s")
+        part = lt.Partition.build(g, best["partition_mb"] << 20)
+        cfg = dict(num_device_partitions=best["slots"], span=best["span"], device_schedule=best["device_schedule"],
+                   slice_chunks=best["slice_chunks"])
+        eng = lt.Engine(g, part, **cfg)
+        # walk-kernel time for the roofline: the same partitioning, host-scheduled, event timers on
+        teng = lt.Engine(g, part, **dict(cfg, device_schedule=0), event_timers=1)
+        run_job(teng, wl, w)
+        m = measure(eng, wl, w, args.warmup, args.steps, args.workload, timed_engine=teng)
+        desc = (f"c4: R-MAT scale {wl['scale']} ef {wl['ef']} (V={g.num_node}, E={g.num_edge}), batched ppr "
+                f"{wl['nq']} queries (sources ~ degree, counts log-uniform [1e3,1e6), {int(cnt.sum())} walkers), "
+                f"p=0.15, 8 GB cache; autotuned: {best['partition_mb']} MB partitions (P={part.count}), "
+                f"{best['slots']} slots, span {best['span']}, "
+                f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}")
+        out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
+                    "config": {"workload": desc, "scheduler": "greedy", "device": info["name"]},
+                    "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, "c4", best["span"]),
+                    "stats_per_step": m["per_job"],
+                    "autotune": {"seconds": tune_s, "sample": "every 16th query", "best": best,
+                                 "trials": [{k: t[k] for k in ("partition_mb", "span", "device_schedule",
+                                                               "slice_chunks", "running_ms", "h2d_bytes")}
+                                            for t in trials]}})
     else:
         eng = lt.Engine(g, part, span=args.span or "partition", **base)
         teng = lt.Engine(g, part, span=args.span or "partition", event_timers=1, **base)
         run_job(teng, wl, w)
         m = measure(eng, wl, w, args.warmup, args.steps, args.workload, timed_engine=teng)
+        if args.workload == "c3":
+            # explicit copies vs zero-copy reads (the reference's figure-14 ablation)
+            variants = {}
+            for name, thr in (("explicit_only", 0), ("zerocopy_only", 1 << 62)):
+                e2 = lt.Engine(g, part, span=args.span or "partition", zerocopy_threshold=thr, **base)
+                v = measure(e2, wl, w, 1, 1, f"c3/{name}")
+                variants[name] = {k: v[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
+                                                    "h2d_GB_per_job", "pcie_GBps")}
+                e2.close()
+            out["c3_variants"] = variants
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "span": args.span or "partition",
                                "device": info["name"]},
-                    "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m),
+                    "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, args.workload, args.span or "partition"),
                     "stats_per_step": m["per_job"]})
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, wl, w)

@@ -70,7 +70,12 @@ typedef struct {
     int32_t  prefetch_depth;        /* LEARNED: partitions copied ahead (0 = off)  */
     int32_t  prefetch_stick_pct;    /* LEARNED: walk a prefetched partition first
                                        when its queue is >= this % of the largest */
-    int32_t  reserved[5];
+    int32_t  device_schedule;       /* 1: GREEDY decisions, partition copies and
+                                       walks all run on the GPU, 32 iterations
+                                       per hipGraph launch (no host round trip) */
+    int32_t  slice_chunks;          /* chunks of 4096 walkers per walk launch
+                                       (0 = 2048); device_schedule: unused      */
+    int32_t  reserved[3];
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines
@@ -172,6 +177,12 @@ lt_status lt_run_genericwalk(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs
 lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, const uint64_t *counts,
                            float prob, uint32_t *endpoints_out);
 lt_status lt_engine_stats(const lt_engine *e, lt_stats *out);
+/* Batched-PPR query table (no counterpart in the reference): query q draws
+ * Philox(seed; q, 0, TAG_QUERY) = (x0,x1,x2,x3); its source is the row that
+ * owns edge bounded(x0,x1,E) (sources proportional to out-degree), and its
+ * walk count is floor(lo * (hi/lo)^(x2/2^32)) (log-uniform in [lo, hi)). */
+lt_status lt_make_ppr_queries(const lt_graph *g, uint32_t nq, uint64_t lo, uint64_t hi, uint64_t seed,
+                              uint32_t *sources_out, uint64_t *counts_out);
 
 /* Scheduler decision trace of the last run (record_trace=1): one record per
  * decision.  kind: 0 partition already resident, 1 zero-copy, 2 pipelined

@@ -50,7 +50,7 @@ class Config(ctypes.Structure):
     _fields_ = [("num_device_partitions", _u32), ("num_device_walkers", _u64), ("zerocopy_threshold", _u64),
                 ("seed", _u64), ("policy", _i32), ("pipeline", _i32), ("span", _i32), ("event_timers", _i32),
                 ("record_trace", _i32), ("prefetch_depth", _i32), ("prefetch_stick_pct", _i32),
-                ("reserved", _i32 * 5)]
+                ("device_schedule", _i32), ("slice_chunks", _i32), ("reserved", _i32 * 3)]
 
 
 class Stats(ctypes.Structure):
@@ -112,6 +112,7 @@ _sig("lt_run_ppr", ctypes.c_int, _vp, _u64, _u32, ctypes.c_float, _vp, _vp)
 _sig("lt_run_genericwalk", ctypes.c_int, _vp, _u64, _u32, _u32, _vp, _vp)
 _sig("lt_run_ppr_batch", ctypes.c_int, _vp, _u32, _vp, _vp, ctypes.c_float, _vp)
 _sig("lt_engine_stats", ctypes.c_int, _vp, ctypes.POINTER(Stats))
+_sig("lt_make_ppr_queries", ctypes.c_int, _vp, _u32, _u64, _u64, _u64, _vp, _vp)
 _sig("lt_engine_trace_len", _u64, _vp)
 _sig("lt_engine_trace", ctypes.c_int, _vp, ctypes.POINTER(TraceRec), _u64)
 _sig("lt_engine_trace_state", ctypes.c_int, _vp, _vp, _vp, _vp, _u64)
@@ -201,6 +202,14 @@ class Graph:
 
     def save(self, path):
         _check(_lib.lt_graph_save(self._h, path.encode()), f"lt_graph_save({path})")
+
+    def ppr_queries(self, nq, lo=1000, hi=1000000, seed=0x51525953):
+        """Batched-PPR query table: sources proportional to degree, log-uniform counts."""
+        src = np.zeros(nq, np.uint32)
+        cnt = np.zeros(nq, np.uint64)
+        _check(_lib.lt_make_ppr_queries(self._h, nq, lo, hi, seed, src.ctypes.data, cnt.ctypes.data),
+               "lt_make_ppr_queries")
+        return src, cnt
 
     @property
     def num_node(self):

@@ -78,8 +78,36 @@ struct AppArgs {
 
 enum StatSlot { ST_MOVES = 0, ST_TELEPORTS, ST_VISITS, ST_RESIDENCIES, ST_ZC_MOVES, ST_ZC_READS, ST_DROPPED, kNumStats = 8 };
 
+// Device-side scheduling (the whole decide / load / walk / fix-up loop runs
+// on the GPU, captured in a hipGraph).  One decision per iteration.
+struct DevSched {
+    int32_t choice;     // partition walked this iteration, -1 = every queue is empty
+    int32_t zerocopy;   // walked from host memory
+    int32_t load;       // copied into `slot` first
+    int32_t slot;
+    int32_t victim;     // partition evicted for the load, -1 = none
+    uint32_t iterations, loads, zc_calls;
+    unsigned long long h2d_bytes;
+};
+
+struct SchedState {
+    DevSched *ds;             // nullptr: the host schedules
+    int32_t *slot_of;         // [P] slot or -1
+    int32_t *part_in_slot;    // [nslots] partition or -1
+    uint8_t *resident;        // [P]
+    const uint64_t **rowptr_tab;  // writable views of Parts' tables
+    const uint32_t **col_tab;
+    uint8_t *slots;
+    uint64_t slot_bytes;
+    uint32_t nslots;
+    const uint64_t *zc_rowptr;  // device view of the pinned host graph
+    const uint32_t *zc_col;
+    uint64_t zc_threshold;
+};
+
 struct WalkArgs {
     AppArgs app;
+    SchedState ss;
     Queues q;
     Parts pt;
     uint32_t src;         // partition whose queue is consumed

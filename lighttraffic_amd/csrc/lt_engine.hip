@@ -132,7 +132,8 @@ struct Engine {
     uint32_t V = 0, P = 0;
     hipStream_t s_compute = nullptr, s_copy = nullptr;
     int num_cu = 0;
-    int max_grid = 0;
+    int max_grid = 0;           // init kernel (4 walkers per thread)
+    int walk_grid[5] = {0, 0, 0, 0, 0};  // walk kernel, by walkers per thread
 
     // partition table
     uint32_t *d_node_off = nullptr;
@@ -178,6 +179,11 @@ struct Engine {
     std::vector<uint8_t> trace_resident; // residency seen by each traced decision
     std::vector<int32_t> trace_recent;   // last kRecent loads at each traced decision
     Scorer *scorer = nullptr;             // learned policy (not owned)
+    // device-side scheduling state (mirrors of slot_of / part_in_slot / resident)
+    DevSched *d_ds = nullptr;
+    int32_t *d_slot_of = nullptr, *d_part_in_slot = nullptr;
+    uint8_t *d_resident = nullptr;
+    uint32_t slice = kSliceChunks;
     std::vector<hipEvent_t> pool_events;  // reusable prefetch-completion events
     std::vector<float> scores;            // scorer output of the current decision
     bool scores_valid = false;
@@ -200,6 +206,8 @@ struct Engine {
     lt_status set_table(uint32_t p, bool present);
     lt_status scheduler_loop();
     lt_status scheduler_loop_learned();
+    lt_status scheduler_loop_device();
+    SchedState sched_state(bool device) const;
     lt_status infer_scores(int32_t cur);
     int32_t learned_victim(int32_t a, int32_t b);
     void recent_loads(int32_t *out) const;
@@ -207,7 +215,8 @@ struct Engine {
 
     AppArgs app{};
     int app_kind = APP_PAGERANK;
-    size_t lds_bytes() const { return append_lds_bytes(P); }
+    size_t lds_bytes(int wpt = kWpt) const { return append_lds_bytes(P, wpt); }
+    lt_status launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s);
     uint64_t partition_bytes(uint32_t p) const {
         return ((uint64_t)part.node_off[p + 1] - part.node_off[p] + 1) * 8 +
                (part.edge_off[p + 1] - part.edge_off[p]) * 4;
@@ -236,6 +245,10 @@ Engine::~Engine() {
     dfree(d_qoff);
     dfree(d_qsrc);
     dfree(d_stats);
+    dfree(d_ds);
+    dfree(d_slot_of);
+    dfree(d_part_in_slot);
+    dfree(d_resident);
     if (s_compute) (void)hipStreamDestroy(s_compute);
     if (s_copy) (void)hipStreamDestroy(s_copy);
 }
@@ -287,8 +300,14 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     LT_HIP(hipStreamCreateWithFlags(&s_compute, hipStreamNonBlocking));
     LT_HIP(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
     int occ = 0;
-    LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, walk_kernel<APP_PAGERANK>, kBlock, lds_bytes()));
+    LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, init_kernel<APP_PAGERANK>, kBlock, lds_bytes()));
     max_grid = std::max(1, occ) * num_cu;
+    const void *wk[5] = {nullptr, (const void *)walk_kernel<APP_PAGERANK, 1>, (const void *)walk_kernel<APP_PAGERANK, 2>,
+                         nullptr, (const void *)walk_kernel<APP_PAGERANK, 4>};
+    for (int wpt : {1, 2, 4}) {
+        LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wk[wpt], kBlock, lds_bytes(wpt)));
+        walk_grid[wpt] = std::max(1, occ) * num_cu;
+    }
 
     // partition table + vertex -> partition lookup table
     LT_TRY(dmalloc(&d_node_off, P + 1));
@@ -322,6 +341,11 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
 
     LT_TRY(dmalloc(&d_meta, P + 4));
+    LT_TRY(dmalloc(&d_ds, 1));
+    LT_TRY(dmalloc(&d_slot_of, P));
+    LT_TRY(dmalloc(&d_part_in_slot, nslots));
+    LT_TRY(dmalloc(&d_resident, P));
+    slice = cfg.slice_chunks > 0 ? (uint32_t)cfg.slice_chunks : kSliceChunks;
     LT_HIP(hipHostMalloc((void **)&h_meta, 4 * (P + 4), hipHostMallocDefault));
     LT_TRY(dmalloc(&d_stats, kNumStats));
     counts.assign(P, 0);
@@ -343,8 +367,9 @@ lt_status Engine::ensure_pool(uint64_t walkers) {
     dfree(q.free_stack);
     dfree(q.lost);
     const uint64_t wchunks = (walkers + kChunk - 1) / kChunk;
-    const uint64_t slice = std::min<uint64_t>(kSliceChunks, wchunks + 1);
-    const uint64_t nchunks = wchunks + P + slice + 2 + kLostCap;
+    // one launch holds at most `held` source chunks while filling new ones
+    const uint64_t held = cfg.device_schedule ? wchunks + 1 : std::min<uint64_t>(slice, wchunks + 1);
+    const uint64_t nchunks = wchunks + P + held + 2 + kLostCap;
     const uint64_t maxc = wchunks + 2;
     if (nchunks >= 0xFFFFFFF0ull || maxc * kChunk > 0xFFFFFFFFull) {
         set_error("%llu walkers exceed the queue limits; lower -b", (unsigned long long)walkers);
@@ -450,6 +475,34 @@ lt_status Engine::load_async(uint32_t p) {
     return LT_OK;
 }
 
+// Walk launch sized to the slice.  When walkers leave the span after a step
+// or two, the append stage dominates and its per-workgroup cost (histogram,
+// scan and one atomic per destination partition) is best spread over 4
+// walkers per thread.  When every partition is resident and walks continue
+// through all of them, nothing is appended and occupancy matters: 1 or 2
+// walkers per thread, so that small queues still fill every SIMD.
+template <int APP>
+static void launch_walk_app(const WalkArgs &a, int wpt, int grid, size_t lds, hipStream_t s) {
+    if (wpt == 4) walk_kernel<APP, 4><<<grid, kBlock, lds, s>>>(a);
+    else if (wpt == 2) walk_kernel<APP, 2><<<grid, kBlock, lds, s>>>(a);
+    else walk_kernel<APP, 1><<<grid, kBlock, lds, s>>>(a);
+}
+
+lt_status Engine::launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s) {
+    const uint64_t fill = (uint64_t)num_cu * 4 * 6 * 64;  // threads for ~6 waves per SIMD
+    const bool walk_bound = cfg.span == LT_SPAN_RESIDENT && nslots == P;
+    const int wpt = !walk_bound || walkers >= 4 * fill ? 4 : walkers >= 2 * fill ? 2 : 1;
+    const uint64_t bw = (uint64_t)kBlock * wpt;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + bw - 1) / bw, (uint64_t)walk_grid[wpt]));
+    switch (app_kind) {
+        case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, wpt, grid, lds_bytes(wpt), s); break;
+        case APP_PPR: launch_walk_app<APP_PPR>(a, wpt, grid, lds_bytes(wpt), s); break;
+        default: launch_walk_app<APP_GENERIC>(a, wpt, grid, lds_bytes(wpt), s); break;
+    }
+    LT_HIP(hipGetLastError());
+    return LT_OK;
+}
+
 // Consume the whole queue of partition p in slices of kSliceChunks chunks:
 // walk kernel, then fix-up kernel per slice (processInMemory with all
 // batches, src/WalkManager.h:375-388).
@@ -469,6 +522,7 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
     a.zerocopy = zerocopy ? 1 : 0;
     a.span_resident = cfg.span == LT_SPAN_RESIDENT ? 1 : 0;
     a.stats = d_stats;
+    a.ss = sched_state(false);
     if (zerocopy) {
         a.rowptr = zc_rowptr + part.node_off[p];
         a.col = zc_col + part.edge_off[p];
@@ -481,22 +535,17 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
     EventTimer &phase = zerocopy ? t_zc : t_compute;
     hipEvent_t tp = phase.begin(s_compute);
     const uint64_t nch = (walkers + kChunk - 1) / kChunk;
-    for (uint64_t lo = 0; lo < nch; lo += kSliceChunks) {
-        const uint64_t hi = std::min<uint64_t>(nch, lo + kSliceChunks);
+    for (uint64_t lo = 0; lo < nch; lo += slice) {
+        const uint64_t hi = std::min<uint64_t>(nch, lo + slice);
         const bool last = hi == nch;
         a.chunk_lo = (uint32_t)lo;
         a.chunk_hi = last ? 0xFFFFFFFFu : (uint32_t)hi;
         const uint64_t sw = std::min<uint64_t>(walkers, hi * kChunk) - lo * kChunk;
-        const int grid = (int)std::min<uint64_t>((sw + kBlockWalkers - 1) / kBlockWalkers, (uint64_t)max_grid);
         hipEvent_t tk = t_kernel.begin(s_compute);
-        switch (app_kind) {
-            case APP_PAGERANK: walk_kernel<APP_PAGERANK><<<grid, kBlock, lds_bytes(), s_compute>>>(a); break;
-            case APP_PPR: walk_kernel<APP_PPR><<<grid, kBlock, lds_bytes(), s_compute>>>(a); break;
-            default: walk_kernel<APP_GENERIC><<<grid, kBlock, lds_bytes(), s_compute>>>(a); break;
-        }
-        LT_HIP(hipGetLastError());
+        LT_TRY(launch_walk(a, sw, s_compute));
         t_kernel.end(tk, s_compute);
-        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, p, (uint32_t)lo, last ? 0xFFFFFFFFu : (uint32_t)hi, last ? 1 : 0);
+        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, p, (uint32_t)lo, last ? 0xFFFFFFFFu : (uint32_t)hi, last ? 1 : 0,
+                                                  nullptr);
         LT_HIP(hipGetLastError());
         st.kernel_launches++;
     }
@@ -798,6 +847,114 @@ lt_status Engine::scheduler_loop_learned() {
     return LT_OK;
 }
 
+SchedState Engine::sched_state(bool device) const {
+    SchedState ss{};
+    ss.ds = device ? d_ds : nullptr;
+    ss.slot_of = d_slot_of;
+    ss.part_in_slot = d_part_in_slot;
+    ss.resident = d_resident;
+    ss.rowptr_tab = d_rowptr_tab;
+    ss.col_tab = d_col_tab;
+    ss.slots = d_slots;
+    ss.slot_bytes = (part.bytes + 255) & ~255ull;
+    ss.nslots = nslots;
+    ss.zc_rowptr = zc_rowptr;
+    ss.zc_col = zc_col;
+    ss.zc_threshold = zc_threshold;
+    return ss;
+}
+
+// Device-scheduled loop: per iteration decide_kernel (plain greedy rules in
+// one workgroup), load_kernel (partition copy over PCIe by the GPU itself),
+// walk_kernel and fixup_kernel, all reading the decision from device memory.
+// kGraphIters iterations are captured once into a hipGraph and the graph is
+// relaunched until an iteration finds every queue empty; the host reads one
+// 40-byte record per launch instead of syncing on every decision.
+constexpr int kGraphIters = 32;
+
+lt_status Engine::scheduler_loop_device() {
+    // host residency -> device mirrors
+    std::vector<int32_t> hs(P), hp(nslots);
+    for (uint32_t i = 0; i < P; i++) hs[i] = resident[i] ? slot_of[i] : -1;
+    for (uint32_t s2 = 0; s2 < nslots; s2++) hp[s2] = part_in_slot[s2];
+    LT_HIP(hipMemcpyAsync(d_slot_of, hs.data(), 4 * P, hipMemcpyHostToDevice, s_compute));
+    LT_HIP(hipMemcpyAsync(d_part_in_slot, hp.data(), 4 * nslots, hipMemcpyHostToDevice, s_compute));
+    LT_HIP(hipMemcpyAsync(d_resident, resident.data(), P, hipMemcpyHostToDevice, s_compute));
+    DevSched zero{};
+    zero.choice = 0;
+    LT_HIP(hipMemcpyAsync(d_ds, &zero, sizeof(zero), hipMemcpyHostToDevice, s_compute));
+    LT_HIP(hipStreamSynchronize(s_compute));
+
+    WalkArgs a{};
+    a.app = app;
+    a.q = q;
+    a.pt.node_off = d_node_off;
+    a.pt.edge_off = d_edge_off;
+    a.pt.lut = d_lut;
+    a.pt.rowptr_tab = d_rowptr_tab;
+    a.pt.col_tab = d_col_tab;
+    a.pt.lut_shift = lut_shift;
+    a.pt.P = P;
+    a.span_resident = cfg.span == LT_SPAN_RESIDENT ? 1 : 0;
+    a.stats = d_stats;
+    a.ss = sched_state(true);
+    const int load_grid = 4 * num_cu;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    LT_HIP(hipStreamBeginCapture(s_compute, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < kGraphIters; it++) {
+        decide_kernel<<<1, 1024, 0, s_compute>>>(q, a.ss, a.pt);
+        load_kernel<<<load_grid, 256, 0, s_compute>>>(a.ss, a.pt);
+        switch (app_kind) {
+            case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
+            case APP_PPR: launch_walk_app<APP_PPR>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
+            default: launch_walk_app<APP_GENERIC>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
+        }
+        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, d_ds);
+    }
+    hipError_t ce = hipStreamEndCapture(s_compute, &graph);
+    if (ce != hipSuccess) {
+        set_error("graph capture failed: %s", hipGetErrorString(ce));
+        return LT_E_HIP;
+    }
+    std::unique_ptr<hipGraph_t, void (*)(hipGraph_t *)> gg(&graph, [](hipGraph_t *x) { (void)hipGraphDestroy(*x); });
+    LT_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    std::unique_ptr<hipGraphExec_t, void (*)(hipGraphExec_t *)> ge(&exec,
+                                                                    [](hipGraphExec_t *x) { (void)hipGraphExecDestroy(*x); });
+    DevSched h{};
+    uint64_t launches = 0;
+    while (true) {
+        LT_HIP(hipGraphLaunch(exec, s_compute));
+        launches++;
+        LT_HIP(hipMemcpyAsync(&h, d_ds, sizeof(h), hipMemcpyDeviceToHost, s_compute));
+        LT_HIP(hipStreamSynchronize(s_compute));
+        if (h.choice < 0) break;
+        if (launches > (1ull << 24)) {
+            set_error("device schedule did not drain the queues");
+            return LT_E_INTERNAL;
+        }
+    }
+    LT_TRY(read_meta());
+    // device residency -> host mirrors
+    LT_HIP(hipMemcpy(hs.data(), d_slot_of, 4 * P, hipMemcpyDeviceToHost));
+    LT_HIP(hipMemcpy(hp.data(), d_part_in_slot, 4 * nslots, hipMemcpyDeviceToHost));
+    LT_HIP(hipMemcpy(resident.data(), d_resident, P, hipMemcpyDeviceToHost));
+    free_slots.clear();
+    for (uint32_t i = 0; i < P; i++) slot_of[i] = resident[i] ? hs[i] : -1;
+    for (int32_t s2 = (int32_t)nslots - 1; s2 >= 0; s2--) {
+        part_in_slot[s2] = hp[s2];
+        if (hp[s2] < 0) free_slots.push_back((uint32_t)s2);
+    }
+    st.iterations += h.iterations;
+    st.explicit_iterations += h.iterations - h.zc_calls;
+    st.graph_loading_calls += h.loads;
+    st.zerocopy_calls += h.zc_calls;
+    st.computing_calls += h.iterations;
+    st.h2d_bytes_explicit += h.h2d_bytes;
+    st.kernel_launches += (uint64_t)launches * kGraphIters;
+    return LT_OK;
+}
+
 lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoints_out, uint32_t *moves_out) {
     LT_HIP(hipSetDevice(device));
     memset(&st, 0, sizeof(st));
@@ -869,12 +1026,13 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
             default: init_kernel<APP_GENERIC><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
         }
         LT_HIP(hipGetLastError());
-        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0);
+        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, nullptr);
         LT_HIP(hipGetLastError());
         LT_HIP(hipStreamSynchronize(s_compute));
         const auto t0 = std::chrono::steady_clock::now();
         w_run = id1 - id0;
-        LT_TRY(cfg.policy == LT_SCHED_LEARNED && scorer ? scheduler_loop_learned() : scheduler_loop());
+        if (cfg.device_schedule) LT_TRY(scheduler_loop_device());
+        else LT_TRY(cfg.policy == LT_SCHED_LEARNED && scorer ? scheduler_loop_learned() : scheduler_loop());
         const auto t1 = std::chrono::steady_clock::now();
         running_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     }

@@ -7,10 +7,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <math.h>
+
 #include <algorithm>
 #include <memory>
 
 #include "lt_internal.h"
+#include "lt_rng.h"
 
 namespace lt {
 
@@ -383,6 +386,23 @@ int32_t lt_sched_computed_partition(uint32_t P, const uint64_t *counts, const ui
         }
     }
     return full >= 0 ? full : partial;
+}
+
+lt_status lt_make_ppr_queries(const lt_graph *g, uint32_t nq, uint64_t lo, uint64_t hi, uint64_t seed,
+                              uint32_t *sources_out, uint64_t *counts_out) {
+    if (!g || !sources_out || !counts_out || g->E == 0 || lo == 0 || hi < lo) return LT_E_ARG;
+    const double llo = log((double)lo), lhi = log((double)hi);
+    for (uint32_t q = 0; q < nq; q++) {
+        const lt::u32x4 x = lt::draw(seed, q, 0, lt::TAG_QUERY);
+        const uint64_t e = lt::bounded(x.x, x.y, g->E);
+        // last row r with rowptr[r] <= e
+        const uint64_t *it = std::upper_bound(g->rowptr, g->rowptr + g->V + 1, e);
+        sources_out[q] = (uint32_t)(it - g->rowptr - 1);
+        const double u = (double)x.z / 4294967296.0;
+        uint64_t c = (uint64_t)floor(exp(llo + u * (lhi - llo)));
+        counts_out[q] = std::min<uint64_t>(std::max<uint64_t>(c, lo), hi);
+    }
+    return LT_OK;
 }
 
 lt_status lt_device_info(char *name, size_t cap, int *num_cu, uint64_t *hbm_bytes) {

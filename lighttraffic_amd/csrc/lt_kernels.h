@@ -82,13 +82,14 @@ struct AppendLds {
     uint16_t *dst;    // [kBlockWalkers] destination of each staged walker
 };
 
-__host__ __device__ inline size_t append_lds_bytes(uint32_t P) {
+// walkers staged per workgroup pass = kBlock * wpt
+__host__ __device__ inline size_t append_lds_bytes(uint32_t P, int wpt) {
     size_t b = (4 * (size_t)P + 1) * 4;
     b = (b + 15) & ~(size_t)15;
-    return b + (size_t)kBlockWalkers * (16 + 2);
+    return b + (size_t)kBlock * wpt * (16 + 2);
 }
 
-__device__ __forceinline__ AppendLds carve_lds(uint8_t *smem, uint32_t P) {
+__device__ __forceinline__ AppendLds carve_lds(uint8_t *smem, uint32_t P, int wpt) {
     AppendLds L;
     L.hist = (uint32_t *)smem;
     L.scan = L.hist + P;
@@ -96,7 +97,7 @@ __device__ __forceinline__ AppendLds carve_lds(uint8_t *smem, uint32_t P) {
     L.ch1 = L.ch0 + P;
     size_t off = ((4 * (size_t)P + 1) * 4 + 15) & ~(size_t)15;
     L.stage = (Walker *)(smem + off);
-    L.dst = (uint16_t *)(L.stage + kBlockWalkers);
+    L.dst = (uint16_t *)(L.stage + kBlock * wpt);
     return L;
 }
 
@@ -137,14 +138,15 @@ __device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint
 // contains a chunk's first slot links it, and a later range that arrives
 // before that link is published links it itself (the CAS loser's chunk goes
 // to the lost list).
-__device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, const Walker (&w)[kWpt],
-                             const uint32_t (&dst)[kWpt], uint32_t *s_wsum) {
+template <int WPT>
+__device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, const Walker (&w)[WPT],
+                             const uint32_t (&dst)[WPT], uint32_t *s_wsum) {
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < P; i += kBlock) L.hist[i] = 0;
     __syncthreads();
-    uint32_t rank[kWpt];
+    uint32_t rank[WPT];
 #pragma unroll
-    for (int k = 0; k < kWpt; k++)
+    for (int k = 0; k < WPT; k++)
         if (dst[k] != kEmpty) rank[k] = atomicAdd(&L.hist[dst[k]], 1u);
     __syncthreads();
     const uint32_t total = block_exclusive_scan(L.hist, L.scan, P, s_wsum);
@@ -175,7 +177,7 @@ __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, co
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kWpt; k++) {
+    for (int k = 0; k < WPT; k++) {
         if (dst[k] != kEmpty) {
             const uint32_t pos = L.scan[dst[k]] + rank[k];
             L.stage[pos] = w[k];
@@ -194,12 +196,13 @@ __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, co
     __syncthreads();
 }
 
+// Per-thread event counts of one launch (far below 2^32), summed as u64.
 struct Counters {
-    uint64_t moves = 0, tele = 0, visits = 0, resid = 0, zc_moves = 0, zc_reads = 0;
+    uint32_t moves = 0, tele = 0, visits = 0, resid = 0, zc_moves = 0, zc_reads = 0;
 };
 
 __device__ void flush_counters(const Counters &c, unsigned long long *stats) {
-    uint64_t v[6] = {c.moves, c.tele, c.visits, c.resid, c.zc_moves, c.zc_reads};
+    const uint64_t v[6] = {c.moves, c.tele, c.visits, c.resid, c.zc_moves, c.zc_reads};
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         uint64_t x = v[i];
@@ -305,7 +308,7 @@ __device__ __forceinline__ void finish_walker(const Walker &w, const AppArgs &a)
 // walk is over.  With span_resident the walker continues into any partition
 // whose slot is resident (and into the source slice itself).
 template <int APP>
-__device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, const Span &src_sp, Counters &ct) {
+__device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, uint32_t src, const Span &src_sp, Counters &ct) {
     Span sp = src_sp;
     while (true) {
         while (w.step > 0 && (w.cur - sp.nb) < (sp.ne - sp.nb)) app_step<APP>(w, sp, a.app, ct);
@@ -315,7 +318,7 @@ __device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, const Span &src
         }
         const uint32_t p = find_part(a.pt, w.cur);
         if (!a.span_resident) return p;
-        if (p == a.src) {
+        if (p == src) {
             sp = src_sp;
             continue;
         }
@@ -330,32 +333,51 @@ __device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, const Span &src
     }
 }
 
-template <int APP>
+// WPT walkers per thread per pass (1, 2 or 4): small queues use 1 so that
+// enough workgroups exist to fill the chip, large ones 4.
+template <int APP, int WPT>
 __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
+    constexpr int BW = kBlock * WPT;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ uint32_t s_wsum[kBlock / 64];
-    const AppendLds L = carve_lds(smem, a.pt.P);
-    const uint32_t n = a.q.qlen[a.src];
-    const uint64_t lo64 = (uint64_t)a.chunk_lo << kChunkLog2;
-    const uint64_t hi64 = min((uint64_t)n, (uint64_t)a.chunk_hi << kChunkLog2);
+    const AppendLds L = carve_lds(smem, a.pt.P, WPT);
+    uint32_t src = a.src, chunk_lo = a.chunk_lo, chunk_hi = a.chunk_hi;
     Span src_sp;
     src_sp.rowptr = a.rowptr;
     src_sp.col = a.col;
-    src_sp.nb = a.pt.node_off[a.src];
-    src_sp.ne = a.pt.node_off[a.src + 1];
-    src_sp.eb = a.pt.edge_off[a.src];
     src_sp.zc = a.zerocopy;
+    if (a.ss.ds) {  // the decision of this iteration was taken on the device
+        const int32_t c = a.ss.ds->choice;
+        if (c < 0) return;
+        src = (uint32_t)c;
+        chunk_lo = 0;
+        chunk_hi = 0xFFFFFFFFu;
+        src_sp.zc = a.ss.ds->zerocopy;
+        if (src_sp.zc) {
+            src_sp.rowptr = a.ss.zc_rowptr + a.pt.node_off[src];
+            src_sp.col = a.ss.zc_col + a.pt.edge_off[src];
+        } else {
+            const uint8_t *base = a.ss.slots + (size_t)a.ss.slot_of[src] * a.ss.slot_bytes;
+            src_sp.rowptr = (const uint64_t *)base;
+            src_sp.col = (const uint32_t *)(base + ((uint64_t)a.pt.node_off[src + 1] - a.pt.node_off[src] + 1) * 8);
+        }
+    }
+    const uint32_t n = a.q.qlen[src];
+    const uint64_t lo64 = (uint64_t)chunk_lo << kChunkLog2;
+    const uint64_t hi64 = min((uint64_t)n, (uint64_t)chunk_hi << kChunkLog2);
+    src_sp.nb = a.pt.node_off[src];
+    src_sp.ne = a.pt.node_off[src + 1];
+    src_sp.eb = a.pt.edge_off[src];
     Counters ct;
-    for (uint64_t base = lo64 + (uint64_t)blockIdx.x * kBlockWalkers; base < hi64;
-         base += (uint64_t)gridDim.x * kBlockWalkers) {
-        const uint32_t chunk = a.q.frame[(size_t)a.src * a.q.maxc + (uint32_t)(base >> kChunkLog2)];
+    for (uint64_t base = lo64 + (uint64_t)blockIdx.x * BW; base < hi64; base += (uint64_t)gridDim.x * BW) {
+        const uint32_t chunk = a.q.frame[(size_t)src * a.q.maxc + (uint32_t)(base >> kChunkLog2)];
         const bool chunk_ok = chunk < a.q.nchunks;
         if (!chunk_ok && threadIdx.x == 0) atomicOr(&a.q.ctl->overflow, 8u);
         const Walker *cw = a.q.slab + (size_t)(chunk_ok ? chunk : 0) * kChunk + (uint32_t)(base & (kChunk - 1));
-        Walker w[kWpt];
-        uint32_t dst[kWpt];
+        Walker w[WPT];
+        uint32_t dst[WPT];
 #pragma unroll
-        for (int k = 0; k < kWpt; k++) {
+        for (int k = 0; k < WPT; k++) {
             const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
             dst[k] = kEmpty;
             if (chunk_ok && i < hi64) {
@@ -366,11 +388,11 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
             }
         }
 #pragma unroll
-        for (int k = 0; k < kWpt; k++) {
+        for (int k = 0; k < WPT; k++) {
             const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-            if (chunk_ok && i < hi64) dst[k] = walk_residency<APP>(w[k], a, src_sp, ct);
+            if (chunk_ok && i < hi64) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
         }
-        block_append(a.q, a.pt.P, L, w, dst, s_wsum);
+        block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum);
     }
     flush_counters(ct, a.stats);
 }
@@ -397,7 +419,7 @@ template <int APP>
 __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ uint32_t s_wsum[kBlock / 64];
-    const AppendLds L = carve_lds(smem, a.pt.P);
+    const AppendLds L = carve_lds(smem, a.pt.P, kWpt);
     for (uint64_t base = a.id_lo + (uint64_t)blockIdx.x * kBlockWalkers; base < a.id_hi;
          base += (uint64_t)gridDim.x * kBlockWalkers) {
         Walker w[kWpt];
@@ -438,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
                 finish_walker(w[k], a.app);
             }
         }
-        block_append(a.q, a.pt.P, L, w, dst, s_wsum);
+        block_append<kWpt>(a.q, a.pt.P, L, w, dst, s_wsum);
     }
 }
 
@@ -446,8 +468,14 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
 // consumed source chunks [chunk_lo, chunk_hi) to the free stack; clear the
 // source queue after its last slice.  One workgroup.
 __global__ __launch_bounds__(kBlock) void fixup_kernel(Queues q, uint32_t src, uint32_t chunk_lo, uint32_t chunk_hi,
-                                                       int last) {
+                                                       int last, const DevSched *ds) {
     __shared__ uint32_t s_top, s_nlost, s_cnt;
+    if (ds) {  // device-scheduled iteration: the whole queue of the choice was consumed
+        src = ds->choice < 0 ? kEmpty : (uint32_t)ds->choice;
+        chunk_lo = 0;
+        chunk_hi = 0xFFFFFFFFu;
+        last = 1;
+    }
     if (threadIdx.x == 0) {
         PoolCtl c = *q.ctl;
         uint32_t popped = c.popped;
@@ -511,6 +539,121 @@ __global__ void set_table_kernel(const uint64_t **rowptr_tab, const uint32_t **c
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (size_t i = tid; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---- device-side scheduling ------------------------------------------------
+// One workgroup decides the iteration with the plain rules: the partition with
+// the most queued walkers (lowest id on ties); walk it in place if resident,
+// read it over PCIe if it holds fewer than zc_threshold walkers, else copy it
+// into a free slot or into the slot of the resident partition with the fewest
+// walkers.  Tables are updated here; the copy is done by load_kernel next.
+__global__ __launch_bounds__(1024) void decide_kernel(Queues q, SchedState ss, Parts pt) {
+    __shared__ unsigned long long s_best[1024];
+    __shared__ unsigned long long s_victim[1024];
+    __shared__ uint32_t s_free[1024];
+    const uint32_t tid = threadIdx.x, P = pt.P;
+    unsigned long long best = 0, victim = ~0ull;
+    for (uint32_t p = tid; p < P; p += 1024) {
+        const unsigned long long c = q.qlen[p];
+        const unsigned long long key = (c << 32) | (0xFFFFFFFFull - p);
+        if (c > 0 && key > best) best = key;
+    }
+    s_best[tid] = best;
+    __syncthreads();
+    for (uint32_t off = 512; off > 0; off >>= 1) {
+        if (tid < off && s_best[tid + off] > s_best[tid]) s_best[tid] = s_best[tid + off];
+        __syncthreads();
+    }
+    best = s_best[0];
+    DevSched *ds = ss.ds;
+    if (best == 0) {
+        if (tid == 0) {
+            ds->choice = -1;
+            ds->load = 0;
+            ds->zerocopy = 0;
+            ds->victim = -1;
+        }
+        return;
+    }
+    const uint32_t c = 0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFull);
+    const uint64_t cnt = best >> 32;
+    const bool need_slot = !ss.resident[c] && cnt >= ss.zc_threshold;
+    uint32_t fr = 0xFFFFFFFFu;
+    if (need_slot) {
+        for (uint32_t s = tid; s < ss.nslots; s += 1024)
+            if (ss.part_in_slot[s] < 0 && s < fr) fr = s;
+        for (uint32_t p = tid; p < P; p += 1024) {
+            if (!ss.resident[p] || p == c) continue;
+            const unsigned long long key = ((unsigned long long)q.qlen[p] << 32) | p;
+            if (key < victim) victim = key;
+        }
+    }
+    s_free[tid] = fr;
+    s_victim[tid] = victim;
+    __syncthreads();
+    for (uint32_t off = 512; off > 0; off >>= 1) {
+        if (tid < off) {
+            if (s_free[tid + off] < s_free[tid]) s_free[tid] = s_free[tid + off];
+            if (s_victim[tid + off] < s_victim[tid]) s_victim[tid] = s_victim[tid + off];
+        }
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    ds->choice = (int32_t)c;
+    ds->iterations++;
+    ds->victim = -1;
+    ds->load = 0;
+    ds->zerocopy = 0;
+    if (ss.resident[c]) return;
+    if (!need_slot) {
+        ds->zerocopy = 1;
+        ds->zc_calls++;
+        return;
+    }
+    uint32_t slot = s_free[0];
+    if (slot == 0xFFFFFFFFu) {
+        if (s_victim[0] == ~0ull) {  // nothing to evict: read over PCIe instead
+            ds->zerocopy = 1;
+            ds->zc_calls++;
+            return;
+        }
+        const uint32_t v = (uint32_t)(s_victim[0] & 0xFFFFFFFFull);
+        slot = (uint32_t)ss.slot_of[v];
+        ss.slot_of[v] = -1;
+        ss.resident[v] = 0;
+        ss.rowptr_tab[v] = nullptr;
+        ss.col_tab[v] = nullptr;
+        ds->victim = (int32_t)v;
+    }
+    uint8_t *base = ss.slots + (size_t)slot * ss.slot_bytes;
+    const uint64_t n1 = (uint64_t)pt.node_off[c + 1] - pt.node_off[c] + 1;
+    const uint64_t m = pt.edge_off[c + 1] - pt.edge_off[c];
+    ss.slot_of[c] = (int32_t)slot;
+    ss.part_in_slot[slot] = (int32_t)c;
+    ss.resident[c] = 1;
+    ss.rowptr_tab[c] = (const uint64_t *)base;
+    ss.col_tab[c] = (const uint32_t *)(base + n1 * 8);
+    ds->slot = (int32_t)slot;
+    ds->load = 1;
+    ds->loads++;
+    ds->h2d_bytes += n1 * 8 + m * 4;
+}
+
+// Copy the decided partition from pinned host memory into its slot: rowptr
+// slice, then col slice, as 4-byte words (grid-stride reads over PCIe).
+__global__ __launch_bounds__(256) void load_kernel(SchedState ss, Parts pt) {
+    const DevSched *ds = ss.ds;
+    if (!ds->load) return;
+    const uint32_t c = (uint32_t)ds->choice;
+    const uint64_t n1 = (uint64_t)pt.node_off[c + 1] - pt.node_off[c] + 1;
+    const uint64_t m = pt.edge_off[c + 1] - pt.edge_off[c];
+    uint32_t *dst = (uint32_t *)(ss.slots + (size_t)ds->slot * ss.slot_bytes);
+    const uint32_t *src_r = (const uint32_t *)(ss.zc_rowptr + pt.node_off[c]);
+    const uint32_t *src_c = ss.zc_col + pt.edge_off[c];
+    const uint64_t nr = 2 * n1, total = nr + m;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride)
+        dst[i] = i < nr ? src_r[i] : src_c[i - nr];
 }
 
 }  // namespace lt

@@ -60,6 +60,10 @@ CONFIGS = [
     ("fifo-nopipe", 16 << 10, {"num_device_partitions": 4, "policy": "fifo", "pipeline": 0, "zerocopy_threshold": 0}),
     ("waves", 16 << 10, {"num_device_partitions": 5, "num_device_walkers": 3000}),
     ("span-resident", 16 << 10, {"num_device_partitions": 6, "span": "resident"}),
+    ("device-sched", 16 << 10, {"num_device_partitions": 3, "device_schedule": 1}),
+    ("device-sched-zc", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "zerocopy_threshold": 2000,
+                                   "span": "resident"}),
+    ("small-slices", 16 << 10, {"num_device_partitions": 4, "slice_chunks": 1, "num_device_walkers": 9000}),
 ]
 
 
@@ -133,10 +137,11 @@ def test_dangling_vertices(kw):
     assert np.array_equal(counts, rc) and np.array_equal(ends, re)
 
 
-def test_ppr_batch_bit_exact(g12):
-    log("ppr batch")
+@pytest.mark.parametrize("kw", [{}, {"device_schedule": 1}, {"device_schedule": 1, "span": "resident"}])
+def test_ppr_batch_bit_exact(g12, kw):
+    log(f"ppr batch {kw}")
     part = lt.Partition.build(g12, 16 << 10)
-    eng = lt.Engine(g12, part, num_device_partitions=4)
+    eng = lt.Engine(g12, part, num_device_partitions=4, **kw)
     rng = np.random.default_rng(9)
     nq = 64
     sources = rng.integers(0, g12.num_node, nq).astype(np.uint32)

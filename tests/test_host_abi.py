@@ -201,3 +201,28 @@ def test_cli_materialises_partitions_then_needs_a_device(tmp_path):
     r = subprocess.run([os.path.join(lt.BIN_DIR, "ppr"), "-f", str(tmp_path / "none.bcsr")], capture_output=True,
                        text=True)
     assert r.returncode == 1 and "open graph" in r.stderr
+
+
+def test_ppr_query_table():
+    """Sources proportional to degree, counts log-uniform; restated with the
+    oracle's Philox and checked statistically."""
+    g = lt.Graph.open(os.path.join(GOLDEN, "rmat_s10_ef8.bcsr"))
+    nq = 20000
+    src, cnt = g.ppr_queries(nq, 1000, 1000000, seed=99)
+    rowptr = g.rowptr()
+    E = int(rowptr[-1])
+    tag = 0x51525953
+    for q in range(0, nq, 997):
+        x = oracle.philox([q, 0, 0, tag], [99, 0])
+        e = (((x[0] << 32) | x[1]) * E) >> 64
+        assert src[q] == int(np.searchsorted(rowptr, e, side="right") - 1)
+        u = x[2] / 2 ** 32
+        assert cnt[q] == int(np.floor(np.exp(np.log(1000) + u * (np.log(1e6) - np.log(1000)))))
+    assert cnt.min() >= 1000 and cnt.max() < 1000000
+    # log-uniform: log10(count) is uniform on [3, 6)
+    assert abs(np.log10(cnt.astype(float)).mean() - 4.5) < 0.02
+    deg = np.diff(rowptr.astype(np.int64))
+    hits = np.bincount(src, minlength=len(deg))
+    exp = nq * deg / deg.sum()
+    big = exp > 20
+    assert np.all(np.abs(hits[big] - exp[big]) < 6 * np.sqrt(exp[big]))
