@@ -179,7 +179,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=20, help="scorer training epochs (c5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="first run a device-to-device copy of MIB MiB (a kernel of known traffic for "
+                    help="first run copy_probe_kernel over MIB MiB, twice (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -192,14 +192,8 @@ def main():
     wl = WORKLOADS[args.workload]
     info = lt.device_info()
     if args.copy_calibration:
-        import torch
-        a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda")
-        a.fill_(1)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+        gbps = lt.device_copy_bandwidth(args.copy_calibration << 20, reps=1)
+        log(f"copy probe {args.copy_calibration} MiB: {gbps:.0f} GB/s (read + write)")
     t0 = time.perf_counter()
     g = lt.Graph.rmat(wl["scale"], wl["ef"])
     log(f"graph s{wl['scale']} ef{wl['ef']}: V={g.num_node} E={g.num_edge} in {time.perf_counter() - t0:.1f} s")

@@ -263,6 +263,10 @@ int32_t lt_sched_computed_partition(uint32_t P, const uint64_t *counts, const ui
 
 /* ---- device probe -------------------------------------------------------- */
 lt_status lt_device_info(char *name, size_t cap, int *num_cu, uint64_t *hbm_bytes);
+/* Device-to-device copy of `bytes` (16-byte vector loads and stores) by the
+ * kernel copy_probe_kernel, reps times; gbps_out = bytes read + written per
+ * second.  A kernel of known traffic, used to scale HBM traffic counters. */
+lt_status lt_device_copy_bandwidth(uint64_t bytes, uint32_t reps, double *gbps_out);
 
 #ifdef __cplusplus
 }

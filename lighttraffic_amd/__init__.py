@@ -135,6 +135,7 @@ _sig("lt_sched_evict_min", _i32, _u32, _vp, _vp, _i32)
 _sig("lt_sched_computed_partition", _i32, _u32, _vp, _vp, _i32, _u64)
 _sig("lt_device_info", ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
      ctypes.POINTER(_u64))
+_sig("lt_device_copy_bandwidth", ctypes.c_int, _u64, _u32, ctypes.POINTER(ctypes.c_double))
 
 
 def lib():
@@ -168,6 +169,13 @@ def device_info():
     hbm = _u64()
     _check(_lib.lt_device_info(name, 256, ctypes.byref(ncu), ctypes.byref(hbm)), "lt_device_info")
     return {"name": name.value.decode(), "num_cu": ncu.value, "hbm_bytes": hbm.value}
+
+
+def device_copy_bandwidth(nbytes, reps=5):
+    """HBM copy bandwidth (read + written bytes per second) of a plain copy kernel."""
+    gbps = ctypes.c_double()
+    _check(_lib.lt_device_copy_bandwidth(nbytes, reps, ctypes.byref(gbps)), "lt_device_copy_bandwidth")
+    return gbps.value
 
 
 class Graph:

@@ -1155,6 +1155,39 @@ lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, c
     return e->run(rs, nullptr, endpoints_out, nullptr);
 }
 
+lt_status lt_device_copy_bandwidth(uint64_t bytes, uint32_t reps, double *gbps_out) {
+    if (bytes < 16 || reps == 0) return LT_E_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device visible");
+        return LT_E_NODEVICE;
+    }
+    const uint64_t n16 = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    LT_TRY(dmalloc(&a, n16));
+    std::unique_ptr<uint4, void (*)(uint4 *)> ga(a, [](uint4 *x) { (void)hipFree(x); });
+    LT_TRY(dmalloc(&b, n16));
+    std::unique_ptr<uint4, void (*)(uint4 *)> gb(b, [](uint4 *x) { (void)hipFree(x); });
+    fill_u32_kernel<<<1024, 256>>>((uint32_t *)a, n16 * 4, 1u);
+    copy_probe_kernel<<<4096, 256>>>(a, b, n16);  // warm
+    LT_HIP(hipGetLastError());
+    LT_HIP(hipDeviceSynchronize());
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    LT_HIP(hipEventRecord(e0, nullptr));
+    for (uint32_t r = 0; r < reps; r++) copy_probe_kernel<<<4096, 256>>>(a, b, n16);
+    LT_HIP(hipEventRecord(e1, nullptr));
+    LT_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (gbps_out) *gbps_out = 2.0 * 16.0 * (double)n16 * reps / (ms * 1e-3) / 1e9;
+    return LT_OK;
+}
+
 lt_status lt_engine_stats(const lt_engine *e, lt_stats *out) {
     if (!e || !out) return LT_E_ARG;
     *out = e->st;

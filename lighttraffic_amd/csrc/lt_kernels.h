@@ -536,6 +536,12 @@ __global__ void set_table_kernel(const uint64_t **rowptr_tab, const uint32_t **c
     }
 }
 
+// Plain streaming copy of n 16-byte vectors (known HBM traffic: 16n read, 16n written).
+__global__ __launch_bounds__(256) void copy_probe_kernel(const uint4 *src, uint4 *dst, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (size_t i = tid; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;

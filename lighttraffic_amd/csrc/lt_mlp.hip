@@ -190,7 +190,7 @@ __global__ void splitk_reduce_kernel(size_t n, int splits, const float *part, fl
 
 // Weight gradient C[M x N] = A^T B with a long K (rows of the batch): K is cut
 // into slices of kSplitK, each slice writes a partial product, then a fixed-order sum.
-constexpr int kSplitK = 1024;
+constexpr int kSplitK = 256;
 static void gemm_wgrad(hipStream_t s, int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
                        float *partial) {
     const int splits = (K + kSplitK - 1) / kSplitK;
@@ -274,8 +274,9 @@ __global__ void colsum_kernel(int M, int N, const float *dZ, float *part, int ro
 }
 
 static void colsum(hipStream_t s, int M, int N, const float *dZ, float *out, float *partial) {
-    const int splits = (M + kSplitK - 1) / kSplitK;
-    colsum_kernel<<<dim3((N + 255) / 256, splits), 256, 0, s>>>(M, N, dZ, partial, kSplitK);
+    constexpr int kRows = 64;  // rows per slice: many short slices keep every CU busy
+    const int splits = (M + kRows - 1) / kRows;
+    colsum_kernel<<<dim3((N + 255) / 256, splits), 256, 0, s>>>(M, N, dZ, partial, kRows);
     splitk_reduce_kernel<<<(N + 255) / 256, 256, 0, s>>>((size_t)N, splits, partial, out);
 }
 
@@ -496,7 +497,8 @@ lt_status Scorer::ensure_cap(uint32_t n) {
         (st = dalloc(&X, rows * kFeat)) || (st = dalloc(&A1, rows * kHidden)) || (st = dalloc(&A2, rows * kHidden)) ||
         (st = dalloc(&Y, rows * kOut)) || (st = dalloc(&dY, rows * kOut)) || (st = dalloc(&dA1, rows * kHidden)) ||
         (st = dalloc(&dA2, rows * kHidden)) ||
-        (st = dalloc(&partial, ((rows + kSplitK - 1) / kSplitK) * (size_t)kHidden * kHidden)))
+        (st = dalloc(&partial, std::max(((rows + kSplitK - 1) / kSplitK) * (size_t)kHidden * kHidden,
+                                         ((rows + 63) / 64) * (size_t)kHidden))))
         return st;
     cap = n;
     return LT_OK;
@@ -635,7 +637,10 @@ lt_status lt_scorer_loss_grad(lt_scorer *s, uint64_t n, const uint32_t *counts, 
                               const int32_t *next_label, const float *keep_label, const float *wa, const float *wb,
                               float *loss_out, float *grad_out) {
     if (!s || !n || !counts || !resident || !recent || !cur || !w_total || !next_label) return LT_E_ARG;
-    if (n > 4096) return LT_E_ARG;
+    if (n > 512) {
+        set_error("at most 512 decisions per call");
+        return LT_E_ARG;
+    }
     lt_status st = s->ensure_cap((uint32_t)n);
     if (st != LT_OK) return st;
     if ((st = upload_samples(s, 0, (uint32_t)n, counts, resident, recent, cur, w_total, next_label, keep_label, wa, wb,
@@ -654,7 +659,7 @@ lt_status lt_scorer_train(lt_scorer *s, uint64_t n, const uint32_t *counts, cons
                           float lr, uint64_t seed, float *loss_out) {
     if (!s || !n || !counts || !resident || !recent || !cur || !w_total || !next_label || !batch || !epochs)
         return LT_E_ARG;
-    batch = std::min<uint32_t>(batch, (uint32_t)std::min<uint64_t>(n, 4096));
+    batch = std::min<uint32_t>(batch, (uint32_t)std::min<uint64_t>(n, 512));
     lt_status st = s->ensure_cap(batch);
     if (st != LT_OK) return st;
     std::vector<uint64_t> order(n);
