@@ -60,7 +60,9 @@ typedef struct {
     uint64_t num_device_walkers;    /* -b: walkers resident at once (0 = all)     */
     uint64_t zerocopy_threshold;    /* walkers below which a non-resident
                                        partition is read over PCIe; UINT64_MAX =
-                                       reference default partition_bytes/256    */
+                                       default: partition_bytes/256 (reference)
+                                       for GREEDY/FIFO, partition_bytes/128 (byte
+                                       break-even of 64 B PCIe reads) for LEARNED */
     uint64_t seed;                  /* Philox key                                 */
     int32_t  policy;                /* lt_sched_policy                            */
     int32_t  pipeline;              /* 1: compute other partitions while loading  */

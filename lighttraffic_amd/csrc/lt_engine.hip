@@ -336,7 +336,13 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     resident.assign(P, 0);
     free_slots.clear();
     for (uint32_t s = 0; s < nslots; s++) free_slots.push_back(nslots - 1 - s);
-    zc_threshold = cfg.zerocopy_threshold == UINT64_MAX ? part.bytes / 128 / 2 : cfg.zerocopy_threshold;
+    // Default threshold: the reference's partitionSize / cachelineSize / 2
+    // (src/Scheduler.h:295) for the plain policies.  The learned policy uses
+    // the byte break-even instead: a zero-copied walker step reads a rowptr
+    // pair and a column id, two 64-byte PCIe reads, so reading S bytes by DMA
+    // costs the same as S/128 zero-copied walkers.
+    if (cfg.zerocopy_threshold != UINT64_MAX) zc_threshold = cfg.zerocopy_threshold;
+    else zc_threshold = cfg.policy == LT_SCHED_LEARNED ? part.bytes / 128 : part.bytes / 128 / 2;
     LT_HIP(hipHostGetDevicePointer((void **)&zc_rowptr, g->rowptr, 0));
     LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
 
