@@ -24,6 +24,9 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kMaxPartitions = 2900;   // append-stage LDS stays within 64 KiB
 constexpr uint32_t kSliceChunks = 2048;     // chunks of one source queue per launch
 constexpr uint32_t kLostCap = 8192;         // chunks lost to allocation races per launch
+constexpr uint32_t kQStride = 32;           // u32 words per queue length: one 128-byte line each,
+                                            // so appends to different queues do not share a line
+constexpr uint32_t kStatSlots = 256;        // event counters are spread over this many lines
 
 // Walker record: one dwordx4.  The reference's walkers are {current, step}
 // (app/pagerank.cu:8-11) or {current, step, id} (app/genericwalk.cu:6-10);
@@ -42,13 +45,19 @@ struct PoolCtl {
 struct Queues {
     Walker *slab;
     uint32_t *frame;       // [P][maxc] chunk ids, kEmpty when unused
-    uint32_t *qlen;        // [P] walkers queued per partition
+    uint32_t *qlen;        // [P * kQStride] walkers queued in partition p at qlen[p * kQStride]
     uint32_t *free_stack;  // [nchunks]
     PoolCtl *ctl;
     uint32_t *lost;        // [kLostCap]
+    uint32_t *packed;      // [P + 4]: the P queue lengths, then PoolCtl (one D2H per decision)
     uint32_t maxc;         // chunk slots per partition
     uint32_t nchunks;
+    uint32_t P;
 };
+
+__host__ __device__ __forceinline__ uint32_t &qlen_of(const Queues &q, uint32_t p) {
+    return q.qlen[(size_t)p * kQStride];
+}
 
 struct Parts {
     const uint32_t *node_off;  // [P+1]
@@ -76,6 +85,7 @@ struct AppArgs {
     uint32_t nq;
 };
 
+// Event counters: stats[slot * kNumStats + StatSlot], summed over kStatSlots slots.
 enum StatSlot { ST_MOVES = 0, ST_TELEPORTS, ST_VISITS, ST_RESIDENCIES, ST_ZC_MOVES, ST_ZC_READS, ST_DROPPED, kNumStats = 8 };
 
 // Device-side scheduling (the whole decide / load / walk / fix-up loop runs

@@ -199,6 +199,7 @@ struct Engine {
 
     // scheduler primitives
     lt_status read_meta();
+    bool meta_dirty = true;  // q.packed may be older than the queues
     lt_status process(uint32_t p, bool zerocopy, uint64_t walkers_hint);
     void record(uint32_t p, int kind, int32_t loaded, int32_t evicted);
     lt_status evict(uint32_t p);
@@ -346,14 +347,19 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     LT_HIP(hipHostGetDevicePointer((void **)&zc_rowptr, g->rowptr, 0));
     LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
 
-    LT_TRY(dmalloc(&d_meta, P + 4));
+    // queue lengths one 128-byte line apart, PoolCtl on its own line, then the packed copy
+    LT_TRY(dmalloc(&d_meta, (size_t)P * kQStride + kQStride + P + 4));
+    q.qlen = d_meta;
+    q.ctl = (PoolCtl *)(d_meta + (size_t)P * kQStride);
+    q.packed = d_meta + (size_t)P * kQStride + kQStride;
+    q.P = P;
     LT_TRY(dmalloc(&d_ds, 1));
     LT_TRY(dmalloc(&d_slot_of, P));
     LT_TRY(dmalloc(&d_part_in_slot, nslots));
     LT_TRY(dmalloc(&d_resident, P));
     slice = cfg.slice_chunks > 0 ? (uint32_t)cfg.slice_chunks : kSliceChunks;
     LT_HIP(hipHostMalloc((void **)&h_meta, 4 * (P + 4), hipHostMallocDefault));
-    LT_TRY(dmalloc(&d_stats, kNumStats));
+    LT_TRY(dmalloc(&d_stats, (size_t)kNumStats * kStatSlots));
     counts.assign(P, 0);
 
     bool timers = cfg.event_timers != 0;
@@ -387,8 +393,6 @@ lt_status Engine::ensure_pool(uint64_t walkers) {
     LT_TRY(dmalloc(&q.frame, (size_t)P * maxc));
     LT_TRY(dmalloc(&q.free_stack, nchunks));
     LT_TRY(dmalloc(&q.lost, kLostCap));
-    q.qlen = d_meta;
-    q.ctl = (PoolCtl *)(d_meta + P);
     wcap = walkers;
     return LT_OK;
 }
@@ -420,7 +424,12 @@ lt_status Engine::ensure_outputs(const RunSpec &rs, bool want_endpoints, bool wa
 }
 
 lt_status Engine::read_meta() {
-    LT_HIP(hipMemcpyAsync(h_meta, d_meta, 4 * (P + 4), hipMemcpyDeviceToHost, s_compute));
+    if (meta_dirty) {  // the last queue update on s_compute was not followed by a fix-up
+        pack_meta_kernel<<<1, kBlock, 0, s_compute>>>(q);
+        LT_HIP(hipGetLastError());
+        meta_dirty = false;
+    }
+    LT_HIP(hipMemcpyAsync(h_meta, q.packed, 4 * (P + 4), hipMemcpyDeviceToHost, s_compute));
     LT_HIP(hipStreamSynchronize(s_compute));
     const PoolCtl *c = (const PoolCtl *)(h_meta + P);
     if (c->overflow) {
@@ -496,8 +505,7 @@ static void launch_walk_app(const WalkArgs &a, int wpt, int grid, size_t lds, hi
 
 lt_status Engine::launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s) {
     const uint64_t fill = (uint64_t)num_cu * 4 * 6 * 64;  // threads for ~6 waves per SIMD
-    const bool walk_bound = cfg.span == LT_SPAN_RESIDENT && nslots == P;
-    const int wpt = !walk_bound || walkers >= 4 * fill ? 4 : walkers >= 2 * fill ? 2 : 1;
+    const int wpt = walkers >= 4 * fill ? 4 : walkers >= 2 * fill ? 2 : 1;
     const uint64_t bw = (uint64_t)kBlock * wpt;
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + bw - 1) / bw, (uint64_t)walk_grid[wpt]));
     switch (app_kind) {
@@ -506,6 +514,7 @@ lt_status Engine::launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s
         default: launch_walk_app<APP_GENERIC>(a, wpt, grid, lds_bytes(wpt), s); break;
     }
     LT_HIP(hipGetLastError());
+    meta_dirty = true;
     return LT_OK;
 }
 
@@ -553,6 +562,7 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
         fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, p, (uint32_t)lo, last ? 0xFFFFFFFFu : (uint32_t)hi, last ? 1 : 0,
                                                   nullptr);
         LT_HIP(hipGetLastError());
+        meta_dirty = false;
         st.kernel_launches++;
     }
     phase.end(tp, s_compute);
@@ -1003,7 +1013,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         app.nq = rs.nq;
     }
     LT_HIP(hipMemsetAsync(d_counts, 0, 4 * (size_t)V, s_compute));
-    LT_HIP(hipMemsetAsync(d_stats, 0, 8 * kNumStats, s_compute));
+    LT_HIP(hipMemsetAsync(d_stats, 0, 8 * (size_t)kNumStats * kStatSlots, s_compute));
     if (moves_out) LT_HIP(hipMemsetAsync(d_moves, 0, 4 * rs.total, s_compute));
 
     double running_ms = 0;
@@ -1012,6 +1022,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         // createWalkers: untimed, as in the reference (src/LightTraffic.h:33-44)
         reset_queues_kernel<<<1024, 256, 0, s_compute>>>(q, P);
         LT_HIP(hipGetLastError());
+        meta_dirty = true;
         InitArgs ia{};
         ia.app = app;
         ia.q = q;
@@ -1034,6 +1045,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         LT_HIP(hipGetLastError());
         fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, nullptr);
         LT_HIP(hipGetLastError());
+        meta_dirty = false;
         LT_HIP(hipStreamSynchronize(s_compute));
         const auto t0 = std::chrono::steady_clock::now();
         w_run = id1 - id0;
@@ -1053,8 +1065,10 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     if (counts_out) LT_HIP(hipMemcpy(counts_out, d_counts, 4 * (size_t)V, hipMemcpyDeviceToHost));
     if (endpoints_out) LT_HIP(hipMemcpy(endpoints_out, d_endpoints, 4 * rs.total, hipMemcpyDeviceToHost));
     if (moves_out) LT_HIP(hipMemcpy(moves_out, d_moves, 4 * rs.total, hipMemcpyDeviceToHost));
-    unsigned long long hs[kNumStats];
-    LT_HIP(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> slots_h((size_t)kNumStats * kStatSlots);
+    LT_HIP(hipMemcpy(slots_h.data(), d_stats, 8 * slots_h.size(), hipMemcpyDeviceToHost));
+    unsigned long long hs[kNumStats] = {};
+    for (size_t i = 0; i < slots_h.size(); i++) hs[i % kNumStats] += slots_h[i];
     st.running_ms = running_ms;
     st.moves = hs[ST_MOVES];
     st.teleports = hs[ST_TELEPORTS];

@@ -158,7 +158,7 @@ __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, co
     for (uint32_t i = tid; i < P; i += kBlock) {
         const uint32_t c = L.hist[i];
         if (c) {
-            const uint32_t b = atomicAdd(&q.qlen[i], c);
+            const uint32_t b = atomicAdd(&qlen_of(q, i), c);
             const uint32_t k0 = b >> kChunkLog2, k1 = (b + c - 1) >> kChunkLog2;
             const uint32_t c0 = (b & (kChunk - 1)) == 0 ? alloc_chunk(q, i, k0) : kEmpty;
             L.ch0[i] = c0;
@@ -201,14 +201,25 @@ struct Counters {
     uint32_t moves = 0, tele = 0, visits = 0, resid = 0, zc_moves = 0, zc_reads = 0;
 };
 
+// Workgroup sum of the counters, added to slot blockIdx.x % kStatSlots, so
+// that workgroups of one launch do not all update one cache line.
 __device__ void flush_counters(const Counters &c, unsigned long long *stats) {
-    const uint64_t v[6] = {c.moves, c.tele, c.visits, c.resid, c.zc_moves, c.zc_reads};
+    __shared__ unsigned long long s_part[kBlock / 64][6];
+    const uint32_t v[6] = {c.moves, c.tele, c.visits, c.resid, c.zc_moves, c.zc_reads};
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        uint64_t x = v[i];
+        unsigned long long x = v[i];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if ((threadIdx.x & 63u) == 0 && x) atomicAdd(&stats[i], (unsigned long long)x);
+        if (lane == 0) s_part[wave][i] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        unsigned long long x = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; w++) x += s_part[w][threadIdx.x];
+        if (x) atomicAdd(&stats[(size_t)(blockIdx.x % kStatSlots) * kNumStats + threadIdx.x], x);
     }
 }
 
@@ -362,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
             src_sp.col = (const uint32_t *)(base + ((uint64_t)a.pt.node_off[src + 1] - a.pt.node_off[src] + 1) * 8);
         }
     }
-    const uint32_t n = a.q.qlen[src];
+    const uint32_t n = qlen_of(a.q, src);
     const uint64_t lo64 = (uint64_t)chunk_lo << kChunkLog2;
     const uint64_t hi64 = min((uint64_t)n, (uint64_t)chunk_hi << kChunkLog2);
     src_sp.nb = a.pt.node_off[src];
@@ -464,9 +475,24 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
     }
 }
 
+// Copy the queue lengths and PoolCtl into q.packed for the host's one D2H.
+// Collective over one workgroup; runs after every queue update of the stream.
+__device__ void pack_meta(const Queues &q) {
+    for (uint32_t i = threadIdx.x; i < q.P; i += blockDim.x) q.packed[i] = qlen_of(q, i);
+    if (threadIdx.x == 0) {
+        const PoolCtl c = *q.ctl;
+        q.packed[q.P] = c.top;
+        q.packed[q.P + 1] = c.popped;
+        q.packed[q.P + 2] = c.nlost;
+        q.packed[q.P + 3] = c.overflow;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void pack_meta_kernel(Queues q) { pack_meta(q); }
+
 // Return the popped-but-unused state of the pool, the lost chunks and the
 // consumed source chunks [chunk_lo, chunk_hi) to the free stack; clear the
-// source queue after its last slice.  One workgroup.
+// source queue after its last slice; pack the queue lengths.  One workgroup.
 __global__ __launch_bounds__(kBlock) void fixup_kernel(Queues q, uint32_t src, uint32_t chunk_lo, uint32_t chunk_hi,
                                                        int last, const DevSched *ds) {
     __shared__ uint32_t s_top, s_nlost, s_cnt;
@@ -483,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void fixup_kernel(Queues q, uint32_t src, u
         uint32_t nlost = c.nlost < kLostCap ? c.nlost : kLostCap;
         uint32_t cnt = 0;
         if (src != kEmpty) {
-            const uint32_t nch = (q.qlen[src] + kChunk - 1) >> kChunkLog2;
+            const uint32_t nch = (qlen_of(q, src) + kChunk - 1) >> kChunkLog2;
             const uint32_t hi = min(chunk_hi, nch);
             cnt = hi > chunk_lo ? hi - chunk_lo : 0;
         }
@@ -509,8 +535,10 @@ __global__ __launch_bounds__(kBlock) void fixup_kernel(Queues q, uint32_t src, u
         q.ctl->top = top + nlost + cnt;
         q.ctl->popped = 0;
         q.ctl->nlost = 0;
-        if (last && src != kEmpty) q.qlen[src] = 0;
+        if (last && src != kEmpty) qlen_of(q, src) = 0;
     }
+    __syncthreads();
+    pack_meta(q);
 }
 
 // Empty every queue and put all chunks on the free stack.
@@ -519,7 +547,7 @@ __global__ void reset_queues_kernel(Queues q, uint32_t P) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = tid; i < (size_t)P * q.maxc; i += stride) q.frame[i] = kEmpty;
     for (size_t i = tid; i < q.nchunks; i += stride) q.free_stack[i] = (uint32_t)i;
-    for (size_t i = tid; i < P; i += stride) q.qlen[i] = 0;
+    for (size_t i = tid; i < (size_t)P * kQStride; i += stride) q.qlen[i] = 0;
     if (tid == 0) {
         q.ctl->top = q.nchunks;
         q.ctl->popped = 0;
@@ -560,7 +588,7 @@ __global__ __launch_bounds__(1024) void decide_kernel(Queues q, SchedState ss, P
     const uint32_t tid = threadIdx.x, P = pt.P;
     unsigned long long best = 0, victim = ~0ull;
     for (uint32_t p = tid; p < P; p += 1024) {
-        const unsigned long long c = q.qlen[p];
+        const unsigned long long c = qlen_of(q, p);
         const unsigned long long key = (c << 32) | (0xFFFFFFFFull - p);
         if (c > 0 && key > best) best = key;
     }
@@ -590,7 +618,7 @@ __global__ __launch_bounds__(1024) void decide_kernel(Queues q, SchedState ss, P
             if (ss.part_in_slot[s] < 0 && s < fr) fr = s;
         for (uint32_t p = tid; p < P; p += 1024) {
             if (!ss.resident[p] || p == c) continue;
-            const unsigned long long key = ((unsigned long long)q.qlen[p] << 32) | p;
+            const unsigned long long key = ((unsigned long long)qlen_of(q, p) << 32) | p;
             if (key < victim) victim = key;
         }
     }
