@@ -27,6 +27,9 @@ constexpr uint32_t kLostCap = 8192;         // chunks lost to allocation races p
 constexpr uint32_t kQStride = 32;           // u32 words per queue length: one 128-byte line each,
                                             // so appends to different queues do not share a line
 constexpr uint32_t kStatSlots = 256;        // event counters are spread over this many lines
+constexpr uint32_t kCountCopies = 8;        // visit-count copies, one per XCD of an MI355X: workgroups
+                                            // are dispatched to the 8 XCDs round-robin, so workgroup b
+                                            // adds into copy b % 8 and its lines stay in one XCD's L2
 
 // Walker record: one dwordx4.  The reference's walkers are {current, step}
 // (app/pagerank.cu:8-11) or {current, step, id} (app/genericwalk.cu:6-10);
@@ -77,7 +80,8 @@ struct AppArgs {
     uint32_t thresh;      // pagerank: teleport threshold; ppr: stop threshold
     uint32_t length;      // pagerank/genericwalk: l
     uint32_t source;      // single-source ppr
-    uint32_t *counts;     // u32[V] or nullptr
+    uint32_t *counts;     // u32[V] or, with counts_stride = V, u32[kCountCopies][V]; or nullptr
+    uint64_t counts_stride;  // 0: one count array; V: workgroup b counts into copy b % kCountCopies
     uint32_t *endpoints;  // u32[N] indexed by walker id, or nullptr
     uint32_t *moves_out;  // u32[N] (genericwalk), or nullptr
     const uint64_t *q_off;  // batched ppr: [nq+1] first walker id of each query

@@ -164,6 +164,7 @@ struct Engine {
     // outputs and statistics
     uint32_t *d_counts = nullptr;
     size_t counts_cap = 0;
+    uint32_t count_copies = 1;  // 1 or kCountCopies arrays of V counts
     uint32_t *d_endpoints = nullptr;
     size_t endpoints_cap = 0;
     uint32_t *d_moves = nullptr;
@@ -199,7 +200,17 @@ struct Engine {
 
     // scheduler primitives
     lt_status read_meta();
+    lt_status take_event(hipEvent_t &ev) {
+        if (!pool_events.empty()) {
+            ev = pool_events.back();
+            pool_events.pop_back();
+            return LT_OK;
+        }
+        LT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        return LT_OK;
+    }
     bool meta_dirty = true;  // q.packed may be older than the queues
+    hipEvent_t ev_evicted = nullptr;  // table updates of prefetch evictions, awaited by the copy stream
     lt_status process(uint32_t p, bool zerocopy, uint64_t walkers_hint);
     void record(uint32_t p, int kind, int32_t loaded, int32_t evicted);
     lt_status evict(uint32_t p);
@@ -228,6 +239,7 @@ Engine::~Engine() {
     if (s_compute) (void)hipStreamSynchronize(s_compute);
     if (s_copy) (void)hipStreamSynchronize(s_copy);
     for (auto ev : pool_events) (void)hipEventDestroy(ev);
+    if (ev_evicted) (void)hipEventDestroy(ev_evicted);
     dfree(d_node_off);
     dfree(d_edge_off);
     dfree(d_lut);
@@ -347,18 +359,19 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     LT_HIP(hipHostGetDevicePointer((void **)&zc_rowptr, g->rowptr, 0));
     LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
 
-    // queue lengths one 128-byte line apart, PoolCtl on its own line, then the packed copy
-    LT_TRY(dmalloc(&d_meta, (size_t)P * kQStride + kQStride + P + 4));
+    // queue lengths one 128-byte line apart, then PoolCtl on its own line; the
+    // fix-up kernel packs both into h_meta (pinned, mapped) for the host
+    LT_TRY(dmalloc(&d_meta, (size_t)P * kQStride + kQStride));
+    LT_HIP(hipHostMalloc((void **)&h_meta, 4 * (P + 4), hipHostMallocMapped | hipHostMallocCoherent));
+    LT_HIP(hipHostGetDevicePointer((void **)&q.packed, h_meta, 0));
     q.qlen = d_meta;
     q.ctl = (PoolCtl *)(d_meta + (size_t)P * kQStride);
-    q.packed = d_meta + (size_t)P * kQStride + kQStride;
     q.P = P;
     LT_TRY(dmalloc(&d_ds, 1));
     LT_TRY(dmalloc(&d_slot_of, P));
     LT_TRY(dmalloc(&d_part_in_slot, nslots));
     LT_TRY(dmalloc(&d_resident, P));
     slice = cfg.slice_chunks > 0 ? (uint32_t)cfg.slice_chunks : kSliceChunks;
-    LT_HIP(hipHostMalloc((void **)&h_meta, 4 * (P + 4), hipHostMallocDefault));
     LT_TRY(dmalloc(&d_stats, (size_t)kNumStats * kStatSlots));
     counts.assign(P, 0);
 
@@ -398,10 +411,11 @@ lt_status Engine::ensure_pool(uint64_t walkers) {
 }
 
 lt_status Engine::ensure_outputs(const RunSpec &rs, bool want_endpoints, bool want_moves) {
-    if (counts_cap < V) {
+    const size_t ncount = (size_t)V * count_copies;
+    if (counts_cap < ncount) {
         dfree(d_counts);
-        LT_TRY(dmalloc(&d_counts, V));
-        counts_cap = V;
+        LT_TRY(dmalloc(&d_counts, ncount));
+        counts_cap = ncount;
     }
     if (want_endpoints && endpoints_cap < rs.total) {
         dfree(d_endpoints);
@@ -429,8 +443,7 @@ lt_status Engine::read_meta() {
         LT_HIP(hipGetLastError());
         meta_dirty = false;
     }
-    LT_HIP(hipMemcpyAsync(h_meta, q.packed, 4 * (P + 4), hipMemcpyDeviceToHost, s_compute));
-    LT_HIP(hipStreamSynchronize(s_compute));
+    LT_HIP(hipStreamSynchronize(s_compute));  // the last fix-up wrote h_meta
     const PoolCtl *c = (const PoolCtl *)(h_meta + P);
     if (c->overflow) {
         set_error("walker queue pool overflow (flags 0x%x, top %u)", c->overflow, c->top);
@@ -490,12 +503,11 @@ lt_status Engine::load_async(uint32_t p) {
     return LT_OK;
 }
 
-// Walk launch sized to the slice.  When walkers leave the span after a step
-// or two, the append stage dominates and its per-workgroup cost (histogram,
-// scan and one atomic per destination partition) is best spread over 4
-// walkers per thread.  When every partition is resident and walks continue
-// through all of them, nothing is appended and occupancy matters: 1 or 2
-// walkers per thread, so that small queues still fill every SIMD.
+// Walk launch sized to the slice: 1 walker per thread while the slice is
+// small enough that more walkers per thread would leave SIMDs without waves
+// (a thread walks its walkers one after another), 2 or 4 for large slices,
+// where the per-workgroup append cost (histogram, scan, one atomic per
+// destination queue) is spread over more walkers.
 template <int APP>
 static void launch_walk_app(const WalkArgs &a, int wpt, int grid, size_t lds, hipStream_t s) {
     if (wpt == 4) walk_kernel<APP, 4><<<grid, kBlock, lds, s>>>(a);
@@ -724,6 +736,7 @@ lt_status Engine::scheduler_loop_learned() {
     std::vector<uint8_t> prefetched(P, 0);  // resident because of a prefetch, not yet picked
     std::vector<uint8_t> pending(P, 0);     // being copied by a prefetch
     const uint32_t depth = (uint32_t)std::max(0, cfg.prefetch_depth);
+    scores_valid = false;
     sched.begin_run();
     // make completed prefetches resident, in order; with `upto`, wait for it
     auto retire = [&](int32_t upto) -> lt_status {
@@ -776,7 +789,6 @@ lt_status Engine::scheduler_loop_learned() {
         sync = false;
         prev = c;
         st.iterations++;
-        scores_valid = false;
         bool zc = false;
         if (!resident[c] && !pending[c] && counts[c] < zc_threshold) {
             zc = true;
@@ -797,7 +809,7 @@ lt_status Engine::scheduler_loop_learned() {
                 LT_TRY(retire(inflight.empty() ? -1 : inflight.back().part));  // the copy engine is in-order
                 int32_t evicted = -1;
                 if (free_slots.empty()) {
-                    LT_TRY(infer_scores(c));
+                    if (!scores_valid) LT_TRY(infer_scores(c));
                     evicted = learned_victim(c, prev_loaded);
                     if (evicted < 0) {
                         set_error("no partition to evict");
@@ -817,10 +829,38 @@ lt_status Engine::scheduler_loop_learned() {
             }
             prev_loaded = c;
         }
-        // prefetch the predicted next loads while c is walked (compute is idle here)
-        if (scorer && inflight.size() < depth) {
+        // Prefetch the predicted next loads while c is walked.  Slots are
+        // freed before the walk is launched (the walk then never reads them),
+        // with the scores of the previous decision; the targets are chosen
+        // with the scores of this decision, computed while the walk runs.
+        size_t want = scorer && scorer_trained(scorer) && inflight.size() < depth ? depth - inflight.size() : 0;
+        if (want) {  // no more slots than there are partitions a prefetch may take
+            size_t cand = 0;
+            for (uint32_t i = 0; i < P && cand < want; i++)
+                cand += !resident[i] && !pending[i] && (int32_t)i != c && counts[i] >= zc_threshold;
+            want = cand;
+        }
+        if (want) {
             if (!scores_valid) LT_TRY(infer_scores(c));
-            while (scores_valid && inflight.size() < depth) {
+            bool evicted_any = false;
+            while (scores_valid && free_slots.size() < want) {
+                const int32_t v = learned_victim(c, -1);
+                if (v < 0) break;
+                LT_TRY(evict((uint32_t)v));
+                prefetched[v] = 0;
+                evicted_any = true;
+            }
+            if (evicted_any) {  // copies into the freed slots start after the table updates
+                if (!ev_evicted) LT_HIP(hipEventCreateWithFlags(&ev_evicted, hipEventDisableTiming));
+                LT_HIP(hipEventRecord(ev_evicted, s_compute));
+                LT_HIP(hipStreamWaitEvent(s_copy, ev_evicted, 0));
+            }
+        }
+        st.computing_calls++;
+        LT_TRY(process((uint32_t)c, zc, counts[c]));
+        if (want && !free_slots.empty()) {
+            LT_TRY(infer_scores(c));  // while c is walked
+            while (scores_valid && inflight.size() < depth && !free_slots.empty()) {
                 int32_t q = -1;
                 float best = 0.f;
                 for (uint32_t i = 0; i < P; i++) {
@@ -832,20 +872,9 @@ lt_status Engine::scheduler_loop_learned() {
                     }
                 }
                 if (q < 0) break;
-                if (free_slots.empty()) {
-                    const int32_t v = learned_victim(c, q);
-                    if (v < 0) break;
-                    LT_TRY(evict((uint32_t)v));
-                    prefetched[v] = 0;
-                }
                 LT_TRY(load_async((uint32_t)q));
                 hipEvent_t ev;
-                if (!pool_events.empty()) {
-                    ev = pool_events.back();
-                    pool_events.pop_back();
-                } else {
-                    LT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-                }
+                LT_TRY(take_event(ev));
                 LT_HIP(hipEventRecord(ev, s_copy));
                 inflight.push_back({q, ev});
                 pending[q] = 1;
@@ -855,8 +884,6 @@ lt_status Engine::scheduler_loop_learned() {
                 prefetched[q] = 1;
             }
         }
-        st.computing_calls++;
-        LT_TRY(process((uint32_t)c, zc, counts[c]));
     }
     LT_TRY(retire(inflight.empty() ? -1 : inflight.back().part));
     LT_HIP(hipDeviceSynchronize());
@@ -985,6 +1012,10 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     t_wevict.reset();
     t_kernel.reset();
     const uint64_t wave = cfg.num_device_walkers ? std::min<uint64_t>(cfg.num_device_walkers, rs.total) : rs.total;
+    // Per-XCD count copies pay when the walkers of one launch stay in a few
+    // partitions (capped cache, PPR around its sources); with every partition
+    // resident the visits spread over all of V and one array is faster.
+    count_copies = rs.app != APP_GENERIC && (nslots < P || rs.app == APP_PPR) ? kCountCopies : 1;
     LT_TRY(ensure_pool(std::max<uint64_t>(wave, 1)));
     LT_TRY(ensure_outputs(rs, endpoints_out != nullptr, moves_out != nullptr));
 
@@ -995,6 +1026,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     app.length = rs.length;
     app.source = rs.source;
     app.counts = d_counts;
+    app.counts_stride = count_copies > 1 ? V : 0;
     app.endpoints = endpoints_out ? d_endpoints : nullptr;
     app.moves_out = moves_out ? d_moves : nullptr;
     if (rs.app == APP_PAGERANK) app.thresh = prob_threshold(1.0 - (double)0.85f);
@@ -1012,7 +1044,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         app.q_src = d_qsrc;
         app.nq = rs.nq;
     }
-    LT_HIP(hipMemsetAsync(d_counts, 0, 4 * (size_t)V, s_compute));
+    LT_HIP(hipMemsetAsync(d_counts, 0, 4 * (size_t)V * count_copies, s_compute));
     LT_HIP(hipMemsetAsync(d_stats, 0, 8 * (size_t)kNumStats * kStatSlots, s_compute));
     if (moves_out) LT_HIP(hipMemsetAsync(d_moves, 0, 4 * rs.total, s_compute));
 
@@ -1053,6 +1085,13 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         else LT_TRY(cfg.policy == LT_SCHED_LEARNED && scorer ? scheduler_loop_learned() : scheduler_loop());
         const auto t1 = std::chrono::steady_clock::now();
         running_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    if (count_copies > 1) {  // part of the walk's work, so inside the timed region
+        const auto t0 = std::chrono::steady_clock::now();
+        sum_count_copies_kernel<<<std::max(1, 8 * num_cu), 256, 0, s_compute>>>(d_counts, V);
+        LT_HIP(hipGetLastError());
+        LT_HIP(hipStreamSynchronize(s_compute));
+        running_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     LT_TRY(read_meta());
     for (uint32_t i = 0; i < P; i++)

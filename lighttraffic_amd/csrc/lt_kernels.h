@@ -234,11 +234,15 @@ struct Span {
 
 __device__ __forceinline__ uint64_t walker_id(const Walker &w) { return ((uint64_t)w.id_hi << 32) | w.id_lo; }
 
+__device__ __forceinline__ void count_visit(const AppArgs &a, uint32_t v) {
+    atomicAdd(&a.counts[(size_t)(blockIdx.x % kCountCopies) * a.counts_stride + v], 1u);
+}
+
 // One update of app/pagerank.cu:26-44: count the visit, then teleport with
 // probability 0.15 or move to a uniform neighbour.  A vertex without
 // neighbours teleports (the reference divides by zero there).
 __device__ __forceinline__ void step_pagerank(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
-    atomicAdd(&a.counts[w.cur], 1u);
+    count_visit(a, w.cur);
     ct.visits++;
     w.step -= 1;
     if (w.step == 0) return;
@@ -265,7 +269,7 @@ __device__ __forceinline__ void step_pagerank(Walker &w, const Span &sp, const A
 // left; a vertex without neighbours is counted and ends the walk.
 __device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
     if (w.step == 1) {
-        if (a.counts) atomicAdd(&a.counts[w.cur], 1u);
+        if (a.counts) count_visit(a, w.cur);
         ct.visits++;
     }
     w.step -= 1;
@@ -278,7 +282,7 @@ __device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArg
         ct.moves++;
         if (sp.zc) { ct.zc_moves++; ct.zc_reads += 2; }
     } else {
-        if (a.counts) atomicAdd(&a.counts[w.cur], 1u);
+        if (a.counts) count_visit(a, w.cur);
         ct.visits++;
         w.step = 0;
         if (sp.zc) ct.zc_reads++;
@@ -568,6 +572,17 @@ __global__ void set_table_kernel(const uint64_t **rowptr_tab, const uint32_t **c
 __global__ __launch_bounds__(256) void copy_probe_kernel(const uint4 *src, uint4 *dst, uint64_t n) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// counts[v] = sum over the kCountCopies copies of v's count (in place into copy 0).
+__global__ __launch_bounds__(256) void sum_count_copies_kernel(uint32_t *counts, uint32_t V) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kCountCopies; c++) s += counts[c * (size_t)V + v];
+        counts[v] = s;
+    }
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {

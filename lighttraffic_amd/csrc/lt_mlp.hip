@@ -300,22 +300,53 @@ __global__ void round_kernel(size_t n, const float *W, uint16_t *Wb) {
 }
 
 // ---- fused inference: one decision, 8 partitions per workgroup --------------
+// Inference copy of the first two layers, output-major: Wi[j * kFeat + k] =
+// W1[k][j] and Wi[kWiW2 + j * kHidden + k] = W2[k][j], so that the thread of
+// output j reads its weights with 16-byte loads.
+constexpr size_t kWiW2 = (size_t)kHidden * kFeat;
+constexpr size_t kWiSize = kWiW2 + (size_t)kHidden * kHidden;
+
+__global__ void transpose_infer_weights_kernel(const uint16_t *Wb, uint16_t *Wi) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kWiW2) {
+        const size_t j = i / kFeat, k = i % kFeat;
+        Wi[i] = Wb[kOffW1 + k * kHidden + j];
+    } else if (i < kWiSize) {
+        const size_t t = i - kWiW2, j = t / kHidden, k = t % kHidden;
+        Wi[i] = Wb[kOffW2 + k * kHidden + j];
+    }
+}
+
 // The decision state arrives packed: u32 counts[P] | u8 resident[P] | pad to 8 |
-// i32 recent[8] | i32 cur | f32 w_total.  Every workgroup recomputes the max
-// and sum of the counts (P is small), builds the features of its 8 rows and
-// runs the three layers with the BF16 weights.
+// i32 recent[8] | i32 cur | f32 w_total, in host memory mapped for the device;
+// the scores go back the same way.  Every workgroup copies the state into LDS
+// with one round of 16-byte loads, recomputes the max and sum of the counts
+// (P is small), builds the features of its 8 rows and runs the three layers.
+// Each output neuron accumulates its inputs in increasing order in FP32.
 __global__ __launch_bounds__(256) void infer_kernel(uint32_t P, const uint8_t *state, const float *vsh,
-                                                    const float *esh, const uint16_t *Wb, float *Y) {
+                                                    const float *esh, const uint16_t *Wb, const uint16_t *Wi,
+                                                    float *Y) {
     constexpr int R = 8;
+    extern __shared__ __attribute__((aligned(16))) uint8_t s_state[];
     __shared__ float s_x[R][kFeat];
     __shared__ float s_h1[R][kHidden];
     __shared__ float s_h2[R][kHidden];
     __shared__ float s_max[256], s_sum[256];
-    const uint32_t *counts = (const uint32_t *)state;
-    const uint8_t *res = state + 4 * (size_t)P;
-    const int32_t *meta = (const int32_t *)(state + (((size_t)P * 5 + 7) & ~(size_t)7));
-    const uint32_t r0 = blockIdx.x * R;
+    const size_t moff = ((size_t)P * 5 + 7) & ~(size_t)7;
+    const size_t bytes = moff + 4 * (kRecent + 2);
     const int j = threadIdx.x;
+    for (size_t o = 16 * (size_t)j; o < bytes; o += 16 * 256) {
+        if (o + 16 <= bytes) {
+            *(uint4 *)(s_state + o) = *(const uint4 *)(state + o);
+        } else {
+            for (size_t b = o; b < bytes; b++) s_state[b] = state[b];
+        }
+    }
+    __syncthreads();
+    const uint32_t *counts = (const uint32_t *)s_state;
+    const uint8_t *res = s_state + 4 * (size_t)P;
+    const int32_t *meta = (const int32_t *)(s_state + moff);
+    const uint32_t r0 = blockIdx.x * R;
     float mx = 0.f, sm = 0.f;
     for (uint32_t q = j; q < P; q += 256) {
         const float v = (float)counts[q];
@@ -356,25 +387,40 @@ __global__ __launch_bounds__(256) void infer_kernel(uint32_t P, const uint8_t *s
         }
     }
     __syncthreads();
-    const uint16_t *W1 = Wb + kOffW1, *B1 = Wb + kOffB1, *W2 = Wb + kOffW2, *B2 = Wb + kOffB2;
-    const uint16_t *W3 = Wb + kOffW3, *B3 = Wb + kOffB3;
+    const uint16_t *B1 = Wb + kOffB1, *B2 = Wb + kOffB2, *W3 = Wb + kOffW3, *B3 = Wb + kOffB3;
     float a[R];
 #pragma unroll
     for (int r = 0; r < R; r++) a[r] = 0.f;
-    for (int k = 0; k < kFeat; k++) {
-        const float w = bf16_to_f32(W1[k * kHidden + j]);
+    {
+        const uint4 *w1 = (const uint4 *)(Wi + (size_t)j * kFeat);
+        uint4 wv[kFeat / 8];
 #pragma unroll
-        for (int r = 0; r < R; r++) a[r] = fmaf(s_x[r][k], w, a[r]);
+        for (int c = 0; c < kFeat / 8; c++) wv[c] = w1[c];
+#pragma unroll
+        for (int k = 0; k < kFeat; k++) {
+            const uint32_t word = (&wv[k / 8].x)[(k % 8) / 2];
+            const float w = __uint_as_float((k & 1) ? (word & 0xFFFF0000u) : (word << 16));
+#pragma unroll
+            for (int r = 0; r < R; r++) a[r] = fmaf(s_x[r][k], w, a[r]);
+        }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) s_h1[r][j] = fmaxf(a[r] + bf16_to_f32(B1[j]), 0.f);
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; r++) a[r] = 0.f;
-    for (int k = 0; k < kHidden; k++) {
-        const float w = bf16_to_f32(W2[k * kHidden + j]);
+    const uint4 *w2 = (const uint4 *)(Wi + kWiW2 + (size_t)j * kHidden);
+    for (int k0 = 0; k0 < kHidden; k0 += 64) {
+        uint4 wv[8];
 #pragma unroll
-        for (int r = 0; r < R; r++) a[r] = fmaf(s_h1[r][k], w, a[r]);
+        for (int c = 0; c < 8; c++) wv[c] = w2[k0 / 8 + c];
+#pragma unroll
+        for (int kk = 0; kk < 64; kk++) {
+            const uint32_t word = (&wv[kk / 8].x)[(kk % 8) / 2];
+            const float w = __uint_as_float((kk & 1) ? (word & 0xFFFF0000u) : (word << 16));
+#pragma unroll
+            for (int r = 0; r < R; r++) a[r] = fmaf(s_h1[r][k0 + kk], w, a[r]);
+        }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) s_h2[r][j] = fmaxf(a[r] + bf16_to_f32(B2[j]), 0.f);
@@ -408,6 +454,8 @@ struct Scorer {
     std::vector<float> h_vsh, h_esh;
     float *W = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
     uint16_t *Wb = nullptr;
+    uint16_t *Wi = nullptr;     // inference copy of W1, W2 (output-major)
+    uint64_t wver = 0, wi_ver = ~0ull;  // bumped on every weight update; Wi is valid for wi_ver
     uint64_t adam_t = 0;
     bool trained = false;
     // training buffers (capacity in decisions)
@@ -420,10 +468,10 @@ struct Scorer {
     float *partial = nullptr;  // split-K partial products
     float *d_loss = nullptr;
     // inference buffers
-    uint8_t *i_state = nullptr;  // packed decision state (see infer_kernel)
-    float *i_Y = nullptr;
-    uint8_t *h_stage = nullptr;  // pinned staging for one decision's state
-    float *h_Y = nullptr;
+    uint8_t *h_stage = nullptr;  // one decision's packed state, pinned and mapped (see infer_kernel)
+    float *h_Y = nullptr;        // its scores, written by the kernel into pinned mapped memory
+    uint8_t *d_stage = nullptr;  // device views of h_stage and h_Y
+    float *d_Y = nullptr;
     Samples data;
 
     ~Scorer();
@@ -434,8 +482,8 @@ struct Scorer {
 
 Scorer::~Scorer() {
     if (stream) (void)hipStreamSynchronize(stream);
-    void *ptrs[] = {d_vsh, d_esh, W, G, Mo, Vo, Wb, d_counts, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa,
-                    d_wb, X, A1, A2, Y, dY, dA1, dA2, partial, d_loss, i_state, i_Y};
+    void *ptrs[] = {d_vsh, d_esh, W, G, Mo, Vo, Wb, Wi, d_counts, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa,
+                    d_wb, X, A1, A2, Y, dY, dA1, dA2, partial, d_loss};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h_stage) (void)hipHostFree(h_stage);
@@ -458,7 +506,8 @@ lt_status Scorer::init(uint32_t P_, const uint32_t *node_off, const uint64_t *ed
     MLP_HIP(hipMemcpy(d_vsh, h_vsh.data(), 4 * P, hipMemcpyHostToDevice));
     MLP_HIP(hipMemcpy(d_esh, h_esh.data(), 4 * P, hipMemcpyHostToDevice));
     if ((st = dalloc(&W, kParams)) || (st = dalloc(&G, kParams)) || (st = dalloc(&Mo, kParams)) ||
-        (st = dalloc(&Vo, kParams)) || (st = dalloc(&Wb, kParams)) || (st = dalloc(&d_loss, 2)))
+        (st = dalloc(&Vo, kParams)) || (st = dalloc(&Wb, kParams)) || (st = dalloc(&Wi, kWiSize)) ||
+        (st = dalloc(&d_loss, 2)))
         return st;
     // He-style initialisation from a seeded generator; biases zero.
     std::vector<float> w(kParams, 0.f);
@@ -476,9 +525,11 @@ lt_status Scorer::init(uint32_t P_, const uint32_t *node_off, const uint64_t *ed
     MLP_HIP(hipMemset(Vo, 0, 4 * kParams));
     round_kernel<<<(kParams + 255) / 256, 256, 0, stream>>>(kParams, W, Wb);
     MLP_HIP(hipGetLastError());
-    if ((st = dalloc(&i_state, (size_t)P * 5 + 64)) || (st = dalloc(&i_Y, (size_t)P * kOut))) return st;
-    MLP_HIP(hipHostMalloc((void **)&h_stage, (size_t)P * 5 + 64, hipHostMallocDefault));
-    MLP_HIP(hipHostMalloc((void **)&h_Y, sizeof(float) * P * kOut, hipHostMallocDefault));
+    wver++;
+    MLP_HIP(hipHostMalloc((void **)&h_stage, (size_t)P * 5 + 64, hipHostMallocMapped | hipHostMallocCoherent));
+    MLP_HIP(hipHostMalloc((void **)&h_Y, sizeof(float) * P * kOut, hipHostMallocMapped | hipHostMallocCoherent));
+    MLP_HIP(hipHostGetDevicePointer((void **)&d_stage, h_stage, 0));
+    MLP_HIP(hipHostGetDevicePointer((void **)&d_Y, h_Y, 0));
     MLP_HIP(hipStreamSynchronize(stream));
     return LT_OK;
 }
@@ -532,6 +583,7 @@ lt_status Scorer::forward_backward(uint32_t nb, bool update, float lr) {
         const float b1t = powf(0.9f, (float)adam_t), b2t = powf(0.999f, (float)adam_t);
         adam_kernel<<<(kParams + 255) / 256, 256, 0, s>>>(kParams, W, Wb, G, Mo, Vo, lr, b1t, b2t);
         MLP_HIP(hipGetLastError());
+        wver++;
     }
     return LT_OK;
 }
@@ -563,10 +615,14 @@ lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float 
     float wt = d.w_total;
     memcpy(&meta[kRecent + 1], &wt, 4);
     const size_t bytes = moff + 4 * (kRecent + 2);
-    MLP_HIP(hipMemcpyAsync(s->i_state, h, bytes, hipMemcpyHostToDevice, st));
-    infer_kernel<<<(P + 7) / 8, 256, 0, st>>>(P, s->i_state, s->d_vsh, s->d_esh, s->Wb, s->i_Y);
+    if (s->wi_ver != s->wver) {
+        transpose_infer_weights_kernel<<<(unsigned)((kWiSize + 255) / 256), 256, 0, st>>>(s->Wb, s->Wi);
+        MLP_HIP(hipGetLastError());
+        s->wi_ver = s->wver;
+    }
+    infer_kernel<<<(P + 7) / 8, 256, (bytes + 15) & ~(size_t)15, st>>>(P, s->d_stage, s->d_vsh, s->d_esh, s->Wb,
+                                                                         s->Wi, s->d_Y);
     MLP_HIP(hipGetLastError());
-    MLP_HIP(hipMemcpyAsync(s->h_Y, s->i_Y, sizeof(float) * P * kOut, hipMemcpyDeviceToHost, st));
     MLP_HIP(hipStreamSynchronize(st));
     memcpy(out, s->h_Y, sizeof(float) * P * kOut);
     return LT_OK;
@@ -737,6 +793,7 @@ lt_status lt_scorer_set_params(lt_scorer *s, const float *in) {
     MLP_HIP(hipMemcpy(s->W, in, 4 * kParams, hipMemcpyHostToDevice));
     round_kernel<<<(kParams + 255) / 256, 256, 0, s->stream>>>(kParams, s->W, s->Wb);
     MLP_HIP(hipGetLastError());
+    s->wver++;
     MLP_HIP(hipStreamSynchronize(s->stream));
     s->trained = true;
     return LT_OK;
