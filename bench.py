@@ -217,6 +217,11 @@ def main():
             eng = lt.Engine(g, part, span=span, record_trace=1, **base)
             plain[span] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}")
             engines[span] = eng
+        # plain rules with the learned policy's non-learned settings: byte break-even
+        # zero-copy threshold and zero-copy batches
+        eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16, **base)
+        plain["tuned"] = measure(eng, wl, w, args.warmup, args.steps, "plain/resident S/128 zc-batch 16")
+        eng.close()
         t1 = time.perf_counter()
         sc = lt.Scorer(part, seed=1)
         n_next = sc.add_trace(engines["partition"], use_next=True, use_keep=True)
@@ -240,7 +245,8 @@ def main():
         out.update({
             "value": learned["steps_per_s"],
             "ms_per_step": learned["ms_per_step"],
-            "config": {"workload": desc, "scheduler": "learned (prefetch depth 2, span resident)",
+            "config": {"workload": desc, "scheduler": "learned (prefetch depth 2, span resident, zero-copy "
+                                                      "threshold S/128, zero-copy batches of 16)",
                        "device": info["name"]},
             "h2d_bytes_per_step": learned["h2d_bytes_per_step"],
             "plain": {"span_partition": {k: ref_plain[k] for k in ("steps_per_s", "ms_per_step",
@@ -248,14 +254,20 @@ def main():
                                                                     "pcie_GBps")},
                       "span_resident": {k: plain["resident"][k] for k in ("steps_per_s", "ms_per_step",
                                                                          "h2d_bytes_per_step", "h2d_GB_per_job",
-                                                                         "pcie_GBps")}},
+                                                                         "pcie_GBps")},
+                      "span_resident_S128_zcbatch16": {k: plain["tuned"][k] for k in (
+                          "steps_per_s", "ms_per_step", "h2d_bytes_per_step", "h2d_GB_per_job", "pcie_GBps")}},
             "learned": {k: learned[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
                                                 "h2d_GB_per_job", "pcie_GBps")},
             "learned_vs_plain": {
                 "speedup_vs_reference_plain": learned["steps_per_s"] / ref_plain["steps_per_s"],
                 "h2d_bytes_ratio_vs_reference_plain": learned["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"],
+                "speedup_vs_plain_resident": learned["steps_per_s"] / plain["resident"]["steps_per_s"],
+                "h2d_bytes_ratio_vs_plain_resident": (learned["h2d_bytes_per_step"] /
+                                                      plain["resident"]["h2d_bytes_per_step"]),
                 "speedup_vs_best_plain": learned["steps_per_s"] / best_plain["steps_per_s"],
                 "h2d_bytes_ratio_vs_best_plain": learned["h2d_bytes_per_step"] / best_plain["h2d_bytes_per_step"],
+                "best_plain": [k for k, v in plain.items() if v is best_plain][0],
                 "visit_counts_identical": same,
                 "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
                 "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],
@@ -273,7 +285,7 @@ def main():
         log(f"autotuner picked {best} in {tune_s:.1f} s")
         part = lt.Partition.build(g, best["partition_mb"] << 20)
         cfg = dict(num_device_partitions=best["slots"], span=best["span"], device_schedule=best["device_schedule"],
-                   slice_chunks=best["slice_chunks"])
+                   slice_chunks=best["slice_chunks"], zc_batch=best["zc_batch"])
         eng = lt.Engine(g, part, **cfg)
         # walk-kernel time for the roofline: the same partitioning, host-scheduled, event timers on
         teng = lt.Engine(g, part, **dict(cfg, device_schedule=0), event_timers=1)
@@ -283,7 +295,8 @@ def main():
                 f"{wl['nq']} queries (sources ~ degree, counts log-uniform [1e3,1e6), {int(cnt.sum())} walkers), "
                 f"p=0.15, 8 GB cache; autotuned: {best['partition_mb']} MB partitions (P={part.count}), "
                 f"{best['slots']} slots, span {best['span']}, "
-                f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}")
+                f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}, "
+                f"zero-copy batch {best['zc_batch']}")
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, "c4", best["span"]),

@@ -77,7 +77,11 @@ typedef struct {
                                        per hipGraph launch (no host round trip) */
     int32_t  slice_chunks;          /* chunks of 4096 walkers per walk launch
                                        (0 = 2048); device_schedule: unused      */
-    int32_t  reserved[3];
+    int32_t  zc_batch;              /* span RESIDENT: walk up to this many
+                                       partitions below the zero-copy threshold
+                                       over PCIe in one launch (at most 16;
+                                       0 = 16 for LEARNED, 1 otherwise)         */
+    int32_t  reserved[2];
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines

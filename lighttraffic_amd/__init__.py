@@ -50,7 +50,8 @@ class Config(ctypes.Structure):
     _fields_ = [("num_device_partitions", _u32), ("num_device_walkers", _u64), ("zerocopy_threshold", _u64),
                 ("seed", _u64), ("policy", _i32), ("pipeline", _i32), ("span", _i32), ("event_timers", _i32),
                 ("record_trace", _i32), ("prefetch_depth", _i32), ("prefetch_stick_pct", _i32),
-                ("device_schedule", _i32), ("slice_chunks", _i32), ("reserved", _i32 * 3)]
+                ("device_schedule", _i32), ("slice_chunks", _i32), ("zc_batch", _i32),
+                ("reserved", _i32 * 2)]
 
 
 class Stats(ctypes.Structure):
@@ -154,7 +155,10 @@ def _ptr(a):
 def default_config(**kw):
     c = Config()
     _lib.lt_config_default(ctypes.byref(c))
+    names = {f for f, _ in Config._fields_ if f != "reserved"}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"unknown engine option {k!r}")
         if k == "policy" and isinstance(v, str):
             v = SCHED[v]
         if k == "span" and isinstance(v, str):

@@ -27,6 +27,7 @@ constexpr uint32_t kLostCap = 8192;         // chunks lost to allocation races p
 constexpr uint32_t kQStride = 32;           // u32 words per queue length: one 128-byte line each,
                                             // so appends to different queues do not share a line
 constexpr uint32_t kStatSlots = 256;        // event counters are spread over this many lines
+constexpr uint32_t kMaxZcBatch = 16;        // partitions walked from host memory by one launch
 constexpr uint32_t kCountCopies = 8;        // visit-count copies, one per XCD of an MI355X: workgroups
                                             // are dispatched to the 8 XCDs round-robin, so workgroup b
                                             // adds into copy b % 8 and its lines stay in one XCD's L2
@@ -68,6 +69,7 @@ struct Parts {
     const uint32_t *lut;       // partition of vertex (j << lut_shift)
     const uint64_t *const *rowptr_tab;  // [P] HBM slot rowptr, nullptr if not resident
     const uint32_t *const *col_tab;     // [P]
+    const uint8_t *zc_part;             // [P] 1 while the table points p at host memory (zero-copy batch)
     uint32_t lut_shift;
     uint32_t P;
 };
@@ -132,6 +134,22 @@ struct WalkArgs {
     int zerocopy;         // src slice is read over PCIe
     int span_resident;    // keep walking through HBM-resident partitions
     unsigned long long *stats;
+    // Zero-copy batch (span_resident only): nsrc > 0 consumes the whole queues
+    // of srcs[0..nsrc), all read over PCIe and entered in the partition table
+    // for the launch, so no walker is appended to a queue being consumed.
+    // Queue k occupies [voff[k], voff[k+1]) of one virtual range, each start
+    // a multiple of kChunk.
+    uint32_t nsrc;
+    uint32_t srcs[kMaxZcBatch];
+    uint64_t voff[kMaxZcBatch + 1];
+};
+
+// Table entries of a zero-copy batch.
+struct ZcBatch {
+    uint32_t n;
+    uint32_t part[kMaxZcBatch];
+    const uint64_t *rowptr[kMaxZcBatch];
+    const uint32_t *col[kMaxZcBatch];
 };
 
 struct InitArgs {

@@ -184,6 +184,8 @@ struct Engine {
     DevSched *d_ds = nullptr;
     int32_t *d_slot_of = nullptr, *d_part_in_slot = nullptr;
     uint8_t *d_resident = nullptr;
+    uint8_t *d_zc_part = nullptr;  // [P] partitions of the running zero-copy batch
+    uint32_t zc_batch = 1;         // partitions per zero-copy launch
     uint32_t slice = kSliceChunks;
     std::vector<hipEvent_t> pool_events;  // reusable prefetch-completion events
     std::vector<float> scores;            // scorer output of the current decision
@@ -212,6 +214,8 @@ struct Engine {
     bool meta_dirty = true;  // q.packed may be older than the queues
     hipEvent_t ev_evicted = nullptr;  // table updates of prefetch evictions, awaited by the copy stream
     lt_status process(uint32_t p, bool zerocopy, uint64_t walkers_hint);
+    lt_status process_zc(uint32_t c, const uint8_t *pending);
+    lt_status process_zc_batch(const std::vector<uint32_t> &parts);
     void record(uint32_t p, int kind, int32_t loaded, int32_t evicted);
     lt_status evict(uint32_t p);
     lt_status load_async(uint32_t p);
@@ -262,6 +266,7 @@ Engine::~Engine() {
     dfree(d_slot_of);
     dfree(d_part_in_slot);
     dfree(d_resident);
+    dfree(d_zc_part);
     if (s_compute) (void)hipStreamDestroy(s_compute);
     if (s_copy) (void)hipStreamDestroy(s_copy);
 }
@@ -371,6 +376,12 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     LT_TRY(dmalloc(&d_slot_of, P));
     LT_TRY(dmalloc(&d_part_in_slot, nslots));
     LT_TRY(dmalloc(&d_resident, P));
+    LT_TRY(dmalloc(&d_zc_part, P));
+    LT_HIP(hipMemset(d_zc_part, 0, P));
+    zc_batch = cfg.zc_batch > 0 ? (uint32_t)cfg.zc_batch
+                                : (cfg.policy == LT_SCHED_LEARNED && cfg.span == LT_SPAN_RESIDENT ? kMaxZcBatch : 1);
+    zc_batch = std::min(zc_batch, kMaxZcBatch);
+    if (cfg.span != LT_SPAN_RESIDENT) zc_batch = 1;  // a batch relies on walking through the table
     slice = cfg.slice_chunks > 0 ? (uint32_t)cfg.slice_chunks : kSliceChunks;
     LT_TRY(dmalloc(&d_stats, (size_t)kNumStats * kStatSlots));
     counts.assign(P, 0);
@@ -543,6 +554,7 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
     a.pt.lut = d_lut;
     a.pt.rowptr_tab = d_rowptr_tab;
     a.pt.col_tab = d_col_tab;
+    a.pt.zc_part = d_zc_part;
     a.pt.lut_shift = lut_shift;
     a.pt.P = P;
     a.src = p;
@@ -578,6 +590,84 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
         st.kernel_launches++;
     }
     phase.end(tp, s_compute);
+    return LT_OK;
+}
+
+// Walk the queue of c over PCIe; with zc_batch > 1 also the next largest
+// queues below the threshold (not resident, not being copied), in one launch.
+lt_status Engine::process_zc(uint32_t c, const uint8_t *pending) {
+    std::vector<uint32_t> batch{c};
+    if (zc_batch > 1) {
+        std::vector<uint32_t> cand;
+        for (uint32_t i = 0; i < P; i++)
+            if (i != c && !resident[i] && !(pending && pending[i]) && counts[i] > 0 && counts[i] < zc_threshold)
+                cand.push_back(i);
+        const size_t take = std::min<size_t>(cand.size(), zc_batch - 1);
+        std::partial_sort(cand.begin(), cand.begin() + take, cand.end(), [&](uint32_t x, uint32_t y) {
+            return counts[x] != counts[y] ? counts[x] > counts[y] : x < y;
+        });
+        uint64_t chunks = (counts[c] + kChunk - 1) / kChunk;
+        for (size_t k = 0; k < take; k++) {
+            const uint64_t ch = (counts[cand[k]] + kChunk - 1) / kChunk;
+            if (chunks + ch > slice) break;  // the pool holds at most `slice` consumed chunks per launch
+            chunks += ch;
+            batch.push_back(cand[k]);
+        }
+    }
+    for (uint32_t p : batch) record(p, 1, -1, -1);
+    st.zerocopy_calls += batch.size();
+    if (batch.size() == 1) return process(c, true, counts[c]);
+    return process_zc_batch(batch);
+}
+
+// One launch over the queues of `parts`, all read from the pinned host graph.
+// The partitions are entered in the partition table as host-memory views for
+// the launch (and flagged in zc_part for the PCIe byte count), so a walker
+// that moves between them keeps walking and no queue being consumed is
+// appended to; every queue is then consumed whole.
+lt_status Engine::process_zc_batch(const std::vector<uint32_t> &parts) {
+    WalkArgs a{};
+    a.app = app;
+    a.q = q;
+    a.pt.node_off = d_node_off;
+    a.pt.edge_off = d_edge_off;
+    a.pt.lut = d_lut;
+    a.pt.rowptr_tab = d_rowptr_tab;
+    a.pt.col_tab = d_col_tab;
+    a.pt.zc_part = d_zc_part;
+    a.pt.lut_shift = lut_shift;
+    a.pt.P = P;
+    a.span_resident = 1;
+    a.stats = d_stats;
+    a.ss = sched_state(false);
+    ZcBatch b{};
+    b.n = (uint32_t)parts.size();
+    a.nsrc = b.n;
+    a.src = parts[0];
+    a.voff[0] = 0;
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < b.n; k++) {
+        const uint32_t p = parts[k];
+        b.part[k] = p;
+        b.rowptr[k] = zc_rowptr + part.node_off[p];
+        b.col[k] = zc_col + part.edge_off[p];
+        a.srcs[k] = p;
+        a.voff[k + 1] = a.voff[k] + (counts[p] + kChunk - 1) / kChunk * kChunk;
+        total += counts[p];
+    }
+    hipEvent_t tp = t_zc.begin(s_compute);
+    set_zc_tables_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, d_zc_part, b, 1);
+    LT_HIP(hipGetLastError());
+    hipEvent_t tk = t_kernel.begin(s_compute);
+    LT_TRY(launch_walk(a, total, s_compute));
+    t_kernel.end(tk, s_compute);
+    for (uint32_t p : parts) fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, p, 0, 0xFFFFFFFFu, 1, nullptr);
+    LT_HIP(hipGetLastError());
+    meta_dirty = false;
+    set_zc_tables_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, d_zc_part, b, 0);
+    LT_HIP(hipGetLastError());
+    st.kernel_launches++;
+    t_zc.end(tp, s_compute);
     return LT_OK;
 }
 
@@ -647,11 +737,7 @@ lt_status Engine::run_partition(uint32_t p, int32_t &prev_loaded) {
     st.iterations++;
     const uint64_t n = counts[p];
     // shouldZerocopy (src/Scheduler.h:243-252): counts were read this iteration.
-    if (!resident[p] && n < zc_threshold) {
-        st.zerocopy_calls++;
-        record(p, 1, -1, -1);
-        return process(p, true, n);
-    }
+    if (!resident[p] && n < zc_threshold) return process_zc(p, nullptr);
     st.explicit_iterations++;
     // graphLoadingPhase (src/Scheduler.h:173-213)
     if (!resident[p]) {
@@ -792,8 +878,6 @@ lt_status Engine::scheduler_loop_learned() {
         bool zc = false;
         if (!resident[c] && !pending[c] && counts[c] < zc_threshold) {
             zc = true;
-            st.zerocopy_calls++;
-            record((uint32_t)c, 1, -1, -1);
         } else {
             st.explicit_iterations++;
             if (pending[c]) {
@@ -857,7 +941,7 @@ lt_status Engine::scheduler_loop_learned() {
             }
         }
         st.computing_calls++;
-        LT_TRY(process((uint32_t)c, zc, counts[c]));
+        LT_TRY(zc ? process_zc((uint32_t)c, pending.data()) : process((uint32_t)c, false, counts[c]));
         if (want && !free_slots.empty()) {
             LT_TRY(infer_scores(c));  // while c is walked
             while (scores_valid && inflight.size() < depth && !free_slots.empty()) {
@@ -936,6 +1020,7 @@ lt_status Engine::scheduler_loop_device() {
     a.pt.lut = d_lut;
     a.pt.rowptr_tab = d_rowptr_tab;
     a.pt.col_tab = d_col_tab;
+    a.pt.zc_part = d_zc_part;
     a.pt.lut_shift = lut_shift;
     a.pt.P = P;
     a.span_resident = cfg.span == LT_SPAN_RESIDENT ? 1 : 0;
@@ -1063,6 +1148,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         ia.pt.lut = d_lut;
         ia.pt.rowptr_tab = d_rowptr_tab;
         ia.pt.col_tab = d_col_tab;
+        ia.pt.zc_part = d_zc_part;
         ia.pt.lut_shift = lut_shift;
         ia.pt.P = P;
         ia.id_lo = id0;

@@ -344,8 +344,39 @@ __device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, uint32_t src, c
         sp.nb = a.pt.node_off[p];
         sp.ne = a.pt.node_off[p + 1];
         sp.eb = a.pt.edge_off[p];
-        sp.zc = 0;
+        sp.zc = a.pt.zc_part[p];
     }
+}
+
+// One workgroup pass: walkers [base, base + kBlock * WPT) of src's queue,
+// clipped to hi64; base is a multiple of kBlock * WPT, so the pass reads one
+// chunk.
+template <int APP, int WPT>
+__device__ __forceinline__ void walk_pass(const WalkArgs &a, const AppendLds &L, uint32_t *s_wsum, uint32_t src,
+                                          const Span &src_sp, uint64_t base, uint64_t hi64, Counters &ct) {
+    const uint32_t chunk = a.q.frame[(size_t)src * a.q.maxc + (uint32_t)(base >> kChunkLog2)];
+    const bool chunk_ok = chunk < a.q.nchunks;
+    if (!chunk_ok && threadIdx.x == 0) atomicOr(&a.q.ctl->overflow, 8u);
+    const Walker *cw = a.q.slab + (size_t)(chunk_ok ? chunk : 0) * kChunk + (uint32_t)(base & (kChunk - 1));
+    Walker w[WPT];
+    uint32_t dst[WPT];
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        dst[k] = kEmpty;
+        if (chunk_ok && i < hi64) {
+            w[k] = cw[k * kBlock + threadIdx.x];
+            ct.resid++;
+        } else {
+            w[k].step = 0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (chunk_ok && i < hi64) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
+    }
+    block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum);
 }
 
 // WPT walkers per thread per pass (1, 2 or 4): small queues use 1 so that
@@ -356,6 +387,26 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ uint32_t s_wsum[kBlock / 64];
     const AppendLds L = carve_lds(smem, a.pt.P, WPT);
+    Counters ct;
+    if (a.nsrc) {  // zero-copy batch: every source is read over PCIe
+        const uint64_t vend = a.voff[a.nsrc];
+        for (uint64_t base = (uint64_t)blockIdx.x * BW; base < vend; base += (uint64_t)gridDim.x * BW) {
+            uint32_t k = 0;
+            while (k + 1 < a.nsrc && base >= a.voff[k + 1]) k++;
+            const uint32_t src = a.srcs[k];
+            Span sp;
+            sp.rowptr = a.ss.zc_rowptr + a.pt.node_off[src];
+            sp.col = a.ss.zc_col + a.pt.edge_off[src];
+            sp.nb = a.pt.node_off[src];
+            sp.ne = a.pt.node_off[src + 1];
+            sp.eb = a.pt.edge_off[src];
+            sp.zc = 1;
+            const uint64_t local = base - a.voff[k], n = qlen_of(a.q, src);
+            if (local < n) walk_pass<APP, WPT>(a, L, s_wsum, src, sp, local, n, ct);  // uniform per workgroup
+        }
+        flush_counters(ct, a.stats);
+        return;
+    }
     uint32_t src = a.src, chunk_lo = a.chunk_lo, chunk_hi = a.chunk_hi;
     Span src_sp;
     src_sp.rowptr = a.rowptr;
@@ -383,32 +434,8 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
     src_sp.nb = a.pt.node_off[src];
     src_sp.ne = a.pt.node_off[src + 1];
     src_sp.eb = a.pt.edge_off[src];
-    Counters ct;
-    for (uint64_t base = lo64 + (uint64_t)blockIdx.x * BW; base < hi64; base += (uint64_t)gridDim.x * BW) {
-        const uint32_t chunk = a.q.frame[(size_t)src * a.q.maxc + (uint32_t)(base >> kChunkLog2)];
-        const bool chunk_ok = chunk < a.q.nchunks;
-        if (!chunk_ok && threadIdx.x == 0) atomicOr(&a.q.ctl->overflow, 8u);
-        const Walker *cw = a.q.slab + (size_t)(chunk_ok ? chunk : 0) * kChunk + (uint32_t)(base & (kChunk - 1));
-        Walker w[WPT];
-        uint32_t dst[WPT];
-#pragma unroll
-        for (int k = 0; k < WPT; k++) {
-            const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-            dst[k] = kEmpty;
-            if (chunk_ok && i < hi64) {
-                w[k] = cw[k * kBlock + threadIdx.x];
-                ct.resid++;
-            } else {
-                w[k].step = 0;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < WPT; k++) {
-            const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-            if (chunk_ok && i < hi64) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
-        }
-        block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum);
-    }
+    for (uint64_t base = lo64 + (uint64_t)blockIdx.x * BW; base < hi64; base += (uint64_t)gridDim.x * BW)
+        walk_pass<APP, WPT>(a, L, s_wsum, src, src_sp, base, hi64, ct);
     flush_counters(ct, a.stats);
 }
 
@@ -557,6 +584,18 @@ __global__ void reset_queues_kernel(Queues q, uint32_t P) {
         q.ctl->popped = 0;
         q.ctl->nlost = 0;
         q.ctl->overflow = 0;
+    }
+}
+
+// Enter (on = 1) or remove a zero-copy batch's partitions in the partition table.
+__global__ void set_zc_tables_kernel(const uint64_t **rowptr_tab, const uint32_t **col_tab, uint8_t *zc_part,
+                                     ZcBatch b, int on) {
+    const uint32_t k = threadIdx.x;
+    if (blockIdx.x == 0 && k < b.n) {
+        const uint32_t p = b.part[k];
+        rowptr_tab[p] = on ? b.rowptr[k] : nullptr;
+        col_tab[p] = on ? b.col[k] : nullptr;
+        zc_part[p] = on ? 1 : 0;
     }
 }
 

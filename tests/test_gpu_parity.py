@@ -64,6 +64,13 @@ CONFIGS = [
     ("device-sched-zc", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "zerocopy_threshold": 2000,
                                    "span": "resident"}),
     ("small-slices", 16 << 10, {"num_device_partitions": 4, "slice_chunks": 1, "num_device_walkers": 9000}),
+    # zero-copy batches: several sub-threshold queues walked over PCIe in one launch
+    ("zc-batch", 16 << 10, {"num_device_partitions": 3, "span": "resident", "zerocopy_threshold": 4000,
+                            "zc_batch": 16}),
+    ("zc-batch-all", 16 << 10, {"num_device_partitions": 2, "span": "resident", "zerocopy_threshold": 1 << 40,
+                                "zc_batch": 5, "slice_chunks": 2}),
+    ("learned-fallback-batch", 16 << 10, {"num_device_partitions": 3, "span": "resident", "policy": "learned",
+                                          "zerocopy_threshold": 3000}),
 ]
 
 
@@ -137,7 +144,8 @@ def test_dangling_vertices(kw):
     assert np.array_equal(counts, rc) and np.array_equal(ends, re)
 
 
-@pytest.mark.parametrize("kw", [{}, {"device_schedule": 1}, {"device_schedule": 1, "span": "resident"}])
+@pytest.mark.parametrize("kw", [{}, {"device_schedule": 1}, {"device_schedule": 1, "span": "resident"},
+                                {"span": "resident", "zerocopy_threshold": 5000, "zc_batch": 16}])
 def test_ppr_batch_bit_exact(g12, kw):
     log(f"ppr batch {kw}")
     part = lt.Partition.build(g12, 16 << 10)
@@ -149,6 +157,21 @@ def test_ppr_batch_bit_exact(g12, kw):
     ends = eng.ppr_batch(sources, qc, prob=0.15)
     re, _ = oracle.ppr_batch(g12.rowptr(), g12.col(), sources, qc, prob=0.15)
     assert np.array_equal(ends, re)
+
+
+def test_autotuner_picks_a_trial():
+    """The batched-PPR autotuner tries every configuration on a sample and
+    returns the fastest; every trial walks the same queries."""
+    from lighttraffic_amd.tune import tune_ppr_batch
+    log("autotuner")
+    g = lt.Graph.rmat(14, 8)
+    src, cnt = g.ppr_queries(256, lo=100, hi=10000, seed=5)
+    best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=48 << 20, sample_every=4, sizes_mb=(16,))
+    # host scheduling: span partition x 2 slices + span resident x 2 slices x 2 batch sizes;
+    # device scheduling: 2 spans (whole queues, no batches)
+    assert len(trials) == 2 + 4 + 2
+    assert best in trials and best["running_ms"] == min(t["running_ms"] for t in trials)
+    assert {t["zc_batch"] for t in trials} == {1, 16}
 
 
 def test_schedule_independence(g12):
