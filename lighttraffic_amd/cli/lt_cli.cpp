@@ -96,6 +96,9 @@ int main(int argc, char **argv) {
     cfg.span = c.span == "resident" ? LT_SPAN_RESIDENT : LT_SPAN_PARTITION;
     cfg.event_timers = c.timers ? 1 : 0;
     if (c.has_zc) cfg.zerocopy_threshold = c.zc;
+    if (c.prefetch_depth >= 0) cfg.prefetch_depth = c.prefetch_depth;
+    cfg.zc_batch = c.zc_batch;
+    cfg.device_schedule = c.device_schedule ? 1 : 0;
     lt_engine *e = nullptr;
     s = lt_engine_create(g, part, &cfg, &e);
     if (s == LT_E_ARG && P > 2900) {
@@ -104,6 +107,12 @@ int main(int argc, char **argv) {
         return 1;
     }
     if (s != LT_OK) return fail(s, "engine");
+    lt_scorer *scorer = nullptr;
+    if (!c.scorer.empty()) {
+        if ((s = lt_scorer_create(part, 1, &scorer)) != LT_OK) return fail(s, "scorer");
+        if ((s = lt_scorer_load(scorer, c.scorer.c_str())) != LT_OK) return fail(s, "scorer weights");
+        if ((s = lt_engine_set_scorer(e, scorer)) != LT_OK) return fail(s, "scorer");
+    }
     printf("number of partitons: %u, graph memory pool: %lu MB, walker memory pool: %lu MB\n", P,
            (unsigned long)(lt_partition_bytes(part) * pdev / 1024 / 1024),
            (unsigned long)(std::min(wdev, total) * 16 / 1024 / 1024));
@@ -182,6 +191,7 @@ int main(int argc, char **argv) {
                (unsigned long long)st.graph_loading_calls);
     }
     lt_engine_destroy(e);
+    if (scorer) lt_scorer_free(scorer);
     lt_partition_free(part);
     lt_graph_free(g);
     return rc;

@@ -13,8 +13,10 @@
 //   --prob <p>                  ppr stop probability       (default: 0.15)
 //   -m/--materialized-partition build <file>.part instead of reading it
 // Flags added by this engine (they only add behaviour):
-//   --scheduler greedy|fifo|learned, --no-pipeline, --zerocopy-threshold <n>,
-//   --span partition|resident, --seed <n>, --timers, --json, --queries <file>
+//   --scheduler greedy|fifo|learned, --scorer <weights file> (learned),
+//   --no-pipeline, --zerocopy-threshold <n>, --span partition|resident,
+//   --prefetch-depth <n>, --zc-batch <n>, --device-schedule, --seed <n>,
+//   --timers, --json, --queries <file>
 #pragma once
 #include <stdint.h>
 #include <stdio.h>
@@ -36,6 +38,10 @@ struct CliConfig {
     float prob = 0.15f;
     bool materialize = false;
     std::string scheduler = "greedy";
+    std::string scorer;      // trained scorer weights (lt_scorer_save) for --scheduler learned
+    int prefetch_depth = -1;  // -1: library default
+    int zc_batch = 0;         // 0: library default
+    bool device_schedule = false;
     bool pipeline = true;
     bool has_zc = false;
     uint64_t zc = 0;
@@ -86,6 +92,10 @@ static bool cli_parse(int argc, char **argv, CliConfig &c) {
         else if (a == "--prob") c.prob = cli_num<float>(next("prob"));
         else if (a == "-m" || a == "--materialized-partition") c.materialize = true;
         else if (a == "--scheduler") c.scheduler = next("scheduler");
+        else if (a == "--scorer") c.scorer = next("scorer");
+        else if (a == "--prefetch-depth") c.prefetch_depth = cli_num<int>(next("prefetch-depth"));
+        else if (a == "--zc-batch") c.zc_batch = cli_num<int>(next("zc-batch"));
+        else if (a == "--device-schedule") c.device_schedule = true;
         else if (a == "--no-pipeline") c.pipeline = false;
         else if (a == "--zerocopy-threshold") { c.zc = cli_num<uint64_t>(next("zerocopy-threshold")); c.has_zc = true; }
         else if (a == "--span") c.span = next("span");

@@ -1,0 +1,87 @@
+"""The pagerank / ppr / genericwalk programs on the GPU (app/*.cu main,
+src/Config.h:38-89): reference flags, the stdout lines the reference's
+benchmark scripts grep (benchmark/figure13/iter_analysis.sh,
+benchmark/figure9/run.sh), the result check of app/pagerank.cu:90-101 and
+app/ppr.cu:78-89, and the learned scheduler driven by a saved scorer."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import lighttraffic_amd as lt
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def graph_file(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cli")
+    path = str(d / "s16.bcsr")
+    lt.Graph.rmat(16, 16).save(path)
+    return path
+
+
+def run(app, *args):
+    cmd = [os.path.join(lt.BIN_DIR, app), *map(str, args)]
+    log(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def grep_num(pattern, out):
+    m = re.search(pattern, out)
+    assert m, (pattern, out)
+    return int(m.group(1))
+
+
+def test_pagerank_cli_capped_cache(graph_file):
+    out = run("pagerank", "-f", graph_file, "-m", "-s", 1, "-p", 3, "-w", 100000, "-l", 10, "--timers")
+    assert "result correct" in out
+    nparts = grep_num(r"number of partitons: (\d+)", out)
+    assert nparts > 3
+    # the three numbers benchmark/figure13/iter_analysis.sh extracts
+    it, ex = grep_num(r"iterations: (\d+)", out), grep_num(r"explicit: (\d+)", out)
+    loads = grep_num(r"graph loading time: [0-9.]+ ms, calls: (\d+)", out)
+    assert it >= ex >= 1 and loads >= nparts - 3
+    assert re.search(r"Running Time: [0-9.]+ ms", out)
+    # figure 13/14 ablations as runtime flags: FIFO, no pipelining, zero copy off / always
+    for flags in (["--scheduler", "fifo"], ["--no-pipeline"], ["--zerocopy-threshold", 1],
+                  ["--zerocopy-threshold", 1 << 40]):
+        out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, *flags)
+        assert "result correct" in out
+    out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, "--zerocopy-threshold", 1 << 40)
+    assert grep_num(r"graph loading time: [0-9.]+ ms, calls: (\d+)", out) == 0
+
+
+def test_ppr_and_genericwalk_cli(graph_file):
+    run("pagerank", "-f", graph_file, "-m", "-s", 1)  # materialise the .part file
+    out = run("ppr", "-f", graph_file, "-p", 4, "-w", 200000, "--source", 7, "--prob", 0.15, "--span", "resident",
+              "--zc-batch", 8, "--zerocopy-threshold", 4000)
+    assert "result correct" in out
+    out = run("genericwalk", "-f", graph_file, "-p", 2, "-w", 100000, "-l", 20, "--device-schedule")
+    assert re.search(r"Running Time: [0-9.]+ ms", out)
+
+
+def test_learned_scheduler_from_saved_scorer(graph_file, tmp_path):
+    g = lt.Graph.open(graph_file)
+    part = lt.Partition.load(graph_file)
+    eng = lt.Engine(g, part, num_device_partitions=3, span="resident", record_trace=1)
+    eng.pagerank(60000, 10)
+    ref_moves = eng.stats()["moves"]
+    sc = lt.Scorer(part, seed=3)
+    sc.add_trace(eng, True, True)
+    sc.fit(epochs=3)
+    weights = str(tmp_path / "scorer.bin")
+    sc.save(weights)
+    out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 60000, "-l", 10, "--scheduler", "learned",
+              "--scorer", weights, "--span", "resident", "--json")
+    assert "result correct" in out
+    # same walkers and seed as the engine run above, so the same moves
+    assert grep_num(r'"moves": (\d+)', out) == ref_moves
