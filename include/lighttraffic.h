@@ -177,6 +177,13 @@ lt_status lt_run_ppr(lt_engine *e, uint64_t w, uint32_t source, float prob, uint
  * number of moves it made (u32[w*runs]). */
 lt_status lt_run_genericwalk(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs,
                              uint32_t *endpoints_out, uint32_t *moves_out);
+/* genericwalk with the whole path of every walker (no counterpart in the
+ * reference's programs; its layout is that of the reference's unused path
+ * samplers, src/walk/NodeSampler.h:36-49 and src/walk/PathSampler.h:6-19):
+ * paths_out is u32[w*runs][l+1], row = walker id, column k = the vertex after
+ * k moves (column 0 the start), 0xFFFFFFFF after a walk that stopped early at
+ * a vertex without neighbours. */
+lt_status lt_run_genericwalk_paths(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, uint32_t *paths_out);
 /* Batched PPR (no counterpart in the reference): nq queries, query q starts
  * counts[q] walkers at sources[q]; walker ids are numbered query by query.
  * endpoints_out: u32[sum counts] in walker-id order (may be NULL). */

@@ -111,6 +111,7 @@ _sig("lt_engine_destroy", None, _vp)
 _sig("lt_run_pagerank", ctypes.c_int, _vp, _u64, _u32, _u32, _vp, _vp)
 _sig("lt_run_ppr", ctypes.c_int, _vp, _u64, _u32, ctypes.c_float, _vp, _vp)
 _sig("lt_run_genericwalk", ctypes.c_int, _vp, _u64, _u32, _u32, _vp, _vp)
+_sig("lt_run_genericwalk_paths", ctypes.c_int, _vp, _u64, _u32, _u32, _vp)
 _sig("lt_run_ppr_batch", ctypes.c_int, _vp, _u32, _vp, _vp, ctypes.c_float, _vp)
 _sig("lt_engine_stats", ctypes.c_int, _vp, ctypes.POINTER(Stats))
 _sig("lt_make_ppr_queries", ctypes.c_int, _vp, _u32, _u64, _u64, _u64, _vp, _vp)
@@ -331,6 +332,12 @@ class Engine:
         mv = np.zeros(w * runs, np.uint32) if moves else None
         _check(_lib.lt_run_genericwalk(self._h, w, l, runs, _ptr(ep), _ptr(mv)), "lt_run_genericwalk")
         return ep, mv
+
+    def genericwalk_paths(self, w, l=10, runs=1):
+        """u32[w * runs, l + 1]: column k is the vertex after k moves, 0xFFFFFFFF after an early stop."""
+        paths = np.zeros((w * runs, l + 1), np.uint32)
+        _check(_lib.lt_run_genericwalk_paths(self._h, w, l, runs, paths.ctypes.data), "lt_run_genericwalk_paths")
+        return paths
 
     def ppr_batch(self, sources, counts, prob=0.15, endpoints=True):
         sources = np.ascontiguousarray(sources, dtype=np.uint32)

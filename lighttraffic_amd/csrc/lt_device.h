@@ -86,6 +86,7 @@ struct AppArgs {
     uint64_t counts_stride;  // 0: one count array; V: workgroup b counts into copy b % kCountCopies
     uint32_t *endpoints;  // u32[N] indexed by walker id, or nullptr
     uint32_t *moves_out;  // u32[N] (genericwalk), or nullptr
+    uint32_t *paths;      // genericwalk: u32[N][length + 1], vertex after k moves at [id][k], or nullptr
     const uint64_t *q_off;  // batched ppr: [nq+1] first walker id of each query
     const uint32_t *q_src;  // batched ppr: [nq] sources
     uint32_t nq;

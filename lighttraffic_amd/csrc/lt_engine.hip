@@ -122,6 +122,7 @@ struct RunSpec {
     const uint32_t *q_src = nullptr;
     const uint64_t *q_cnt = nullptr;
     uint32_t nq = 0;
+    uint32_t *paths_out = nullptr;  // genericwalk: u32[total][length + 1] on the host
 };
 
 struct Engine {
@@ -168,6 +169,8 @@ struct Engine {
     uint32_t *d_endpoints = nullptr;
     size_t endpoints_cap = 0;
     uint32_t *d_moves = nullptr;
+    uint32_t *d_paths = nullptr;
+    size_t paths_cap = 0;
     size_t moves_cap = 0;
     uint64_t *d_qoff = nullptr;
     uint32_t *d_qsrc = nullptr;
@@ -259,6 +262,7 @@ Engine::~Engine() {
     dfree(d_counts);
     dfree(d_endpoints);
     dfree(d_moves);
+    dfree(d_paths);
     dfree(d_qoff);
     dfree(d_qsrc);
     dfree(d_stats);
@@ -432,6 +436,14 @@ lt_status Engine::ensure_outputs(const RunSpec &rs, bool want_endpoints, bool wa
         dfree(d_endpoints);
         LT_TRY(dmalloc(&d_endpoints, rs.total));
         endpoints_cap = rs.total;
+    }
+    if (rs.paths_out) {
+        const size_t n = (size_t)rs.total * ((size_t)rs.length + 1);
+        if (paths_cap < n) {
+            dfree(d_paths);
+            LT_TRY(dmalloc(&d_paths, n));
+            paths_cap = n;
+        }
     }
     if (want_moves && moves_cap < rs.total) {
         dfree(d_moves);
@@ -1114,6 +1126,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     app.counts_stride = count_copies > 1 ? V : 0;
     app.endpoints = endpoints_out ? d_endpoints : nullptr;
     app.moves_out = moves_out ? d_moves : nullptr;
+    app.paths = rs.paths_out ? d_paths : nullptr;
     if (rs.app == APP_PAGERANK) app.thresh = prob_threshold(1.0 - (double)0.85f);
     else if (rs.app == APP_PPR) app.thresh = prob_threshold((double)rs.prob);
     if (rs.app == APP_PPR && app.thresh == 0) {
@@ -1132,6 +1145,8 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     LT_HIP(hipMemsetAsync(d_counts, 0, 4 * (size_t)V * count_copies, s_compute));
     LT_HIP(hipMemsetAsync(d_stats, 0, 8 * (size_t)kNumStats * kStatSlots, s_compute));
     if (moves_out) LT_HIP(hipMemsetAsync(d_moves, 0, 4 * rs.total, s_compute));
+    if (rs.paths_out)  // columns after an early stop stay 0xFFFFFFFF
+        LT_HIP(hipMemsetAsync(d_paths, 0xFF, 4 * (size_t)rs.total * (rs.length + 1), s_compute));
 
     double running_ms = 0;
     for (uint64_t id0 = 0; id0 < rs.total; id0 += wave) {
@@ -1190,6 +1205,8 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     if (counts_out) LT_HIP(hipMemcpy(counts_out, d_counts, 4 * (size_t)V, hipMemcpyDeviceToHost));
     if (endpoints_out) LT_HIP(hipMemcpy(endpoints_out, d_endpoints, 4 * rs.total, hipMemcpyDeviceToHost));
     if (moves_out) LT_HIP(hipMemcpy(moves_out, d_moves, 4 * rs.total, hipMemcpyDeviceToHost));
+    if (rs.paths_out)
+        LT_HIP(hipMemcpy(rs.paths_out, d_paths, 4 * (size_t)rs.total * (rs.length + 1), hipMemcpyDeviceToHost));
     std::vector<unsigned long long> slots_h((size_t)kNumStats * kStatSlots);
     LT_HIP(hipMemcpy(slots_h.data(), d_stats, 8 * slots_h.size(), hipMemcpyDeviceToHost));
     unsigned long long hs[kNumStats] = {};
@@ -1275,6 +1292,17 @@ lt_status lt_run_genericwalk(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs
     rs.length = l;
     rs.nominal = w * (uint64_t)l * runs;
     return e->run(rs, nullptr, endpoints_out, moves_out);
+}
+
+lt_status lt_run_genericwalk_paths(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, uint32_t *paths_out) {
+    if (!e || w == 0 || runs == 0 || !paths_out) return LT_E_ARG;
+    RunSpec rs;
+    rs.app = APP_GENERIC;
+    rs.total = w * runs;
+    rs.length = l;
+    rs.nominal = w * (uint64_t)l * runs;
+    rs.paths_out = paths_out;
+    return e->run(rs, nullptr, nullptr, nullptr);
 }
 
 lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, const uint64_t *counts, float prob,

@@ -294,13 +294,18 @@ __device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArg
 __device__ __forceinline__ void step_generic(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
     const uint64_t *r = sp.rowptr + (w.cur - sp.nb);
     const uint64_t b = r[0], d = r[1] - b;
+    uint32_t *path = a.paths ? a.paths + walker_id(w) * ((uint64_t)a.length + 1) : nullptr;
+    if (path) path[a.length - w.step] = w.cur;  // the vertex after (length - step) moves
     if (d > 0) {
         const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_GENERIC);
         w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
         ct.moves++;
         if (sp.zc) { ct.zc_moves++; ct.zc_reads += 2; }
         w.step -= 1;
-        if (w.step == 0 && a.moves_out) a.moves_out[walker_id(w)] = a.length;
+        if (w.step == 0) {
+            if (a.moves_out) a.moves_out[walker_id(w)] = a.length;
+            if (path) path[a.length] = w.cur;
+        }
     } else {
         if (a.moves_out) a.moves_out[walker_id(w)] = a.length - w.step;
         w.step = 0;
@@ -499,6 +504,7 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
                 dst[k] = find_part(a.pt, w[k].cur);
             } else {
                 if (APP == APP_GENERIC && a.app.moves_out) a.app.moves_out[id] = 0;
+                if (APP == APP_GENERIC && a.app.paths) a.app.paths[id * ((uint64_t)a.app.length + 1)] = w[k].cur;
                 finish_walker(w[k], a.app);
             }
         }

@@ -41,6 +41,7 @@ def _load():
     sig("ltor_ppr_batch", ctypes.c_int, _vp, _vp, _u32, _u32, _vp, _vp, ctypes.c_float, _u64, _vp, ctypes.c_int,
         _vp)
     sig("ltor_genericwalk", ctypes.c_int, _vp, _vp, _u32, _u32, _u64, _u64, _u64, _vp, _vp, ctypes.c_int, _vp)
+    sig("ltor_genericwalk_paths", ctypes.c_int, _vp, _vp, _u32, _u32, _u64, _u64, _u64, _vp, ctypes.c_int)
     sig("ltor_rmat_edges", ctypes.c_int, _u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp,
         _vp, ctypes.c_int)
     sig("ltor_convert", ctypes.c_int, _u64, _vp, _vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64),
@@ -126,6 +127,16 @@ def genericwalk(rowptr, col, w, l, runs=1, seed=0x4C54524B, id_begin=0, id_end=N
                             m.ctypes.data, threads, st.ctypes.data)
     assert rc == 0
     return e, m, st
+
+
+def genericwalk_paths(rowptr, col, w, l, runs=1, seed=0x4C54524B, threads=0):
+    """u32[w * runs, l + 1]: vertex after k moves at column k, 0xFFFFFFFF after an early stop."""
+    rowptr, col = _arrs(rowptr, col)
+    V = len(rowptr) - 1
+    p = np.zeros((w * runs, l + 1), np.uint32)
+    rc = L.ltor_genericwalk_paths(rowptr.ctypes.data, col.ctypes.data, V, l, 0, w * runs, seed, p.ctypes.data, threads)
+    assert rc == 0
+    return p
 
 
 def rmat_edges(scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=None, threads=0):

@@ -241,6 +241,38 @@ int ltor_genericwalk(const uint64_t *rowptr, const uint32_t *col, uint32_t V, ui
     return 0;
 }
 
+/* genericwalk paths: row id of paths (l + 1 entries) holds the vertex after
+ * k moves at column k, the start at column 0, and 0xFFFFFFFF after a walk that
+ * stopped early (the layout of sampleNodeWithId / pathSampler,
+ * src/walk/NodeSampler.h:36-49, src/walk/PathSampler.h:6-19; the steps of
+ * app/genericwalk.cu:22-32 as in ltor_genericwalk). */
+int ltor_genericwalk_paths(const uint64_t *rowptr, const uint32_t *col, uint32_t V, uint32_t l, uint64_t id_begin,
+                           uint64_t id_end, uint64_t seed, uint32_t *paths, int nthreads) {
+    if (V == 0 || !paths) return -1;
+    const uint64_t row = (uint64_t)l + 1;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t k = (int64_t)id_begin; k < (int64_t)id_end; k++) {
+        uint64_t id = (uint64_t)k;
+        uint32_t *p = paths + (id - id_begin) * row;
+        for (uint64_t j = 0; j < row; j++) p[j] = 0xFFFFFFFFu;
+        uint32_t r[4];
+        rnd(seed, id, 0, TAG_START, r);
+        uint32_t cur = (uint32_t)pick(r[0], r[1], V);
+        uint32_t s = l, mv = 0;
+        p[0] = cur;
+        while (s > 0) {
+            uint64_t d = deg_of(rowptr, cur);
+            if (d == 0) break;
+            rnd(seed, id, s, TAG_GENERIC, r);
+            cur = col[rowptr[cur] + pick(r[1], r[2], d)];
+            p[++mv] = cur;
+            s--;
+        }
+    }
+    return 0;
+}
+
 /* ---- R-MAT edge sampling ------------------------------------------------- */
 /* Edge e, level t: word t%4 of Philox(seed; e, t/4, TAG_RMAT); quadrant by
  * cumulative thresholds of (a, a+b, a+b+c); level t sets bit (scale-1-t). */

@@ -116,6 +116,31 @@ def test_genericwalk_bit_exact(g12, label, pbytes, kw):
     assert np.array_equal(moves, rm)
 
 
+@pytest.mark.parametrize("label,pbytes,kw", [c for c in CONFIGS if c[0] in (
+    "one-partition", "capped", "all-zc", "span-resident", "device-sched-zc", "small-slices", "zc-batch")],
+    ids=lambda x: x if isinstance(x, str) else "")
+def test_genericwalk_paths_bit_exact(g12, label, pbytes, kw):
+    """Whole paths (layout of src/walk/NodeSampler.h:36-49) against the oracle."""
+    log(f"genericwalk paths {label}")
+    part = lt.Partition.build(g12, pbytes)
+    eng = lt.Engine(g12, part, **kw)
+    w, l = 3 * g12.num_node, 9
+    paths = eng.genericwalk_paths(w, l, runs=2)
+    assert np.array_equal(paths, oracle.genericwalk_paths(g12.rowptr(), g12.col(), w, l, runs=2))
+
+
+def test_genericwalk_paths_with_dead_ends():
+    log("genericwalk paths dangling")
+    g = _dangling_graph()
+    part = lt.Partition.build(g, 2048)
+    eng = lt.Engine(g, part, num_device_partitions=3, zerocopy_threshold=100)
+    for l in (0, 1, 7):
+        paths = eng.genericwalk_paths(4000, l)
+        ref = oracle.genericwalk_paths(g.rowptr(), g.col(), 4000, l)
+        assert np.array_equal(paths, ref)
+    assert (ref == 0xFFFFFFFF).any()
+
+
 def _dangling_graph():
     rng = np.random.default_rng(5)
     V = 3000

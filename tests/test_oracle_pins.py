@@ -287,3 +287,32 @@ def test_find_partition_bruteforce():
     for v in range(40):
         expect = max(k for k in range(len(no) - 1) if no[k] <= v)
         assert oracle.find_partition(no, v) == expect
+
+
+def test_genericwalk_paths_follow_edges_and_match_walk():
+    """Paths restate the same walk as ltor_genericwalk: column 0 is the Philox
+    start, every next column a neighbour of the previous one, the last valid
+    column the endpoint, the valid columns one more than the moves; columns
+    after an early stop are 0xFFFFFFFF."""
+    rng = np.random.default_rng(4)
+    V = 500
+    deg = rng.integers(0, 5, V)
+    deg[rng.random(V) < 0.2] = 0
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    col = rng.integers(0, V, int(rowptr[-1])).astype(np.uint32)
+    w, l = 3000, 6
+    paths = oracle.genericwalk_paths(rowptr, col, w, l, runs=2)
+    ends, moves, _ = oracle.genericwalk(rowptr, col, w, l, runs=2)
+    assert paths.shape == (2 * w, l + 1)
+    for i in range(2 * w):
+        p = paths[i]
+        valid = int(moves[i]) + 1
+        assert np.all(p[valid:] == 0xFFFFFFFF) and np.all(p[:valid] < V)
+        assert p[valid - 1] == ends[i]
+        x = oracle.philox([i & 0xFFFFFFFF, i >> 32, 0, 0x53545254], [0x4C54524B, 0])
+        assert p[0] == (((x[0] << 32) | x[1]) * V) >> 64
+        for k in range(valid - 1):
+            a = int(p[k])
+            assert p[k + 1] in col[rowptr[a]:rowptr[a + 1]]
+        if valid <= l:  # stopped early: at a vertex without neighbours
+            assert rowptr[p[valid - 1] + 1] == rowptr[p[valid - 1]]
