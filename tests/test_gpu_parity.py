@@ -252,3 +252,43 @@ def test_generator_dedupe_seams(monkeypatch):
         monkeypatch.setenv("LT_RMAT_DEDUPE_PIECE", piece)
         g = lt.Graph.rmat(12, 16)
         assert np.array_equal(g.rowptr(), rowptr) and np.array_equal(g.col(), col)
+
+
+def _many_partition_graph():
+    rng = np.random.default_rng(21)
+    V = 60000
+    deg = rng.integers(0, 9, V)
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    col = rng.integers(0, V, int(rowptr[-1])).astype(np.uint32)
+    return lt.Graph.from_csr(rowptr, col)
+
+
+@pytest.mark.parametrize("kw", [{"num_device_partitions": 600},
+                                {"num_device_partitions": 600, "span": "resident", "zerocopy_threshold": 40,
+                                 "zc_batch": 16},
+                                {"num_device_partitions": 600, "device_schedule": 1}])
+def test_near_the_partition_limit(kw):
+    """About 2,400 partitions (the append stage's LDS histogram is sized by P;
+    the engine limit is 2,900) with a capped cache."""
+    g = _many_partition_graph()
+    part = lt.Partition.build(g, 600)
+    log(f"P={part.count} {kw}")
+    assert 2000 < part.count <= 2900
+    eng = lt.Engine(g, part, **kw)
+    w = 100000
+    counts, ends = eng.pagerank(w, 8, endpoints=True)
+    rc, re, _ = oracle.pagerank(g.rowptr(), g.col(), w, 8, endpoints=True)
+    assert np.array_equal(counts, rc) and np.array_equal(ends, re)
+    counts = eng.ppr(w, source=5)
+    assert np.array_equal(counts, oracle.ppr(g.rowptr(), g.col(), w, source=5)[0])
+    ends, _ = eng.genericwalk(w, 12)
+    assert np.array_equal(ends, oracle.genericwalk(g.rowptr(), g.col(), w, 12)[0])
+
+
+def test_more_partitions_than_the_limit_is_an_error():
+    g = _many_partition_graph()
+    part = lt.Partition.build(g, 400)
+    assert part.count > 2900
+    with pytest.raises(lt.LTError) as e:
+        lt.Engine(g, part)
+    assert e.value.status == 6  # LT_E_ARG
