@@ -319,6 +319,13 @@ def main():
                 variants[name] = {k: v[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
                                                     "h2d_GB_per_job", "pcie_GBps")}
                 e2.close()
+            # same plain rules, walks continue through every cached partition and
+            # sub-threshold queues are walked over PCIe 16 at a time
+            e2 = lt.Engine(g, part, span="resident", zc_batch=16, **base)
+            v = measure(e2, wl, w, 1, 2, "c3/resident_zc_batch16")
+            variants["span_resident_zc_batch16"] = {k: v[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
+                                                                      "h2d_GB_per_job", "pcie_GBps")}
+            e2.close()
             out["c3_variants"] = variants
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "span": args.span or "partition",
