@@ -10,7 +10,8 @@ Workloads (R-MAT a,b,c,d = .57/.19/.19/.05, seeded, deduped, symmetrised,
 self loops and isolated vertices removed):
   c5  (default) scale 26 ef 27, 32 MB partitions, 8 GB partition cache
       (256 slots), pagerank w=2|V| l=10: plain scheduler vs learned scheduler
-  c1  scale 16 ef 16, one 32 MB partition, pagerank w=2|V| l=10
+  c1  scale 16 ef 16, one 32 MB partition loaded into HBM (zero-copy threshold 0),
+      pagerank w=2|V| l=10
   c2  scale 22 ef 16, 32 MB partitions all HBM-resident, genericwalk w=2|V| l=10
   c3  scale 25 ef 44, 4 GB partition cache, 2^26 walkers at once, ppr from vertex 0
   c4  same graph, 4,096 PPR queries (sources proportional to degree, walk counts
@@ -35,7 +36,7 @@ METRIC = "walk steps/s & H2D bytes/step: learned vs plain scheduler, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10),
+    "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10, zc=0),
     "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10),
     "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26),
     "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
@@ -202,6 +203,8 @@ def main():
     base = dict(num_device_partitions=wl["slots"])
     if "wdev" in wl:
         base["num_device_walkers"] = wl["wdev"]
+    if "zc" in wl:  # c1: load the one partition (its 2|V| walkers are below S/256) so it stays cached
+        base["zerocopy_threshold"] = wl["zc"]
     desc = (f"{args.workload}: R-MAT scale {wl['scale']} ef {wl['ef']} (V={g.num_node}, E={g.num_edge}), "
             f"{wl['app']} w=2|V|={w}" + (f" l={wl['l']}" if wl['l'] else " p=0.15") +
             f", {wl['mb']} MB partitions (P={part.count}), cache " +
