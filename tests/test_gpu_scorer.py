@@ -156,3 +156,43 @@ def test_learned_policy_without_trained_scorer_falls_back():
     assert np.array_equal(counts, rc)
     st = eng.stats()
     assert st["prefetch_issued"] == 0 and st["learned_decisions"] == 0 and st["fallback_decisions"] > 0
+
+
+def test_trained_scorer_gains_over_untrained_weights():
+    """What training buys: on a capped cache, the learned policy with the
+    trained network prefetches partitions that are walked next far more often
+    than the same policy with the network's initial (untrained) weights, and
+    so copies fewer partitions.  Prefetching still copies more than the plain
+    rules on this small graph (63 partitions, about +25 %); on C5 the two load
+    about the same.  Visit counts are identical in all three runs."""
+    log("trained vs untrained scorer")
+    g = lt.Graph.rmat(20, 16)
+    part = lt.Partition.build(g, 2 << 20)
+    slots = part.count // 2
+    w = 2 * g.num_node
+    common = dict(num_device_partitions=slots, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16)
+    plain = lt.Engine(g, part, record_trace=1, **common)
+    ref = plain.pagerank(w, 10)
+    ref = plain.pagerank(w, 10)
+    plain_loads = plain.stats()["graph_loading_calls"]
+    trained = lt.Scorer(part, seed=5)
+    untrained = lt.Scorer(part, seed=5)
+    untrained.set_params(untrained.get_params())  # the initial weights, marked usable
+    trained.add_trace(plain, True, True)
+    trained.fit(epochs=20)
+    res = {}
+    for name, sc in (("trained", trained), ("untrained", untrained)):
+        eng = lt.Engine(g, part, policy="learned", **common)
+        eng.set_scorer(sc)
+        eng.pagerank(w, 10)
+        counts = eng.pagerank(w, 10)
+        assert np.array_equal(counts, ref)
+        st = eng.stats()
+        res[name] = (st["prefetch_hits"] / max(1, st["prefetch_issued"]), st["graph_loading_calls"], st["prefetch_issued"])
+        log(f"{name}: P={part.count} hit rate {res[name][0]:.2f}, loads {res[name][1]} "
+            f"(plain {plain_loads}), prefetches {res[name][2]}")
+    assert res["trained"][2] > 20
+    assert res["trained"][0] >= 0.6
+    assert res["trained"][0] >= res["untrained"][0] + 0.1
+    assert res["trained"][1] < res["untrained"][1]
+    assert res["trained"][1] <= 1.5 * plain_loads
