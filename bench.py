@@ -12,7 +12,8 @@ self loops and isolated vertices removed):
       (256 slots), pagerank w=2|V| l=10: plain scheduler vs learned scheduler
   c1  scale 16 ef 16, one 32 MB partition loaded into HBM (zero-copy threshold 0),
       pagerank w=2|V| l=10
-  c2  scale 22 ef 16, 32 MB partitions all HBM-resident, genericwalk w=2|V| l=10
+  c2  scale 22 ef 16, 32 MB partitions all HBM-resident, genericwalk w=2|V| l=10,
+      walks continue through every partition (the reference's span is reported beside)
   c3  scale 25 ef 44, 4 GB partition cache, 2^26 walkers at once, ppr from vertex 0
   c4  same graph, 4,096 PPR queries (sources proportional to degree, walk counts
       log-uniform in [1e3, 1e6]), 8 GB cache; configuration chosen by the
@@ -37,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10, zc=0),
-    "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10),
+    "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10, span="resident"),
     "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26),
     "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
     "c5": dict(scale=26, ef=27, mb=32, app="pagerank", slots=256, l=10),
@@ -309,15 +310,21 @@ def main():
                                                                "slice_chunks", "running_ms", "h2d_bytes")}
                                             for t in trials]}})
     else:
-        eng = lt.Engine(g, part, span=args.span or "partition", **base)
-        teng = lt.Engine(g, part, span=args.span or "partition", event_timers=1, **base)
+        span = args.span or wl.get("span", "partition")
+        eng = lt.Engine(g, part, span=span, **base)
+        teng = lt.Engine(g, part, span=span, event_timers=1, **base)
         run_job(teng, wl, w)
         m = measure(eng, wl, w, args.warmup, args.steps, args.workload, timed_engine=teng)
+        if args.workload == "c2":  # the reference's span: walks stop at every partition boundary
+            e2 = lt.Engine(g, part, span="partition", **base)
+            v = measure(e2, wl, w, 1, 2, "c2/span partition")
+            out["c2_variants"] = {"span_partition": {k: v[k] for k in ("steps_per_s", "ms_per_step")}}
+            e2.close()
         if args.workload == "c3":
             # explicit copies vs zero-copy reads (the reference's figure-14 ablation)
             variants = {}
             for name, thr in (("explicit_only", 0), ("zerocopy_only", 1 << 62)):
-                e2 = lt.Engine(g, part, span=args.span or "partition", zerocopy_threshold=thr, **base)
+                e2 = lt.Engine(g, part, span=span, zerocopy_threshold=thr, **base)
                 v = measure(e2, wl, w, 1, 1, f"c3/{name}")
                 variants[name] = {k: v[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
                                                     "h2d_GB_per_job", "pcie_GBps")}
@@ -331,9 +338,9 @@ def main():
             e2.close()
             out["c3_variants"] = variants
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
-                    "config": {"workload": desc, "scheduler": "greedy", "span": args.span or "partition",
+                    "config": {"workload": desc, "scheduler": "greedy", "span": span,
                                "device": info["name"]},
-                    "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, args.workload, args.span or "partition"),
+                    "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, args.workload, span),
                     "stats_per_step": m["per_job"]})
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, wl, w)

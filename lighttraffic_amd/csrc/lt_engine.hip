@@ -111,6 +111,23 @@ struct EventTimer {
     }
 };
 
+// Waits on the scheduler's critical path poll instead of blocking: every
+// decision waits for the fix-up of the previous walk, and a blocking wait adds
+// its wake-up latency to each of them.
+static hipError_t spin_stream(hipStream_t st) {
+    hipError_t e;
+    while ((e = hipStreamQuery(st)) == hipErrorNotReady) {
+    }
+    return e;
+}
+
+static hipError_t spin_event(hipEvent_t ev) {
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+    }
+    return e;
+}
+
 struct RunSpec {
     int app = APP_PAGERANK;
     uint64_t total = 0;     // walker ids [0, total)
@@ -466,7 +483,7 @@ lt_status Engine::read_meta() {
         LT_HIP(hipGetLastError());
         meta_dirty = false;
     }
-    LT_HIP(hipStreamSynchronize(s_compute));  // the last fix-up wrote h_meta
+    LT_HIP(spin_stream(s_compute));  // the last fix-up wrote h_meta
     const PoolCtl *c = (const PoolCtl *)(h_meta + P);
     if (c->overflow) {
         set_error("walker queue pool overflow (flags 0x%x, top %u)", c->overflow, c->top);
@@ -841,7 +858,7 @@ lt_status Engine::scheduler_loop_learned() {
         while (!inflight.empty()) {
             Inflight f = inflight.front();
             const bool must = upto >= 0;
-            if (must) LT_HIP(hipEventSynchronize(f.done));
+            if (must) LT_HIP(spin_event(f.done));
             else if (hipEventQuery(f.done) != hipSuccess) break;
             inflight.erase(inflight.begin());
             pool_events.push_back(f.done);

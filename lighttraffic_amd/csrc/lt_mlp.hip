@@ -623,7 +623,10 @@ lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float 
     infer_kernel<<<(P + 7) / 8, 256, (bytes + 15) & ~(size_t)15, st>>>(P, s->d_stage, s->d_vsh, s->d_esh, s->Wb,
                                                                          s->Wi, s->d_Y);
     MLP_HIP(hipGetLastError());
-    MLP_HIP(hipStreamSynchronize(st));
+    hipError_t e;  // poll: this wait is on the scheduler's critical path
+    while ((e = hipStreamQuery(st)) == hipErrorNotReady) {
+    }
+    MLP_HIP(e);
     memcpy(out, s->h_Y, sizeof(float) * P * kOut);
     return LT_OK;
 }
