@@ -28,6 +28,7 @@ constexpr uint32_t kQStride = 32;           // u32 words per queue length: one 1
                                             // so appends to different queues do not share a line
 constexpr uint32_t kStatSlots = 256;        // event counters are spread over this many lines
 constexpr uint32_t kMaxZcBatch = 16;        // partitions walked from host memory by one launch
+constexpr uint32_t kMaxHostPieces = 32;     // spilled walker pieces (<= kChunk each) read by one launch
 constexpr uint32_t kCountCopies = 8;        // visit-count copies, one per XCD of an MI355X: workgroups
                                             // are dispatched to the 8 XCDs round-robin, so workgroup b
                                             // adds into copy b % 8 and its lines stay in one XCD's L2
@@ -143,6 +144,12 @@ struct WalkArgs {
     uint32_t nsrc;
     uint32_t srcs[kMaxZcBatch];
     uint64_t voff[kMaxZcBatch + 1];
+    // Spilled walkers (nhost > 0): walkers of src kept in pinned host memory,
+    // read by the kernel over PCIe; piece k holds hn[k] <= kChunk walkers at
+    // hptr[k] and occupies [k * kChunk, k * kChunk + hn[k]) of the range.
+    uint32_t nhost;
+    const Walker *hptr[kMaxHostPieces];
+    uint32_t hn[kMaxHostPieces];
 };
 
 // Table entries of a zero-copy batch.

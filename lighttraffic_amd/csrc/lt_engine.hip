@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <deque>
 #include <memory>
 #include <vector>
 
@@ -177,7 +178,31 @@ struct Engine {
     Queues q{};
     uint32_t *d_meta = nullptr;   // [P] qlen followed by PoolCtl
     uint32_t *h_meta = nullptr;   // pinned mirror
-    std::vector<uint64_t> counts; // host view of qlen
+    std::vector<uint64_t> counts; // walkers queued per partition: device queue + spilled to host
+    std::vector<uint64_t> dcounts;  // device part of counts (qlen)
+
+    // Host walker spill (src/WalkManager.h:11-58, src/Scheduler.h:53-82,
+    // 215-233): with more walkers than num_device_walkers, whole device
+    // queues move to per-partition queues in pinned, mapped host memory, and
+    // come back when their partition is walked (the kernel reads them over
+    // PCIe).  Host chunks are kChunk walkers, handed out from pinned blocks.
+    static constexpr uint32_t kHostBlockChunks = 256;  // 16 MiB per pinned block
+    bool spill_on = false;
+    uint64_t dev_cap = 0;                   // walkers the device queues may hold
+    std::vector<Walker *> hblocks, hblocks_dev;
+    std::vector<uint32_t> hfree;            // free host chunk ids
+    std::vector<std::deque<uint32_t>> hq;   // per partition: host chunks, oldest first
+    std::vector<uint64_t> hhead, hcount;    // first walker's slot in hq[p].front(); walkers
+    uint64_t spill_h2d_bytes = 0;           // spilled walkers read back over PCIe
+    Walker *host_chunk_ptr(uint32_t id, bool dev) const {
+        return (dev ? hblocks_dev : hblocks)[id / kHostBlockChunks] + (size_t)(id % kHostBlockChunks) * kChunk;
+    }
+    lt_status host_chunk(uint32_t &id);
+    void host_reset();
+    int32_t spill_victim(uint32_t except) const;
+    lt_status spill(uint32_t v);
+    lt_status ensure_room(uint64_t want, uint32_t except, uint64_t &room);
+    lt_status process_host(uint32_t p, const WalkArgs &a);
 
     // outputs and statistics
     uint32_t *d_counts = nullptr;
@@ -276,6 +301,7 @@ Engine::~Engine() {
     dfree(q.lost);
     dfree(d_meta);
     if (h_meta) (void)hipHostFree(h_meta);
+    for (Walker *b : hblocks) (void)hipHostFree(b);
     dfree(d_counts);
     dfree(d_endpoints);
     dfree(d_moves);
@@ -406,6 +432,10 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     slice = cfg.slice_chunks > 0 ? (uint32_t)cfg.slice_chunks : kSliceChunks;
     LT_TRY(dmalloc(&d_stats, (size_t)kNumStats * kStatSlots));
     counts.assign(P, 0);
+    dcounts.assign(P, 0);
+    hq.assign(P, {});
+    hhead.assign(P, 0);
+    hcount.assign(P, 0);
 
     bool timers = cfg.event_timers != 0;
     t_load.enabled = t_compute.enabled = t_zc.enabled = t_wload.enabled = t_wevict.enabled = timers;
@@ -489,7 +519,10 @@ lt_status Engine::read_meta() {
         set_error("walker queue pool overflow (flags 0x%x, top %u)", c->overflow, c->top);
         return LT_E_INTERNAL;
     }
-    for (uint32_t i = 0; i < P; i++) counts[i] = h_meta[i];
+    for (uint32_t i = 0; i < P; i++) {
+        dcounts[i] = h_meta[i];
+        counts[i] = dcounts[i] + hcount[i];
+    }
     return LT_OK;
 }
 
@@ -574,7 +607,9 @@ lt_status Engine::launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s
 // walk kernel, then fix-up kernel per slice (processInMemory with all
 // batches, src/WalkManager.h:375-388).
 lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
-    if (walkers == 0) return LT_OK;
+    if (spill_on) walkers = dcounts[p];  // counts[p] also holds the walkers spilled to host
+    const uint64_t host = spill_on ? hcount[p] : 0;
+    if (walkers == 0 && host == 0) return LT_OK;
     WalkArgs a{};
     a.app = app;
     a.q = q;
@@ -618,7 +653,157 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
         meta_dirty = false;
         st.kernel_launches++;
     }
+    if (host) LT_TRY(process_host(p, a));
     phase.end(tp, s_compute);
+    return LT_OK;
+}
+
+// ---- host walker spill -------------------------------------------------------
+lt_status Engine::host_chunk(uint32_t &id) {
+    if (hfree.empty()) {
+        Walker *b = nullptr, *bd = nullptr;
+        LT_HIP(hipHostMalloc((void **)&b, sizeof(Walker) * kChunk * kHostBlockChunks,
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        hblocks.push_back(b);
+        LT_HIP(hipHostGetDevicePointer((void **)&bd, b, 0));
+        hblocks_dev.push_back(bd);
+        const uint32_t base = (uint32_t)(hblocks.size() - 1) * kHostBlockChunks;
+        for (uint32_t k = kHostBlockChunks; k-- > 0;) hfree.push_back(base + k);
+    }
+    id = hfree.back();
+    hfree.pop_back();
+    return LT_OK;
+}
+
+void Engine::host_reset() {
+    for (uint32_t p = 0; p < P; p++) {
+        for (uint32_t c : hq[p]) hfree.push_back(c);
+        hq[p].clear();
+        hhead[p] = hcount[p] = 0;
+    }
+}
+
+// getEvictedPartition (src/Scheduler.h:149-166): among the partitions other
+// than `except` holding at least one full chunk on the device, the one with the
+// fewest walkers in all; else any partition with walkers on the device.
+int32_t Engine::spill_victim(uint32_t except) const {
+    int32_t full = -1, any = -1;
+    for (uint32_t i = 0; i < P; i++) {
+        if (i == except || dcounts[i] == 0) continue;
+        if (dcounts[i] >= kChunk) {
+            if (full < 0 || counts[i] < counts[full]) full = (int32_t)i;
+        } else if (any < 0 || counts[i] < counts[any]) {
+            any = (int32_t)i;
+        }
+    }
+    return full >= 0 ? full : any;
+}
+
+// evictWalker (src/Scheduler.h:53-58): copy the whole device queue of v behind
+// its host queue, then return its chunks to the pool.  Called with the compute
+// stream idle and dcounts fresh.
+lt_status Engine::spill(uint32_t v) {
+    const uint64_t n = dcounts[v];
+    if (n == 0) return LT_OK;
+    const uint64_t nch = (n + kChunk - 1) / kChunk;
+    std::vector<uint32_t> frame(nch);
+    LT_HIP(hipMemcpyAsync(frame.data(), q.frame + (size_t)v * q.maxc, 4 * nch, hipMemcpyDeviceToHost, s_compute));
+    LT_HIP(spin_stream(s_compute));
+    hipEvent_t te = t_wevict.begin(s_compute);
+    uint64_t pos = hhead[v] + hcount[v];  // next free slot behind the host queue
+    for (uint64_t i = 0; i < n;) {
+        while ((pos >> kChunkLog2) >= hq[v].size()) {
+            uint32_t id;
+            LT_TRY(host_chunk(id));
+            hq[v].push_back(id);
+        }
+        const uint64_t doff = i & (kChunk - 1), hoff = pos & (kChunk - 1);
+        const uint64_t len = std::min<uint64_t>(n - i, std::min<uint64_t>(kChunk - doff, kChunk - hoff));
+        if (frame[i >> kChunkLog2] >= q.nchunks) {
+            set_error("queue of partition %u has no chunk %llu", v, (unsigned long long)(i >> kChunkLog2));
+            return LT_E_INTERNAL;
+        }
+        LT_HIP(hipMemcpyAsync(host_chunk_ptr(hq[v][pos >> kChunkLog2], false) + hoff,
+                              q.slab + (size_t)frame[i >> kChunkLog2] * kChunk + doff, len * sizeof(Walker),
+                              hipMemcpyDeviceToHost, s_compute));
+        i += len;
+        pos += len;
+    }
+    fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, v, 0, 0xFFFFFFFFu, 1, nullptr);  // chunks back, qlen[v] = 0
+    LT_HIP(hipGetLastError());
+    meta_dirty = false;
+    t_wevict.end(te, s_compute);
+    hcount[v] += n;
+    dcounts[v] = 0;
+    st.walker_evicting_calls++;
+    return LT_OK;
+}
+
+// handleWalkerPoolOverflow (src/Scheduler.h:59-82): spill device queues
+// (never `except`'s) until `room` more walkers fit under dev_cap; room is
+// `want` or at least half the capacity.
+lt_status Engine::ensure_room(uint64_t want, uint32_t except, uint64_t &room) {
+    const uint64_t target = std::max<uint64_t>(1, std::min<uint64_t>(want, dev_cap / 2));
+    while (true) {
+        LT_TRY(read_meta());
+        uint64_t live = 0;
+        for (uint32_t i = 0; i < P; i++) live += dcounts[i];
+        if (live + target <= dev_cap) {
+            room = std::min<uint64_t>(want, dev_cap - live);
+            return LT_OK;
+        }
+        const int32_t v = spill_victim(except);
+        if (v < 0) {
+            set_error("no walkers left to spill (%llu on the device, capacity %llu)", (unsigned long long)live,
+                      (unsigned long long)dev_cap);
+            return LT_E_INTERNAL;
+        }
+        LT_TRY(spill((uint32_t)v));
+    }
+}
+
+// walkerLoadingPhase + computeOffMemory (src/Scheduler.h:215-233,
+// src/WalkManager.h:389-421): walk p's spilled walkers in launches of at most
+// kMaxHostPieces pieces, read from host memory by the kernel, after making
+// room on the device for the walkers they may append.
+lt_status Engine::process_host(uint32_t p, const WalkArgs &a) {
+    while (hcount[p] > 0) {
+        uint64_t room;
+        LT_TRY(ensure_room(std::min<uint64_t>(hcount[p], (uint64_t)kMaxHostPieces * kChunk), p, room));
+        WalkArgs b = a;
+        b.nhost = 0;
+        uint64_t pos = hhead[p], total = 0;
+        while (b.nhost < kMaxHostPieces && total < room) {
+            const uint64_t off = pos & (kChunk - 1);
+            const uint64_t len = std::min<uint64_t>(kChunk - off, room - total);
+            b.hptr[b.nhost] = host_chunk_ptr(hq[p][pos >> kChunkLog2], true) + off;
+            b.hn[b.nhost] = (uint32_t)len;
+            b.nhost++;
+            pos += len;
+            total += len;
+        }
+        hipEvent_t tl = t_wload.begin(s_compute);
+        hipEvent_t tk = t_kernel.begin(s_compute);
+        LT_TRY(launch_walk(b, total, s_compute));
+        t_kernel.end(tk, s_compute);
+        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, nullptr);
+        LT_HIP(hipGetLastError());
+        meta_dirty = false;
+        t_wload.end(tl, s_compute);
+        LT_HIP(spin_stream(s_compute));  // the pieces go back to the free list below
+        hhead[p] += total;
+        hcount[p] -= total;
+        while (!hq[p].empty() && (hhead[p] >= kChunk || hcount[p] == 0)) {
+            hfree.push_back(hq[p].front());
+            hq[p].pop_front();
+            hhead[p] = hhead[p] >= kChunk ? hhead[p] - kChunk : 0;
+        }
+        if (hcount[p] == 0) hhead[p] = 0;
+        counts[p] = dcounts[p] + hcount[p];
+        spill_h2d_bytes += total * sizeof(Walker);
+        st.walker_loading_calls++;
+        st.kernel_launches++;
+    }
     return LT_OK;
 }
 
@@ -626,7 +811,7 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
 // queues below the threshold (not resident, not being copied), in one launch.
 lt_status Engine::process_zc(uint32_t c, const uint8_t *pending) {
     std::vector<uint32_t> batch{c};
-    if (zc_batch > 1) {
+    if (zc_batch > 1 && !spill_on) {  // a batch consumes device queues only
         std::vector<uint32_t> cand;
         for (uint32_t i = 0; i < P; i++)
             if (i != c && !resident[i] && !(pending && pending[i]) && counts[i] > 0 && counts[i] < zc_threshold)
@@ -1115,6 +1300,7 @@ lt_status Engine::scheduler_loop_device() {
 lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoints_out, uint32_t *moves_out) {
     LT_HIP(hipSetDevice(device));
     memset(&st, 0, sizeof(st));
+    spill_h2d_bytes = 0;
     trace.clear();
     trace_counts.clear();
     trace_resident.clear();
@@ -1125,12 +1311,17 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     t_wload.reset();
     t_wevict.reset();
     t_kernel.reset();
-    const uint64_t wave = cfg.num_device_walkers ? std::min<uint64_t>(cfg.num_device_walkers, rs.total) : rs.total;
+    const uint64_t cap = cfg.num_device_walkers ? std::min<uint64_t>(cfg.num_device_walkers, rs.total) : rs.total;
+    // More walkers than the device holds: spill to host queues (the
+    // reference's -b); the device-scheduled loop runs them in waves instead.
+    spill_on = cap < rs.total && !cfg.device_schedule;
+    dev_cap = cap;
+    const uint64_t wave = spill_on ? rs.total : cap;
     // Per-XCD count copies pay when the walkers of one launch stay in a few
     // partitions (capped cache, PPR around its sources); with every partition
     // resident the visits spread over all of V and one array is faster.
     count_copies = rs.app != APP_GENERIC && (nslots < P || rs.app == APP_PPR) ? kCountCopies : 1;
-    LT_TRY(ensure_pool(std::max<uint64_t>(wave, 1)));
+    LT_TRY(ensure_pool(std::max<uint64_t>(cap, 1)));
     LT_TRY(ensure_outputs(rs, endpoints_out != nullptr, moves_out != nullptr));
 
     app_kind = rs.app;
@@ -1183,19 +1374,25 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         ia.pt.zc_part = d_zc_part;
         ia.pt.lut_shift = lut_shift;
         ia.pt.P = P;
-        ia.id_lo = id0;
-        ia.id_hi = id1;
         ia.stats = d_stats;
-        const int grid = (int)std::min<uint64_t>((id1 - id0 + kBlockWalkers - 1) / kBlockWalkers, (uint64_t)max_grid);
-        switch (rs.app) {
-            case APP_PAGERANK: init_kernel<APP_PAGERANK><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
-            case APP_PPR: init_kernel<APP_PPR><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
-            default: init_kernel<APP_GENERIC><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
+        host_reset();
+        for (uint64_t lo = id0; lo < id1;) {  // with spill, in pieces that fit on the device
+            uint64_t n = id1 - lo;
+            if (spill_on) LT_TRY(ensure_room(n, kEmpty, n));
+            ia.id_lo = lo;
+            ia.id_hi = lo + n;
+            const int grid = (int)std::min<uint64_t>((n + kBlockWalkers - 1) / kBlockWalkers, (uint64_t)max_grid);
+            switch (rs.app) {
+                case APP_PAGERANK: init_kernel<APP_PAGERANK><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
+                case APP_PPR: init_kernel<APP_PPR><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
+                default: init_kernel<APP_GENERIC><<<grid, kBlock, lds_bytes(), s_compute>>>(ia); break;
+            }
+            LT_HIP(hipGetLastError());
+            fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, nullptr);
+            LT_HIP(hipGetLastError());
+            meta_dirty = false;
+            lo += n;
         }
-        LT_HIP(hipGetLastError());
-        fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, nullptr);
-        LT_HIP(hipGetLastError());
-        meta_dirty = false;
         LT_HIP(hipStreamSynchronize(s_compute));
         const auto t0 = std::chrono::steady_clock::now();
         w_run = id1 - id0;
@@ -1234,7 +1431,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     st.visits = hs[ST_VISITS];
     st.residencies = hs[ST_RESIDENCIES];
     st.zerocopy_moves = hs[ST_ZC_MOVES];
-    st.h2d_bytes_zerocopy = hs[ST_ZC_READS] * 64ull;
+    st.h2d_bytes_zerocopy = hs[ST_ZC_READS] * 64ull + spill_h2d_bytes;
     st.nominal_steps = rs.nominal;
     t_load.collect();
     t_compute.collect();
@@ -1243,8 +1440,10 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     st.graph_loading_ms = t_load.ms;
     st.computing_ms = t_compute.ms;
     st.zerocopy_ms = t_zc.ms;
-    st.walker_loading_ms = 0;
-    st.walker_evicting_ms = 0;
+    t_wload.collect();
+    t_wevict.collect();
+    st.walker_loading_ms = t_wload.ms;
+    st.walker_evicting_ms = t_wevict.ms;
     st.walk_kernel_ms = t_kernel.ms;
     st.learned_decisions = sched.learned_decisions;
     st.fallback_decisions = sched.fallback_decisions;

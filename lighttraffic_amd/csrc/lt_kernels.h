@@ -353,6 +353,31 @@ __device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, uint32_t src, c
     }
 }
 
+// One workgroup pass over nvalid <= kBlock * WPT walkers stored
+// contiguously at cw: walk each, then append the survivors to their queues.
+template <int APP, int WPT>
+__device__ __forceinline__ void walk_walkers(const WalkArgs &a, const AppendLds &L, uint32_t *s_wsum, uint32_t src,
+                                            const Span &src_sp, const Walker *cw, uint32_t nvalid, Counters &ct) {
+    Walker w[WPT];
+    uint32_t dst[WPT];
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        const uint32_t i = (uint32_t)k * kBlock + threadIdx.x;
+        dst[k] = kEmpty;
+        if (i < nvalid) {
+            w[k] = cw[i];
+            ct.resid++;
+        } else {
+            w[k].step = 0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        if ((uint32_t)k * kBlock + threadIdx.x < nvalid) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
+    }
+    block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum);
+}
+
 // One workgroup pass: walkers [base, base + kBlock * WPT) of src's queue,
 // clipped to hi64; base is a multiple of kBlock * WPT, so the pass reads one
 // chunk.
@@ -363,25 +388,9 @@ __device__ __forceinline__ void walk_pass(const WalkArgs &a, const AppendLds &L,
     const bool chunk_ok = chunk < a.q.nchunks;
     if (!chunk_ok && threadIdx.x == 0) atomicOr(&a.q.ctl->overflow, 8u);
     const Walker *cw = a.q.slab + (size_t)(chunk_ok ? chunk : 0) * kChunk + (uint32_t)(base & (kChunk - 1));
-    Walker w[WPT];
-    uint32_t dst[WPT];
-#pragma unroll
-    for (int k = 0; k < WPT; k++) {
-        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-        dst[k] = kEmpty;
-        if (chunk_ok && i < hi64) {
-            w[k] = cw[k * kBlock + threadIdx.x];
-            ct.resid++;
-        } else {
-            w[k].step = 0;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < WPT; k++) {
-        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-        if (chunk_ok && i < hi64) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
-    }
-    block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum);
+    const uint64_t left = hi64 > base ? hi64 - base : 0;
+    const uint32_t nvalid = chunk_ok ? (uint32_t)min(left, (uint64_t)(kBlock * WPT)) : 0u;
+    walk_walkers<APP, WPT>(a, L, s_wsum, src, src_sp, cw, nvalid, ct);
 }
 
 // WPT walkers per thread per pass (1, 2 or 4): small queues use 1 so that
@@ -417,6 +426,20 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
     src_sp.rowptr = a.rowptr;
     src_sp.col = a.col;
     src_sp.zc = a.zerocopy;
+    if (a.nhost) {  // walkers of src spilled to host memory, read over PCIe
+        src_sp.nb = a.pt.node_off[src];
+        src_sp.ne = a.pt.node_off[src + 1];
+        src_sp.eb = a.pt.edge_off[src];
+        const uint64_t vend = (uint64_t)a.nhost << kChunkLog2;
+        for (uint64_t base = (uint64_t)blockIdx.x * BW; base < vend; base += (uint64_t)gridDim.x * BW) {
+            const uint32_t k = (uint32_t)(base >> kChunkLog2), local = (uint32_t)(base & (kChunk - 1));
+            if (local < a.hn[k])  // uniform per workgroup
+                walk_walkers<APP, WPT>(a, L, s_wsum, src, src_sp, a.hptr[k] + local,
+                                       min(a.hn[k] - local, (uint32_t)BW), ct);
+        }
+        flush_counters(ct, a.stats);
+        return;
+    }
     if (a.ss.ds) {  // the decision of this iteration was taken on the device
         const int32_t c = a.ss.ds->choice;
         if (c < 0) return;

@@ -71,6 +71,12 @@ CONFIGS = [
                                 "zc_batch": 5, "slice_chunks": 2}),
     ("learned-fallback-batch", 16 << 10, {"num_device_partitions": 3, "span": "resident", "policy": "learned",
                                           "zerocopy_threshold": 3000}),
+    # more walkers than the device holds: queues spill to host memory and come back
+    ("spill", 16 << 10, {"num_device_partitions": 4, "num_device_walkers": 5000}),
+    ("spill-resident-zc", 16 << 10, {"num_device_partitions": 3, "span": "resident", "zerocopy_threshold": 3000,
+                                     "num_device_walkers": 7000, "zc_batch": 16}),
+    ("spill-tiny", 16 << 10, {"num_device_partitions": 2, "num_device_walkers": 1000, "slice_chunks": 1}),
+    ("waves-device-sched", 16 << 10, {"num_device_partitions": 3, "device_schedule": 1, "num_device_walkers": 3000}),
 ]
 
 
@@ -320,3 +326,38 @@ def test_c5_full_size_sampled_walkers():
     learned.set_scorer(sc)
     assert np.array_equal(learned.pagerank(w, l), counts)
     assert learned.stats()["prefetch_issued"] > 0
+
+
+def test_spill_moves_walkers_through_host_memory(g12):
+    """Walker spill (src/Scheduler.h:53-82, 215-233): with 2,000 device walker
+    slots for 8,192 walkers, queues go to host memory and come back; results
+    equal the oracle's, and each reloaded walker adds 16 bytes of H2D."""
+    log("spill stats")
+    part = lt.Partition.build(g12, 16 << 10)
+    eng = lt.Engine(g12, part, num_device_partitions=3, num_device_walkers=2000, zerocopy_threshold=0)
+    w = 2 * g12.num_node
+    counts, ends = eng.pagerank(w, 10, endpoints=True)
+    rc, re, _ = oracle.pagerank(g12.rowptr(), g12.col(), w, 10, endpoints=True)
+    assert np.array_equal(counts, rc) and np.array_equal(ends, re)
+    st = eng.stats()
+    assert st["walker_evicting_calls"] > 0 and st["walker_loading_calls"] > 0
+    assert st["h2d_bytes_zerocopy"] > 0 and st["h2d_bytes_zerocopy"] % 16 == 0  # no zero-copy graph reads here
+    # the same engine without a cap keeps everything on the device
+    eng2 = lt.Engine(g12, part, num_device_partitions=3, zerocopy_threshold=0)
+    assert np.array_equal(eng2.pagerank(w, 10), rc)
+    assert eng2.stats()["walker_evicting_calls"] == 0 and eng2.stats()["h2d_bytes_zerocopy"] == 0
+
+
+def test_spill_with_learned_scheduler(g12):
+    log("spill learned")
+    part = lt.Partition.build(g12, 16 << 10)
+    tr = lt.Engine(g12, part, num_device_partitions=3, span="resident", record_trace=1)
+    w = 2 * g12.num_node
+    ref = tr.pagerank(w, 10)
+    sc = lt.Scorer(part, seed=2)
+    sc.add_trace(tr, True, True)
+    sc.fit(epochs=3)
+    eng = lt.Engine(g12, part, num_device_partitions=3, span="resident", policy="learned", num_device_walkers=3000)
+    eng.set_scorer(sc)
+    assert np.array_equal(eng.pagerank(w, 10), ref)
+    assert eng.stats()["walker_evicting_calls"] > 0
