@@ -152,6 +152,19 @@ struct WalkArgs {
     uint32_t hn[kMaxHostPieces];
 };
 
+// A spill round: the whole device queues of nv partitions are copied to their
+// host queues and their chunks returned to the pool.
+struct SpillArgs {
+    Queues q;
+    uint32_t nv;
+    const uint32_t *part;    // [nv] partitions spilled
+    const uint64_t *prefix;  // [nv + 1] walkers of the partitions before k
+    const uint32_t *cprefix; // [nv + 1] device chunks of the partitions before k
+    const uint32_t *hstart;  // [nv] slot of partition k's first walker in its first host chunk
+    const uint32_t *hbase;   // [nv] index of that host chunk in hptr
+    Walker *const *hptr;     // host chunks, mapped for the device
+};
+
 // Table entries of a zero-copy batch.
 struct ZcBatch {
     uint32_t n;

@@ -200,7 +200,9 @@ struct Engine {
     lt_status host_chunk(uint32_t &id);
     void host_reset();
     int32_t spill_victim(uint32_t except) const;
-    lt_status spill(uint32_t v);
+    lt_status spill_round(const std::vector<uint32_t> &victims);
+    uint8_t *d_plan = nullptr, *h_plan = nullptr;  // spill round plan (device copy, pinned staging)
+    size_t plan_cap = 0;
     lt_status ensure_room(uint64_t want, uint32_t except, uint64_t &room);
     lt_status process_host(uint32_t p, const WalkArgs &a);
 
@@ -302,6 +304,8 @@ Engine::~Engine() {
     dfree(d_meta);
     if (h_meta) (void)hipHostFree(h_meta);
     for (Walker *b : hblocks) (void)hipHostFree(b);
+    dfree(d_plan);
+    if (h_plan) (void)hipHostFree(h_plan);
     dfree(d_counts);
     dfree(d_endpoints);
     dfree(d_moves);
@@ -699,67 +703,110 @@ int32_t Engine::spill_victim(uint32_t except) const {
     return full >= 0 ? full : any;
 }
 
-// evictWalker (src/Scheduler.h:53-58): copy the whole device queue of v behind
-// its host queue, then return its chunks to the pool.  Called with the compute
-// stream idle and dcounts fresh.
-lt_status Engine::spill(uint32_t v) {
-    const uint64_t n = dcounts[v];
-    if (n == 0) return LT_OK;
-    const uint64_t nch = (n + kChunk - 1) / kChunk;
-    std::vector<uint32_t> frame(nch);
-    LT_HIP(hipMemcpyAsync(frame.data(), q.frame + (size_t)v * q.maxc, 4 * nch, hipMemcpyDeviceToHost, s_compute));
-    LT_HIP(spin_stream(s_compute));
-    hipEvent_t te = t_wevict.begin(s_compute);
-    uint64_t pos = hhead[v] + hcount[v];  // next free slot behind the host queue
-    for (uint64_t i = 0; i < n;) {
-        while ((pos >> kChunkLog2) >= hq[v].size()) {
+// evictWalker (src/Scheduler.h:53-58) for a set of partitions at once: their
+// whole device queues go behind their host queues (one gather launch writing
+// to mapped host memory), then their chunks return to the pool.  Called with
+// the compute stream idle and dcounts fresh.
+lt_status Engine::spill_round(const std::vector<uint32_t> &victims) {
+    if (victims.empty()) return LT_OK;
+    const uint32_t nv = (uint32_t)victims.size();
+    std::vector<uint64_t> prefix(nv + 1, 0);
+    std::vector<uint32_t> cprefix(nv + 1, 0), hstart(nv), hbase(nv);
+    std::vector<Walker *> hptr;
+    for (uint32_t k = 0; k < nv; k++) {
+        const uint32_t v = victims[k];
+        const uint64_t n = dcounts[v], pos = hhead[v] + hcount[v];
+        while (((pos + n - 1) >> kChunkLog2) >= hq[v].size()) {
             uint32_t id;
             LT_TRY(host_chunk(id));
             hq[v].push_back(id);
         }
-        const uint64_t doff = i & (kChunk - 1), hoff = pos & (kChunk - 1);
-        const uint64_t len = std::min<uint64_t>(n - i, std::min<uint64_t>(kChunk - doff, kChunk - hoff));
-        if (frame[i >> kChunkLog2] >= q.nchunks) {
-            set_error("queue of partition %u has no chunk %llu", v, (unsigned long long)(i >> kChunkLog2));
-            return LT_E_INTERNAL;
-        }
-        LT_HIP(hipMemcpyAsync(host_chunk_ptr(hq[v][pos >> kChunkLog2], false) + hoff,
-                              q.slab + (size_t)frame[i >> kChunkLog2] * kChunk + doff, len * sizeof(Walker),
-                              hipMemcpyDeviceToHost, s_compute));
-        i += len;
-        pos += len;
+        hstart[k] = (uint32_t)(pos & (kChunk - 1));
+        hbase[k] = (uint32_t)hptr.size();
+        for (uint64_t c = pos >> kChunkLog2; c <= (pos + n - 1) >> kChunkLog2; c++)
+            hptr.push_back(host_chunk_ptr(hq[v][c], true));
+        prefix[k + 1] = prefix[k] + n;
+        cprefix[k + 1] = cprefix[k] + (uint32_t)((n + kChunk - 1) / kChunk);
     }
-    fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, v, 0, 0xFFFFFFFFu, 1, nullptr);  // chunks back, qlen[v] = 0
+    // plan: part | prefix | cprefix | hstart | hbase | hptr, 8-byte aligned pieces
+    const size_t o_part = 0, o_prefix = (4 * (size_t)nv + 7) & ~(size_t)7, o_cprefix = o_prefix + 8 * ((size_t)nv + 1);
+    const size_t o_hstart = o_cprefix + 4 * ((size_t)nv + 1), o_hbase = o_hstart + 4 * (size_t)nv;
+    const size_t o_hptr = (o_hbase + 4 * (size_t)nv + 7) & ~(size_t)7, bytes = o_hptr + 8 * hptr.size();
+    if (plan_cap < bytes) {
+        dfree(d_plan);
+        if (h_plan) (void)hipHostFree(h_plan);
+        h_plan = nullptr;
+        LT_TRY(dmalloc(&d_plan, bytes));
+        LT_HIP(hipHostMalloc((void **)&h_plan, bytes, hipHostMallocDefault));
+        plan_cap = bytes;
+    }
+    memcpy(h_plan + o_part, victims.data(), 4 * (size_t)nv);
+    memcpy(h_plan + o_prefix, prefix.data(), 8 * ((size_t)nv + 1));
+    memcpy(h_plan + o_cprefix, cprefix.data(), 4 * ((size_t)nv + 1));
+    memcpy(h_plan + o_hstart, hstart.data(), 4 * (size_t)nv);
+    memcpy(h_plan + o_hbase, hbase.data(), 4 * (size_t)nv);
+    memcpy(h_plan + o_hptr, hptr.data(), 8 * hptr.size());
+    hipEvent_t te = t_wevict.begin(s_compute);
+    LT_HIP(hipMemcpyAsync(d_plan, h_plan, bytes, hipMemcpyHostToDevice, s_compute));
+    SpillArgs sa{};
+    sa.q = q;
+    sa.nv = nv;
+    sa.part = (const uint32_t *)(d_plan + o_part);
+    sa.prefix = (const uint64_t *)(d_plan + o_prefix);
+    sa.cprefix = (const uint32_t *)(d_plan + o_cprefix);
+    sa.hstart = (const uint32_t *)(d_plan + o_hstart);
+    sa.hbase = (const uint32_t *)(d_plan + o_hbase);
+    sa.hptr = (Walker *const *)(d_plan + o_hptr);
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((prefix[nv] + 255) / 256, (uint64_t)num_cu * 8));
+    spill_gather_kernel<<<grid, 256, 0, s_compute>>>(sa);
+    LT_HIP(hipGetLastError());
+    release_queues_kernel<<<1, kBlock, 0, s_compute>>>(sa);
     LT_HIP(hipGetLastError());
     meta_dirty = false;
     t_wevict.end(te, s_compute);
-    hcount[v] += n;
-    dcounts[v] = 0;
-    st.walker_evicting_calls++;
+    LT_HIP(spin_stream(s_compute));  // the plan and host chunks are final before they are reused
+    for (uint32_t k = 0; k < nv; k++) {
+        hcount[victims[k]] += dcounts[victims[k]];
+        dcounts[victims[k]] = 0;
+    }
+    st.walker_evicting_calls += nv;
     return LT_OK;
 }
 
-// handleWalkerPoolOverflow (src/Scheduler.h:59-82): spill device queues
-// (never `except`'s) until `room` more walkers fit under dev_cap; room is
-// `want` or at least half the capacity.
+// handleWalkerPoolOverflow (src/Scheduler.h:59-82): make room for `want`
+// walkers (at least half the capacity when want is larger) by spilling device
+// queues, never `except`'s.  Once spilling is needed it goes down to half the
+// capacity in one round, so that the reloads that follow do not spill again
+// one queue at a time.
 lt_status Engine::ensure_room(uint64_t want, uint32_t except, uint64_t &room) {
     const uint64_t target = std::max<uint64_t>(1, std::min<uint64_t>(want, dev_cap / 2));
-    while (true) {
-        LT_TRY(read_meta());
-        uint64_t live = 0;
-        for (uint32_t i = 0; i < P; i++) live += dcounts[i];
-        if (live + target <= dev_cap) {
-            room = std::min<uint64_t>(want, dev_cap - live);
-            return LT_OK;
+    LT_TRY(read_meta());
+    uint64_t live = 0;
+    for (uint32_t i = 0; i < P; i++) live += dcounts[i];
+    if (live + target > dev_cap) {
+        const uint64_t goal = std::min<uint64_t>(dev_cap - target, dev_cap / 2);
+        std::vector<uint64_t> keep(dcounts);  // spill_victim reads dcounts: mark victims by zeroing
+        std::vector<uint32_t> victims;
+        while (live > goal) {
+            const int32_t v = spill_victim(except);
+            if (v < 0) break;
+            live -= dcounts[v];
+            victims.push_back((uint32_t)v);
+            dcounts[v] = 0;
         }
-        const int32_t v = spill_victim(except);
-        if (v < 0) {
+        dcounts = keep;
+        LT_TRY(spill_round(victims));
+        LT_TRY(read_meta());
+        live = 0;
+        for (uint32_t i = 0; i < P; i++) live += dcounts[i];
+        if (live + target > dev_cap) {
             set_error("no walkers left to spill (%llu on the device, capacity %llu)", (unsigned long long)live,
                       (unsigned long long)dev_cap);
             return LT_E_INTERNAL;
         }
-        LT_TRY(spill((uint32_t)v));
     }
+    room = std::min<uint64_t>(want, dev_cap - live);
+    return LT_OK;
 }
 
 // walkerLoadingPhase + computeOffMemory (src/Scheduler.h:215-233,

@@ -601,6 +601,60 @@ __global__ __launch_bounds__(kBlock) void fixup_kernel(Queues q, uint32_t src, u
     pack_meta(q);
 }
 
+// Copy the queues of a spill round into host memory: walker i of partition
+// k goes to slot hstart[k] + i of its host chunks (vector stores over PCIe).
+__global__ __launch_bounds__(256) void spill_gather_kernel(SpillArgs s) {
+    const uint64_t total = s.prefix[s.nv];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        uint32_t lo = 0, hi = s.nv;  // last k with prefix[k] <= idx
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s.prefix[mid] <= idx) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t i = idx - s.prefix[lo];
+        const uint32_t c = s.q.frame[(size_t)s.part[lo] * s.q.maxc + (uint32_t)(i >> kChunkLog2)];
+        if (c >= s.q.nchunks) {
+            atomicOr(&s.q.ctl->overflow, 8u);
+            continue;
+        }
+        const uint64_t hp = s.hstart[lo] + i;
+        s.hptr[s.hbase[lo] + (uint32_t)(hp >> kChunkLog2)][hp & (kChunk - 1)] =
+            s.q.slab[(size_t)c * kChunk + (i & (kChunk - 1))];
+    }
+}
+
+// Return the spilled partitions' chunks to the free stack and empty their
+// queues; pack the queue lengths.  One workgroup, after spill_gather_kernel.
+__global__ __launch_bounds__(kBlock) void release_queues_kernel(SpillArgs s) {
+    __shared__ uint32_t s_top;
+    if (threadIdx.x == 0) {
+        s_top = s.q.ctl->top;
+        if (s_top + s.cprefix[s.nv] > s.q.nchunks) {
+            s.q.ctl->overflow |= 16u;
+            s_top = 0xFFFFFFFFu;
+        }
+    }
+    __syncthreads();
+    const uint32_t top = s_top;
+    if (top != 0xFFFFFFFFu) {
+        for (uint32_t k = 0; k < s.nv; k++) {
+            const uint32_t nch = s.cprefix[k + 1] - s.cprefix[k];
+            uint32_t *f = s.q.frame + (size_t)s.part[k] * s.q.maxc;
+            for (uint32_t j = threadIdx.x; j < nch; j += blockDim.x) {
+                s.q.free_stack[top + s.cprefix[k] + j] = f[j];
+                f[j] = kEmpty;
+            }
+        }
+        for (uint32_t k = threadIdx.x; k < s.nv; k += blockDim.x) qlen_of(s.q, s.part[k]) = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && top != 0xFFFFFFFFu) s.q.ctl->top = top + s.cprefix[s.nv];
+    __syncthreads();
+    pack_meta(s.q);
+}
+
 // Empty every queue and put all chunks on the free stack.
 __global__ void reset_queues_kernel(Queues q, uint32_t P) {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
