@@ -58,6 +58,12 @@ def test_pagerank_cli_capped_cache(graph_file):
         assert "result correct" in out
     out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, "--zerocopy-threshold", 1 << 40)
     assert grep_num(r"graph loading time: [0-9.]+ ms, calls: (\d+)", out) == 0
+    # -b below -w: walker queues spill to host memory and come back (the
+    # reference's walker loading / evicting lines, src/Scheduler.h:440-453)
+    out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 100000, "-b", 20000, "--timers")
+    assert "result correct" in out
+    assert grep_num(r"walker loading time: [0-9.]+ ms, calls: (\d+)", out) > 0
+    assert grep_num(r"walker evicting time: [0-9.]+ ms, calls: (\d+)", out) > 0
 
 
 def test_ppr_and_genericwalk_cli(graph_file):
