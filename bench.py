@@ -101,6 +101,15 @@ def measure(eng, wl, w, warmup, steps, label, timed_engine=None):
     return res
 
 
+def timed_walk(eng, wl, w):
+    """Walk-kernel time of one job for the roofline: a warm-up job, then one
+    job on an engine with HIP event timers around every walk launch."""
+    run_job(eng, wl, w)
+    tst = run_job(eng, wl, w)
+    return {"kernel_ms": tst["walk_kernel_ms"], "launches": tst["kernel_launches"],
+            "alg_bytes": algorithmic_bytes(tst), "timed_job_ms": tst["running_ms"]}
+
+
 def measured_traffic(workload, span, alg_bytes_per_launch):
     """HBM bytes per walk-kernel launch from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
@@ -215,35 +224,41 @@ def main():
            "data": "synthetic R-MAT (a,b,c,d=.57/.19/.19/.05), generated on the GPU, Philox walks"}
 
     if args.workload == "c5":
+        # One engine at a time (plus the scorer's stream): HIP maps streams onto
+        # GPU_MAX_HW_QUEUES hardware queues (4 by default), and an engine whose
+        # copy and compute streams share one loses its copy/compute overlap.
+        sc = lt.Scorer(part, seed=1)
         plain = {}
-        engines = {}
+        prep_s = 0.0
         for span in ("partition", "resident"):
             eng = lt.Engine(g, part, span=span, record_trace=1, **base)
             plain[span] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}")
-            engines[span] = eng
+            t1 = time.perf_counter()
+            if span == "partition":
+                n_next = sc.add_trace(eng, use_next=True, use_keep=True)
+            else:
+                n_res = sc.add_trace(eng, use_next=True, use_keep=False)
+                ref = eng.pagerank(w, wl["l"])
+            prep_s += time.perf_counter() - t1
+            eng.close()
         # plain rules with the learned policy's non-learned settings: byte break-even
         # zero-copy threshold and zero-copy batches
         eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16, **base)
         plain["tuned"] = measure(eng, wl, w, args.warmup, args.steps, "plain/resident S/128 zc-batch 16")
         eng.close()
         t1 = time.perf_counter()
-        sc = lt.Scorer(part, seed=1)
-        n_next = sc.add_trace(engines["partition"], use_next=True, use_keep=True)
-        n_res = sc.add_trace(engines["resident"], use_next=True, use_keep=False)
         loss = sc.fit(epochs=args.epochs, batch=32, lr=1e-3)
-        train_s = time.perf_counter() - t1
+        train_s = time.perf_counter() - t1 + prep_s
         log(f"scorer trained on {n_next + n_res} decisions in {train_s:.1f} s, losses {loss.tolist()}")
-        ref = engines["resident"].pagerank(w, wl["l"])
-        for e in engines.values():
-            e.close()
         eng = lt.Engine(g, part, span="resident", policy="learned", **base)
         eng.set_scorer(sc)
+        learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident")
+        same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
+        eng.close()
         teng = lt.Engine(g, part, span="resident", policy="learned", event_timers=1, **base)
         teng.set_scorer(sc)
-        run_job(teng, wl, w)
-        learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident", timed_engine=teng)
+        learned.update(timed_walk(teng, wl, w))
         teng.close()
-        same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
         best_plain = min(plain.values(), key=lambda m: m["ms_per_step"])
         ref_plain = plain["partition"]
         out.update({

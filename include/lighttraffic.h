@@ -81,7 +81,12 @@ typedef struct {
                                        partitions below the zero-copy threshold
                                        over PCIe in one launch (at most 16;
                                        0 = 16 for LEARNED, 1 otherwise)         */
-    int32_t  reserved[2];
+    int32_t  rowptr_resident;       /* 1: keep the whole rowptr array (8 B per
+                                       vertex) in HBM, taken from the partition
+                                       cache's slots, so a zero-copied move reads
+                                       only its column entry over PCIe (default 0:
+                                       off)                                     */
+    int32_t  reserved[1];
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines

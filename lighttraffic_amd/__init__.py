@@ -51,7 +51,7 @@ class Config(ctypes.Structure):
                 ("seed", _u64), ("policy", _i32), ("pipeline", _i32), ("span", _i32), ("event_timers", _i32),
                 ("record_trace", _i32), ("prefetch_depth", _i32), ("prefetch_stick_pct", _i32),
                 ("device_schedule", _i32), ("slice_chunks", _i32), ("zc_batch", _i32),
-                ("reserved", _i32 * 2)]
+                ("rowptr_resident", _i32), ("reserved", _i32 * 1)]
 
 
 class Stats(ctypes.Structure):

@@ -88,6 +88,7 @@ struct AppArgs {
     uint32_t *endpoints;  // u32[N] indexed by walker id, or nullptr
     uint32_t *moves_out;  // u32[N] (genericwalk), or nullptr
     uint32_t *paths;      // genericwalk: u32[N][length + 1], vertex after k moves at [id][k], or nullptr
+    uint32_t rp_host;     // 1: zero-copied spans read rowptr over PCIe too; 0: rowptr is in HBM
     const uint64_t *q_off;  // batched ppr: [nq+1] first walker id of each query
     const uint32_t *q_src;  // batched ppr: [nq] sources
     uint32_t nq;
@@ -118,9 +119,10 @@ struct SchedState {
     uint8_t *slots;
     uint64_t slot_bytes;
     uint32_t nslots;
-    const uint64_t *zc_rowptr;  // device view of the pinned host graph
-    const uint32_t *zc_col;
+    const uint64_t *zc_rowptr;  // rowptr for zero-copied spans: the pinned host graph's, or an HBM copy
+    const uint32_t *zc_col;     // device view of the pinned host graph's col
     uint64_t zc_threshold;
+    uint32_t rp_host;           // 1: zc_rowptr is host memory (counted as H2D by the copies)
 };
 
 struct WalkArgs {

@@ -232,6 +232,9 @@ struct Engine {
     int32_t *d_slot_of = nullptr, *d_part_in_slot = nullptr;
     uint8_t *d_resident = nullptr;
     uint8_t *d_zc_part = nullptr;  // [P] partitions of the running zero-copy batch
+    bool rp_resident = false;      // rowptr of the whole graph kept in HBM (zc_rowptr points there)
+    uint64_t *d_rowptr_all = nullptr;
+    uint64_t pending_h2d = 0;      // one-off H2D bytes not yet charged to a run
     uint32_t zc_batch = 1;         // partitions per zero-copy launch
     uint32_t slice = kSliceChunks;
     std::vector<hipEvent_t> pool_events;  // reusable prefetch-completion events
@@ -318,6 +321,7 @@ Engine::~Engine() {
     dfree(d_part_in_slot);
     dfree(d_resident);
     dfree(d_zc_part);
+    dfree(d_rowptr_all);
     if (s_compute) (void)hipStreamDestroy(s_compute);
     if (s_copy) (void)hipStreamDestroy(s_copy);
 }
@@ -399,6 +403,17 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     // HBM cache slots (GPUGraph, src/GraphLoader.h:513-520)
     nslots = cfg.num_device_partitions == 0 ? P : std::min(cfg.num_device_partitions, P);
     const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
+    // The whole rowptr array in HBM, paid for with slots of the same cache:
+    // a zero-copied move then reads only its column entry over PCIe.
+    const uint64_t rp_bytes = 8ull * ((uint64_t)V + 1);
+    const uint32_t rp_slots = (uint32_t)((rp_bytes + slot_bytes - 1) / slot_bytes);
+    // Off unless asked for: on C5 the slots it takes cost about the PCIe bytes
+    // it saves at the default threshold.  At least two slots stay, so that
+    // eviction, which spares the partition loaded last, has a victim.
+    rp_resident = cfg.rowptr_resident > 0;
+    if (nslots == P) rp_resident = false;  // nothing is zero-copied with every partition cached
+    if (rp_resident && nslots >= rp_slots + 2) nslots -= rp_slots;
+    else rp_resident = false;
     LT_TRY(dmalloc(&d_slots, (size_t)nslots * slot_bytes));
     slot_of.assign(P, -1);
     part_in_slot.assign(nslots, -1);
@@ -414,6 +429,12 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     else zc_threshold = cfg.policy == LT_SCHED_LEARNED ? part.bytes / 128 : part.bytes / 128 / 2;
     LT_HIP(hipHostGetDevicePointer((void **)&zc_rowptr, g->rowptr, 0));
     LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
+    if (rp_resident) {
+        LT_TRY(dmalloc(&d_rowptr_all, (size_t)V + 1));
+        LT_HIP(hipMemcpy(d_rowptr_all, g->rowptr, rp_bytes, hipMemcpyHostToDevice));
+        pending_h2d = rp_bytes;  // charged to the first run
+        zc_rowptr = d_rowptr_all;
+    }
 
     // queue lengths one 128-byte line apart, then PoolCtl on its own line; the
     // fix-up kernel packs both into h_meta (pinned, mapped) for the host
@@ -571,10 +592,15 @@ lt_status Engine::load_async(uint32_t p) {
     const uint64_t eb = part.edge_off[p], ee = part.edge_off[p + 1];
     const size_t rbytes = ((size_t)ne - nb + 1) * 8, cbytes = (size_t)(ee - eb) * 4;
     hipEvent_t t = t_load.begin(s_copy);
-    LT_HIP(hipMemcpyAsync(base, g->rowptr + nb, rbytes, hipMemcpyHostToDevice, s_copy));
+    if (rp_resident) {
+        LT_HIP(hipMemcpyAsync(base, d_rowptr_all + nb, rbytes, hipMemcpyDeviceToDevice, s_copy));
+    } else {
+        LT_HIP(hipMemcpyAsync(base, g->rowptr + nb, rbytes, hipMemcpyHostToDevice, s_copy));
+        st.h2d_bytes_explicit += rbytes;
+    }
     if (cbytes) LT_HIP(hipMemcpyAsync(base + rbytes, g->col + eb, cbytes, hipMemcpyHostToDevice, s_copy));
     t_load.end(t, s_copy);
-    st.h2d_bytes_explicit += rbytes + cbytes;
+    st.h2d_bytes_explicit += cbytes;
     slot_of[p] = (int32_t)s;
     part_in_slot[s] = (int32_t)p;
     return LT_OK;
@@ -1247,6 +1273,7 @@ SchedState Engine::sched_state(bool device) const {
     ss.slot_bytes = (part.bytes + 255) & ~255ull;
     ss.nslots = nslots;
     ss.zc_rowptr = zc_rowptr;
+    ss.rp_host = rp_resident ? 0 : 1;
     ss.zc_col = zc_col;
     ss.zc_threshold = zc_threshold;
     return ss;
@@ -1348,6 +1375,8 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     LT_HIP(hipSetDevice(device));
     memset(&st, 0, sizeof(st));
     spill_h2d_bytes = 0;
+    st.h2d_bytes_explicit = pending_h2d;  // the rowptr upload, once
+    pending_h2d = 0;
     trace.clear();
     trace_counts.clear();
     trace_resident.clear();
@@ -1382,6 +1411,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     app.endpoints = endpoints_out ? d_endpoints : nullptr;
     app.moves_out = moves_out ? d_moves : nullptr;
     app.paths = rs.paths_out ? d_paths : nullptr;
+    app.rp_host = rp_resident ? 0 : 1;
     if (rs.app == APP_PAGERANK) app.thresh = prob_threshold(1.0 - (double)0.85f);
     else if (rs.app == APP_PPR) app.thresh = prob_threshold((double)rs.prob);
     if (rs.app == APP_PPR && app.thresh == 0) {

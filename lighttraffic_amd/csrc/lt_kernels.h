@@ -253,10 +253,11 @@ __device__ __forceinline__ void step_pagerank(Walker &w, const Span &sp, const A
         const uint64_t b = r[0], d = r[1] - b;
         if (d == 0) {
             tele = true;
+            if (sp.zc) ct.zc_reads += a.rp_host;
         } else {
             w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
             ct.moves++;
-            if (sp.zc) { ct.zc_moves++; ct.zc_reads += 2; }
+            if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
         }
     }
     if (tele) {
@@ -280,12 +281,12 @@ __device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArg
         const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_PPR);
         w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
         ct.moves++;
-        if (sp.zc) { ct.zc_moves++; ct.zc_reads += 2; }
+        if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
     } else {
         if (a.counts) count_visit(a, w.cur);
         ct.visits++;
         w.step = 0;
-        if (sp.zc) ct.zc_reads++;
+        if (sp.zc) ct.zc_reads += a.rp_host;
     }
 }
 
@@ -300,7 +301,7 @@ __device__ __forceinline__ void step_generic(Walker &w, const Span &sp, const Ap
         const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_GENERIC);
         w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
         ct.moves++;
-        if (sp.zc) { ct.zc_moves++; ct.zc_reads += 2; }
+        if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
         w.step -= 1;
         if (w.step == 0) {
             if (a.moves_out) a.moves_out[walker_id(w)] = a.length;
@@ -309,7 +310,7 @@ __device__ __forceinline__ void step_generic(Walker &w, const Span &sp, const Ap
     } else {
         if (a.moves_out) a.moves_out[walker_id(w)] = a.length - w.step;
         w.step = 0;
-        if (sp.zc) ct.zc_reads++;
+        if (sp.zc) ct.zc_reads += a.rp_host;
     }
 }
 
@@ -807,7 +808,7 @@ __global__ __launch_bounds__(1024) void decide_kernel(Queues q, SchedState ss, P
     ds->slot = (int32_t)slot;
     ds->load = 1;
     ds->loads++;
-    ds->h2d_bytes += n1 * 8 + m * 4;
+    ds->h2d_bytes += n1 * 8 * ss.rp_host + m * 4;
 }
 
 // Copy the decided partition from pinned host memory into its slot: rowptr

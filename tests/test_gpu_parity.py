@@ -77,6 +77,12 @@ CONFIGS = [
                                      "num_device_walkers": 7000, "zc_batch": 16}),
     ("spill-tiny", 16 << 10, {"num_device_partitions": 2, "num_device_walkers": 1000, "slice_chunks": 1}),
     ("waves-device-sched", 16 << 10, {"num_device_partitions": 3, "device_schedule": 1, "num_device_walkers": 3000}),
+    # rowptr of the whole graph in HBM: zero-copied moves read only col over PCIe
+    ("rp-resident-zc", 16 << 10, {"num_device_partitions": 6, "zerocopy_threshold": 4000, "rowptr_resident": 1}),
+    ("rp-resident-batch", 16 << 10, {"num_device_partitions": 6, "span": "resident", "zerocopy_threshold": 1 << 40,
+                                     "zc_batch": 16, "rowptr_resident": 1}),
+    ("rp-resident-device", 16 << 10, {"num_device_partitions": 6, "device_schedule": 1, "zerocopy_threshold": 3000,
+                                      "span": "resident", "rowptr_resident": 1}),
 ]
 
 
@@ -361,3 +367,23 @@ def test_spill_with_learned_scheduler(g12):
     eng.set_scorer(sc)
     assert np.array_equal(eng.pagerank(w, 10), ref)
     assert eng.stats()["walker_evicting_calls"] > 0
+
+
+def test_resident_rowptr_halves_zero_copy_reads(g12):
+    """With rowptr in HBM a zero-copied move reads one 64-byte line (its col
+    entry) instead of two; the rowptr upload is charged once, to the first run."""
+    log("resident rowptr bytes")
+    part = lt.Partition.build(g12, 16 << 10)
+    w = 2 * g12.num_node
+    kw = dict(num_device_partitions=6, zerocopy_threshold=1 << 40)
+    a = lt.Engine(g12, part, rowptr_resident=-1, **kw)
+    b = lt.Engine(g12, part, rowptr_resident=1, **kw)
+    ca, cb = a.pagerank(w, 10), b.pagerank(w, 10)
+    assert np.array_equal(ca, cb)
+    sa, sb = a.stats(), b.stats()
+    assert sa["zerocopy_moves"] == sb["zerocopy_moves"] > 0
+    assert sa["h2d_bytes_zerocopy"] == 128 * sa["zerocopy_moves"]
+    assert sb["h2d_bytes_zerocopy"] == 64 * sb["zerocopy_moves"]
+    assert sb["h2d_bytes_explicit"] == 8 * (g12.num_node + 1)  # the upload; nothing else is copied
+    b.pagerank(w, 10)
+    assert b.stats()["h2d_bytes_explicit"] == 0
