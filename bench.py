@@ -19,8 +19,10 @@ self loops and isolated vertices removed):
       log-uniform in [1e3, 1e6]), 8 GB cache; configuration chosen by the
       autotuner (partition size 16/32/64 MB, span, host or device scheduling)
 For c5 the learned scorer is trained here, on the GPU, from the traces of the
-plain scheduler's warm-up runs; `value` is the learned scheduler's steps/s and
-the plain scheduler's figures are reported beside it.
+plain scheduler's warm-up runs; `value` is the learned scheduler's steps/s with
+partitions copied packed (gap-coded columns unpacked in HBM).  Beside it: the
+plain scheduler as the reference runs it (raw copies), the plain scheduler with
+the same packed copies, and the learned scheduler with raw copies.
 """
 import argparse
 import json
@@ -230,6 +232,7 @@ def main():
         sc = lt.Scorer(part, seed=1)
         plain = {}
         prep_s = 0.0
+        keep = ("steps_per_s", "ms_per_step", "h2d_bytes_per_step", "h2d_GB_per_job", "pcie_GBps")
         for span in ("partition", "resident"):
             eng = lt.Engine(g, part, span=span, record_trace=1, **base)
             plain[span] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}")
@@ -241,10 +244,16 @@ def main():
                 ref = eng.pagerank(w, wl["l"])
             prep_s += time.perf_counter() - t1
             eng.close()
-        # plain rules with the learned policy's non-learned settings: byte break-even
-        # zero-copy threshold and zero-copy batches
-        eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16, **base)
-        plain["tuned"] = measure(eng, wl, w, args.warmup, args.steps, "plain/resident S/128 zc-batch 16")
+            # the same rules with packed partition copies
+            eng = lt.Engine(g, part, span=span, pack_partitions=1, **base)
+            plain[span + "_packed"] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span} packed")
+            eng.close()
+        # plain rules with all of the learned configuration's non-learned settings:
+        # byte break-even zero-copy threshold, zero-copy batches, packed copies
+        eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
+                        pack_partitions=1, **base)
+        plain["tuned_packed"] = measure(eng, wl, w, args.warmup, args.steps,
+                                        "plain/resident S/128 zc-batch 16 packed")
         eng.close()
         t1 = time.perf_counter()
         loss = sc.fit(epochs=args.epochs, batch=32, lr=1e-3)
@@ -252,41 +261,48 @@ def main():
         log(f"scorer trained on {n_next + n_res} decisions in {train_s:.1f} s, losses {loss.tolist()}")
         eng = lt.Engine(g, part, span="resident", policy="learned", **base)
         eng.set_scorer(sc)
-        learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident")
+        learned_raw = measure(eng, wl, w, args.warmup, args.steps, "learned/resident")
+        eng.close()
+        eng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=1, **base)
+        eng.set_scorer(sc)
+        learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident packed")
         same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
         eng.close()
-        teng = lt.Engine(g, part, span="resident", policy="learned", event_timers=1, **base)
+        teng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=1, event_timers=1, **base)
         teng.set_scorer(sc)
         learned.update(timed_walk(teng, wl, w))
         teng.close()
-        best_plain = min(plain.values(), key=lambda m: m["ms_per_step"])
         ref_plain = plain["partition"]
+        same_copy = {k: v for k, v in plain.items() if k.endswith("packed")}
+        best_name = min(same_copy, key=lambda k: same_copy[k]["ms_per_step"])
+        best_plain = same_copy[best_name]
+
+        def vs(m):
+            return {"speedup": learned["steps_per_s"] / m["steps_per_s"],
+                    "h2d_bytes_ratio": learned["h2d_bytes_per_step"] / m["h2d_bytes_per_step"]}
+
         out.update({
             "value": learned["steps_per_s"],
             "ms_per_step": learned["ms_per_step"],
             "config": {"workload": desc, "scheduler": "learned (prefetch depth 2, span resident, zero-copy "
-                                                      "threshold S/128, zero-copy batches of 16)",
+                                                      "threshold S/128, zero-copy batches of 16), partitions "
+                                                      "copied packed (gap-coded columns, unpacked in HBM)",
                        "device": info["name"]},
             "h2d_bytes_per_step": learned["h2d_bytes_per_step"],
-            "plain": {"span_partition": {k: ref_plain[k] for k in ("steps_per_s", "ms_per_step",
-                                                                    "h2d_bytes_per_step", "h2d_GB_per_job",
-                                                                    "pcie_GBps")},
-                      "span_resident": {k: plain["resident"][k] for k in ("steps_per_s", "ms_per_step",
-                                                                         "h2d_bytes_per_step", "h2d_GB_per_job",
-                                                                         "pcie_GBps")},
-                      "span_resident_S128_zcbatch16": {k: plain["tuned"][k] for k in (
-                          "steps_per_s", "ms_per_step", "h2d_bytes_per_step", "h2d_GB_per_job", "pcie_GBps")}},
-            "learned": {k: learned[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
-                                                "h2d_GB_per_job", "pcie_GBps")},
+            "plain": {"span_partition": {k: ref_plain[k] for k in keep},
+                      "span_partition_packed": {k: plain["partition_packed"][k] for k in keep},
+                      "span_resident": {k: plain["resident"][k] for k in keep},
+                      "span_resident_packed": {k: plain["resident_packed"][k] for k in keep},
+                      "span_resident_S128_zcbatch16_packed": {k: plain["tuned_packed"][k] for k in keep}},
+            "learned": {"packed": {k: learned[k] for k in keep}, "unpacked": {k: learned_raw[k] for k in keep}},
             "learned_vs_plain": {
                 "speedup_vs_reference_plain": learned["steps_per_s"] / ref_plain["steps_per_s"],
                 "h2d_bytes_ratio_vs_reference_plain": learned["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"],
-                "speedup_vs_plain_resident": learned["steps_per_s"] / plain["resident"]["steps_per_s"],
-                "h2d_bytes_ratio_vs_plain_resident": (learned["h2d_bytes_per_step"] /
-                                                      plain["resident"]["h2d_bytes_per_step"]),
-                "speedup_vs_best_plain": learned["steps_per_s"] / best_plain["steps_per_s"],
-                "h2d_bytes_ratio_vs_best_plain": learned["h2d_bytes_per_step"] / best_plain["h2d_bytes_per_step"],
-                "best_plain": [k for k, v in plain.items() if v is best_plain][0],
+                "vs_plain_same_copies": {k: vs(v) for k, v in same_copy.items()},
+                "best_plain_same_copies": best_name,
+                "unpacked_learned_vs_reference_plain": {
+                    "speedup": learned_raw["steps_per_s"] / ref_plain["steps_per_s"],
+                    "h2d_bytes_ratio": learned_raw["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"]},
                 "visit_counts_identical": same,
                 "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
                 "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],

@@ -86,7 +86,11 @@ typedef struct {
                                        cache's slots, so a zero-copied move reads
                                        only its column entry over PCIe (default 0:
                                        off)                                     */
-    int32_t  reserved[1];
+    int32_t  pack_partitions;       /* 1: copy partition columns gap-coded
+                                       (lt_partition_pack) and unpack them in
+                                       HBM; a partition that does not get
+                                       smaller is copied as is (default 0: off;
+                                       device_schedule: unused)                 */
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines
@@ -114,7 +118,9 @@ typedef struct {
     uint64_t fallback_decisions;    /* evictions that fell back to the plain rule */
     uint64_t prefetch_issued;       /* partitions copied ahead of their turn     */
     uint64_t prefetch_hits;         /* prefetched partitions picked next         */
-    uint64_t reserved[4];
+    uint64_t h2d_bytes_unpacked;    /* the bytes the explicit copies stand for
+                                       (= h2d_bytes_explicit without packing)   */
+    uint64_t reserved[3];
 } lt_stats;
 
 /* ---- errors ------------------------------------------------------------- */
@@ -159,6 +165,23 @@ const uint64_t *lt_partition_edge_offsets(const lt_partition *p);
 /* Host findPartition (PartitionStrategy.h:151-168): partition holding v. */
 uint32_t lt_partition_find(const lt_partition *p, uint32_t v);
 void lt_partition_free(lt_partition *p);
+/* Packed image of partition part's column slice, the form in which an engine
+ * with pack_partitions = 1 copies it to HBM (replaces the raw col copy of
+ * GPUGraph::loadAsync, src/GraphLoader.h:531-552; the rowptr slice is copied
+ * as is).  For n rows and m edges, with vbits = bits of V-1 (at least 1):
+ *   u8  code[n], zero-padded to a multiple of 4 bytes: 0..32 = bit width of
+ *       the row's gaps, 255 = row stored absolute;
+ *   u32 chunk_word[ceil(m/64)]: first word of chunk c (partition-local edges
+ *       64c .. 64c+63) in words[];
+ *   u32 words[]: each chunk's bit stream (LSB first, from its first word),
+ *       then two zero words.
+ * An edge is an anchor when it is its row's first, its chunk's first, or its
+ * row's code is 255; an anchor holds its column in vbits bits, any other edge
+ * holds the gap to the previous column in its row's width.  *bytes_out gets
+ * the image size; the image is written when out is not NULL and cap is large
+ * enough. */
+lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
+                            uint64_t *bytes_out);
 
 /* ---- engine (src/LightTraffic.h:19-44, src/Device.h:22-59) -------------- */
 lt_status lt_engine_create(const lt_graph *g, const lt_partition *p, const lt_config *cfg,
@@ -195,6 +218,11 @@ lt_status lt_run_genericwalk_paths(lt_engine *e, uint64_t w, uint32_t l, uint32_
 lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, const uint64_t *counts,
                            float prob, uint32_t *endpoints_out);
 lt_status lt_engine_stats(const lt_engine *e, lt_stats *out);
+/* Partition part as the engine's partition copy writes it into an HBM cache
+ * slot (packed or not, per the engine's config), copied back to the host:
+ * rowptr_out u64[n+1], col_out u32[m].  Leaves the cache untouched (no
+ * counterpart in the reference; it lets a caller check a slot's bytes). */
+lt_status lt_engine_fetch_partition(lt_engine *e, uint32_t part, uint64_t *rowptr_out, uint32_t *col_out);
 /* Batched-PPR query table (no counterpart in the reference): query q draws
  * Philox(seed; q, 0, TAG_QUERY) = (x0,x1,x2,x3); its source is the row that
  * owns edge bounded(x0,x1,E) (sources proportional to out-degree), and its

@@ -51,7 +51,7 @@ class Config(ctypes.Structure):
                 ("seed", _u64), ("policy", _i32), ("pipeline", _i32), ("span", _i32), ("event_timers", _i32),
                 ("record_trace", _i32), ("prefetch_depth", _i32), ("prefetch_stick_pct", _i32),
                 ("device_schedule", _i32), ("slice_chunks", _i32), ("zc_batch", _i32),
-                ("rowptr_resident", _i32), ("reserved", _i32 * 1)]
+                ("rowptr_resident", _i32), ("pack_partitions", _i32)]
 
 
 class Stats(ctypes.Structure):
@@ -66,7 +66,7 @@ class Stats(ctypes.Structure):
                 ("teleports", _u64), ("visits", _u64), ("residencies", _u64), ("zerocopy_moves", _u64),
                 ("nominal_steps", _u64), ("kernel_launches", _u64), ("walk_kernel_ms", ctypes.c_double),
                 ("learned_decisions", _u64), ("fallback_decisions", _u64), ("prefetch_issued", _u64),
-                ("prefetch_hits", _u64), ("reserved", _u64 * 4)]
+                ("prefetch_hits", _u64), ("h2d_bytes_unpacked", _u64), ("reserved", _u64 * 3)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
@@ -106,6 +106,8 @@ _sig("lt_partition_node_offsets", _vp, _vp)
 _sig("lt_partition_edge_offsets", _vp, _vp)
 _sig("lt_partition_find", _u32, _vp, _u32)
 _sig("lt_partition_free", None, _vp)
+_sig("lt_partition_pack", ctypes.c_int, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64))
+_sig("lt_engine_fetch_partition", ctypes.c_int, _vp, _u32, _vp, _vp)
 _sig("lt_engine_create", ctypes.c_int, _vp, _vp, ctypes.POINTER(Config), ctypes.POINTER(_vp))
 _sig("lt_engine_destroy", None, _vp)
 _sig("lt_run_pagerank", ctypes.c_int, _vp, _u64, _u32, _u32, _vp, _vp)
@@ -293,6 +295,16 @@ class Partition:
     def find(self, v):
         return _lib.lt_partition_find(self._h, v)
 
+    def pack(self, graph, part):
+        """Packed image (u8 array) of partition `part`'s column slice, the
+        form an engine with pack_partitions=1 copies to HBM."""
+        n = _u64()
+        _check(_lib.lt_partition_pack(graph._h, self._h, part, None, 0, ctypes.byref(n)), "lt_partition_pack")
+        out = np.zeros(n.value, np.uint8)
+        _check(_lib.lt_partition_pack(graph._h, self._h, part, out.ctypes.data, n.value, ctypes.byref(n)),
+               "lt_partition_pack")
+        return out
+
     def close(self):
         if self._h and _lib is not None:
             _lib.lt_partition_free(self._h)
@@ -346,6 +358,17 @@ class Engine:
         _check(_lib.lt_run_ppr_batch(self._h, len(sources), sources.ctypes.data, counts.ctypes.data, prob,
                                      _ptr(ep)), "lt_run_ppr_batch")
         return ep
+
+    def fetch_partition(self, part, node_offsets, edge_offsets):
+        """(rowptr u64[n+1], col u32[m]) of partition `part` as the engine's
+        partition copy writes them into an HBM slot."""
+        n = int(node_offsets[part + 1]) - int(node_offsets[part])
+        m = int(edge_offsets[part + 1]) - int(edge_offsets[part])
+        rowptr = np.zeros(n + 1, np.uint64)
+        col = np.zeros(max(m, 1), np.uint32)
+        _check(_lib.lt_engine_fetch_partition(self._h, part, rowptr.ctypes.data, col.ctypes.data),
+               "lt_engine_fetch_partition")
+        return rowptr, col[:m]
 
     def stats(self):
         s = Stats()

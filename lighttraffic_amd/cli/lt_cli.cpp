@@ -99,6 +99,8 @@ int main(int argc, char **argv) {
     if (c.prefetch_depth >= 0) cfg.prefetch_depth = c.prefetch_depth;
     cfg.zc_batch = c.zc_batch;
     cfg.device_schedule = c.device_schedule ? 1 : 0;
+    cfg.pack_partitions = c.pack ? 1 : 0;
+    cfg.rowptr_resident = c.rowptr_resident ? 1 : 0;
     lt_engine *e = nullptr;
     s = lt_engine_create(g, part, &cfg, &e);
     if (s == LT_E_ARG && P > 2900) {
@@ -146,6 +148,8 @@ int main(int argc, char **argv) {
            (unsigned long)st.walker_evicting_calls);
     printf("H2D bytes: explicit %llu, zero-copy %llu\n", (unsigned long long)st.h2d_bytes_explicit,
            (unsigned long long)st.h2d_bytes_zerocopy);
+    if (st.h2d_bytes_unpacked != st.h2d_bytes_explicit)
+        printf("H2D bytes unpacked: %llu\n", (unsigned long long)st.h2d_bytes_unpacked);
     printf("steps: nominal %llu, moves %llu, teleports %llu\n", (unsigned long long)st.nominal_steps,
            (unsigned long long)st.moves, (unsigned long long)st.teleports);
     printf("throughput: %f steps/s\n", st.running_ms > 0 ? st.nominal_steps / (st.running_ms * 1e-3) : 0.0);

@@ -15,8 +15,8 @@
 // Flags added by this engine (they only add behaviour):
 //   --scheduler greedy|fifo|learned, --scorer <weights file> (learned),
 //   --no-pipeline, --zerocopy-threshold <n>, --span partition|resident,
-//   --prefetch-depth <n>, --zc-batch <n>, --device-schedule, --seed <n>,
-//   --timers, --json, --queries <file>
+//   --prefetch-depth <n>, --zc-batch <n>, --device-schedule, --pack,
+//   --rowptr-resident, --seed <n>, --timers, --json, --queries <file>
 #pragma once
 #include <stdint.h>
 #include <stdio.h>
@@ -42,6 +42,8 @@ struct CliConfig {
     int prefetch_depth = -1;  // -1: library default
     int zc_batch = 0;         // 0: library default
     bool device_schedule = false;
+    bool pack = false;           // copy partitions packed (lt_config.pack_partitions)
+    bool rowptr_resident = false;
     bool pipeline = true;
     bool has_zc = false;
     uint64_t zc = 0;
@@ -96,6 +98,8 @@ static bool cli_parse(int argc, char **argv, CliConfig &c) {
         else if (a == "--prefetch-depth") c.prefetch_depth = cli_num<int>(next("prefetch-depth"));
         else if (a == "--zc-batch") c.zc_batch = cli_num<int>(next("zc-batch"));
         else if (a == "--device-schedule") c.device_schedule = true;
+        else if (a == "--pack") c.pack = true;
+        else if (a == "--rowptr-resident") c.rowptr_resident = true;
         else if (a == "--no-pipeline") c.pipeline = false;
         else if (a == "--zerocopy-threshold") { c.zc = cli_num<uint64_t>(next("zerocopy-threshold")); c.has_zc = true; }
         else if (a == "--span") c.span = next("span");

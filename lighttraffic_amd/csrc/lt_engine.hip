@@ -172,6 +172,8 @@ struct Engine {
     uint64_t zc_threshold = 0;
     const uint64_t *zc_rowptr = nullptr;  // device view of the pinned host graph
     const uint32_t *zc_col = nullptr;
+    const PackedImage *pk = nullptr;      // packed partition images (pack_partitions)
+    uint8_t *d_stage = nullptr;           // HBM landing buffer of one packed image
 
     // walker queues
     uint64_t wcap = 0;  // walkers the pool is sized for
@@ -269,6 +271,8 @@ struct Engine {
     void record(uint32_t p, int kind, int32_t loaded, int32_t evicted);
     lt_status evict(uint32_t p);
     lt_status load_async(uint32_t p);
+    lt_status copy_partition(uint32_t p, uint8_t *base);
+    lt_status fetch_partition(uint32_t p, uint64_t *rowptr_out, uint32_t *col_out);
     lt_status set_table(uint32_t p, bool present);
     lt_status scheduler_loop();
     lt_status scheduler_loop_learned();
@@ -322,6 +326,7 @@ Engine::~Engine() {
     dfree(d_resident);
     dfree(d_zc_part);
     dfree(d_rowptr_all);
+    dfree(d_stage);
     if (s_compute) (void)hipStreamDestroy(s_compute);
     if (s_copy) (void)hipStreamDestroy(s_copy);
 }
@@ -429,6 +434,12 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     else zc_threshold = cfg.policy == LT_SCHED_LEARNED ? part.bytes / 128 : part.bytes / 128 / 2;
     LT_HIP(hipHostGetDevicePointer((void **)&zc_rowptr, g->rowptr, 0));
     LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
+    // Packed copies: the images are built once per graph and partition table.
+    if (cfg.pack_partitions > 0 && !cfg.device_schedule) {
+        LT_TRY(packed_image(*g, part, &pk));
+        if (pk->max_size) LT_TRY(dmalloc(&d_stage, pk->max_size));
+        else pk = nullptr;  // no partition gets smaller
+    }
     if (rp_resident) {
         LT_TRY(dmalloc(&d_rowptr_all, (size_t)V + 1));
         LT_HIP(hipMemcpy(d_rowptr_all, g->rowptr, rp_bytes, hipMemcpyHostToDevice));
@@ -587,22 +598,66 @@ lt_status Engine::load_async(uint32_t p) {
     const uint32_t s = free_slots.back();
     free_slots.pop_back();
     const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
-    uint8_t *base = d_slots + (size_t)s * slot_bytes;
+    hipEvent_t t = t_load.begin(s_copy);
+    LT_TRY(copy_partition(p, d_slots + (size_t)s * slot_bytes));
+    t_load.end(t, s_copy);
+    slot_of[p] = (int32_t)s;
+    part_in_slot[s] = (int32_t)p;
+    return LT_OK;
+}
+
+// Slot bytes of partition p, written on the copy stream: the rowptr slice,
+// then the col slice, either copied as is or copied packed into d_stage and
+// unpacked from there (the next copy into d_stage queues behind the unpack).
+lt_status Engine::copy_partition(uint32_t p, uint8_t *base) {
     const uint32_t nb = part.node_off[p], ne = part.node_off[p + 1];
     const uint64_t eb = part.edge_off[p], ee = part.edge_off[p + 1];
     const size_t rbytes = ((size_t)ne - nb + 1) * 8, cbytes = (size_t)(ee - eb) * 4;
-    hipEvent_t t = t_load.begin(s_copy);
     if (rp_resident) {
         LT_HIP(hipMemcpyAsync(base, d_rowptr_all + nb, rbytes, hipMemcpyDeviceToDevice, s_copy));
     } else {
         LT_HIP(hipMemcpyAsync(base, g->rowptr + nb, rbytes, hipMemcpyHostToDevice, s_copy));
         st.h2d_bytes_explicit += rbytes;
+        st.h2d_bytes_unpacked += rbytes;
     }
-    if (cbytes) LT_HIP(hipMemcpyAsync(base + rbytes, g->col + eb, cbytes, hipMemcpyHostToDevice, s_copy));
-    t_load.end(t, s_copy);
-    st.h2d_bytes_explicit += cbytes;
-    slot_of[p] = (int32_t)s;
-    part_in_slot[s] = (int32_t)p;
+    st.h2d_bytes_unpacked += cbytes;
+    if (pk && pk->size[p]) {
+        const uint64_t n = ne - nb, nch = (ee - eb + 63) / 64;
+        const uint64_t nwords = (pk->size[p] - ((n + 3) & ~3ull) - 4 * nch) / 4 - 2;
+        const uint64_t waves = (nch + kUnpackChunks - 1) / kUnpackChunks;
+        LT_HIP(hipMemcpyAsync(d_stage, pk->host + pk->off[p], pk->size[p], hipMemcpyHostToDevice, s_copy));
+        unpack_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, s_copy>>>(
+            d_stage, (uint32_t)n, ee - eb, nwords, pk->vbits, (const uint64_t *)base, (uint32_t *)(base + rbytes));
+        LT_HIP(hipGetLastError());
+        st.h2d_bytes_explicit += pk->size[p];
+    } else {
+        if (cbytes) LT_HIP(hipMemcpyAsync(base + rbytes, g->col + eb, cbytes, hipMemcpyHostToDevice, s_copy));
+        st.h2d_bytes_explicit += cbytes;
+    }
+    return LT_OK;
+}
+
+// A partition's slot bytes, built by copy_partition in a scratch buffer and
+// copied back; the cache and the statistics are left as they were.
+lt_status Engine::fetch_partition(uint32_t p, uint64_t *rowptr_out, uint32_t *col_out) {
+    LT_HIP(hipSetDevice(device));
+    LT_HIP(hipStreamSynchronize(s_copy));
+    const size_t rn = (size_t)part.node_off[p + 1] - part.node_off[p] + 1;
+    const size_t m = part.edge_off[p + 1] - part.edge_off[p];
+    uint8_t *buf = nullptr;
+    LT_TRY(dmalloc(&buf, rn * 8 + m * 4));
+    const lt_stats keep = st;
+    lt_status rc = copy_partition(p, buf);
+    st = keep;
+    hipError_t e = rc == LT_OK ? hipStreamSynchronize(s_copy) : hipSuccess;
+    if (rc == LT_OK && e == hipSuccess) e = hipMemcpy(rowptr_out, buf, rn * 8, hipMemcpyDeviceToHost);
+    if (rc == LT_OK && e == hipSuccess && m) e = hipMemcpy(col_out, buf + rn * 8, m * 4, hipMemcpyDeviceToHost);
+    dfree(buf);
+    if (rc != LT_OK) return rc;
+    if (e != hipSuccess) {
+        set_error("fetch of partition %u: %s", p, hipGetErrorString(e));
+        return LT_E_HIP;
+    }
     return LT_OK;
 }
 
@@ -1367,6 +1422,7 @@ lt_status Engine::scheduler_loop_device() {
     st.zerocopy_calls += h.zc_calls;
     st.computing_calls += h.iterations;
     st.h2d_bytes_explicit += h.h2d_bytes;
+    st.h2d_bytes_unpacked += h.h2d_bytes;
     st.kernel_launches += (uint64_t)launches * kGraphIters;
     return LT_OK;
 }
@@ -1376,6 +1432,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     memset(&st, 0, sizeof(st));
     spill_h2d_bytes = 0;
     st.h2d_bytes_explicit = pending_h2d;  // the rowptr upload, once
+    st.h2d_bytes_unpacked = pending_h2d;
     pending_h2d = 0;
     trace.clear();
     trace_counts.clear();
@@ -1658,6 +1715,14 @@ lt_status lt_engine_stats(const lt_engine *e, lt_stats *out) {
     if (!e || !out) return LT_E_ARG;
     *out = e->st;
     return LT_OK;
+}
+
+lt_status lt_engine_fetch_partition(lt_engine *e, uint32_t part, uint64_t *rowptr_out, uint32_t *col_out) {
+    if (!e || !rowptr_out || !col_out || part >= e->P) {
+        lt::set_error("lt_engine_fetch_partition: null argument or partition %u out of range", part);
+        return LT_E_ARG;
+    }
+    return e->fetch_partition(part, rowptr_out, col_out);
 }
 
 uint64_t lt_engine_trace_len(const lt_engine *e) { return e ? e->trace.size() : 0; }

@@ -1,6 +1,7 @@
 // lt_internal.h — host-side objects behind the opaque handles of lighttraffic.h.
 #pragma once
 #include <stdint.h>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -9,6 +10,8 @@
 namespace lt {
 
 void set_error(const char *fmt, ...);
+
+struct PackedImage;
 
 // bcsr graph (src/GraphLoader.h:30-48).  rowptr/col live in pinned,
 // device-mapped host memory when a HIP device is present, so that the engine
@@ -20,6 +23,9 @@ struct Graph {
     uint64_t *rowptr = nullptr;
     uint32_t *col = nullptr;
     bool pinned = false;
+    // Packed partition images built for this graph, one per partition table
+    // (built on first use by an engine that copies packed partitions).
+    mutable std::vector<std::unique_ptr<PackedImage>> packs;
     ~Graph();
 };
 
@@ -34,6 +40,43 @@ struct Partition {
     uint32_t P() const { return (uint32_t)node_off.size() - 1; }
     uint32_t find(uint32_t v) const;
 };
+
+// Packed partition image: the column slice of a partition, gap-coded for the
+// host->HBM copy and unpacked in HBM by unpack_kernel.  Layout of one image
+// for a partition of n rows and m edges (partition-local edge index e):
+//   u8  code[n], zero-padded to a multiple of 4 bytes:
+//       0..32 = bit width of the row's gaps, kPackRaw = row stored absolute;
+//   u32 chunk_word[ceil(m/64)]: first 32-bit word of chunk c (edges 64c..64c+63);
+//   u32 words[]: the chunks' bit streams, each starting on a word, LSB first,
+//       followed by two zero words.
+// Edge e is an anchor if it is the first edge of its row, the first edge of
+// its chunk, or its row is kPackRaw; an anchor holds col[e] in vbits bits,
+// any other edge holds col[e] - col[e-1] in its row's width.  So every chunk
+// decodes without the others, and a row is gap-coded only when its
+// non-anchor neighbours are in non-decreasing order.
+constexpr uint32_t kPackChunk = 64;
+constexpr uint8_t kPackRaw = 0xFF;
+
+struct PackedImage {
+    uint64_t bytes = 0;                  // partition budget S the image is for
+    std::vector<uint32_t> node_off;      // partition table it was built for
+    std::vector<uint64_t> edge_off;
+    std::vector<uint64_t> off, size;     // per partition: offset, bytes (0 = copy unpacked)
+    uint8_t *host = nullptr;
+    bool pinned = false;
+    uint64_t total = 0, max_size = 0;
+    uint32_t vbits = 1;
+    ~PackedImage();
+};
+
+// Bits of an anchor: enough for V-1, at least 1.
+uint32_t pack_value_bits(uint32_t V);
+// Bytes of the image of rows [nb, ne), UINT64_MAX if its bit streams would
+// exceed 2^32 words; writes it when out is not null (out must hold that many
+// zeroed bytes).
+uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, uint8_t *out);
+// The graph's image for partition table p, built on first call.
+lt_status packed_image(const Graph &g, const Partition &p, const PackedImage **out);
 
 }  // namespace lt
 

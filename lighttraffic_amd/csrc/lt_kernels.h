@@ -828,4 +828,110 @@ __global__ __launch_bounds__(256) void load_kernel(SchedState ss, Parts pt) {
         dst[i] = i < nr ? src_r[i] : src_c[i - nr];
 }
 
+// Unpacks a packed partition image into the col slice of an HBM slot whose
+// rowptr slice is already in place (it gives the row boundaries; rowptr[0] is
+// the partition's first global edge).  Image layout: code u8[n] padded to 4
+// bytes, chunk_word u32[nch] (nch = ceil(m/64)), bit streams u32[nwords] + 2
+// zero words; an anchor (first edge of its row or of its chunk, or any edge of
+// a row coded 255) holds the column in vbits bits, any other edge the gap to
+// the previous column in its row's width.  The bytes written equal the raw
+// col slice.
+//
+// One wave per kUnpackChunks consecutive 64-edge chunks, one lane per edge:
+//   - the row holding the wave's first edge comes from a 64-ary search of
+//     rowptr (one ballot per level); later chunks carry it forward;
+//   - each lane's row: rows ending inside the chunk are read 64 at a time
+//     (coalesced) and counted with a shuffle binary search;
+//   - each lane's bit offset: exclusive scan of the field widths; the chunk's
+//     words (at most 64) sit one per lane and are fetched by shuffle;
+//   - columns: inclusive scan of the fields minus the scan just before the
+//     lane's anchor (u32 arithmetic; the difference is exact);
+//   - the 64 columns of a chunk are stored by consecutive lanes.
+constexpr uint32_t kUnpackChunks = 8;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *__restrict__ img, uint32_t n, uint64_t m,
+                                                     uint64_t nwords, uint32_t vbits,
+                                                     const uint64_t *__restrict__ rowptr,
+                                                     uint32_t *__restrict__ col) {
+    const uint64_t nch = (m + 63) / 64;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t cb = wave * kUnpackChunks;
+    if (cb >= nch) return;  // whole wave
+    const uint64_t ce = cb + kUnpackChunks < nch ? cb + kUnpackChunks : nch;
+    const uint64_t eb = rowptr[0];
+    const uint8_t *code = img;
+    const uint32_t *cw = (const uint32_t *)(img + (((uint64_t)n + 3) & ~3ull));
+    const uint32_t *wd = cw + nch;
+
+    // row r0 holding edge 64*cb: the last r < n with rowptr[r] - eb <= 64*cb
+    uint64_t e0 = cb * 64;
+    uint32_t lo = 0, hi = n - 1;
+    while (hi > lo) {
+        const uint64_t span = (uint64_t)hi - lo + 1;
+        const uint32_t idx = lo + (uint32_t)(span * lane / 64);
+        const uint64_t mask = __ballot(rowptr[idx] - eb <= e0);
+        const uint32_t k = (uint32_t)__popcll(mask) - 1;  // lane 0 (idx = lo) always holds
+        const uint32_t nlo = __shfl(idx, k, 64);
+        const uint32_t nxt = __shfl(idx, k < 63 ? k + 1 : 63, 64);
+        hi = k < 63 ? nxt - 1 : hi;
+        lo = nlo;
+        if (hi < lo) hi = lo;
+    }
+    uint32_t r0 = lo;
+
+    for (uint64_t c = cb; c < ce; c++) {
+        e0 = c * 64;
+        const uint32_t cnt = (uint32_t)(m - e0 < 64 ? m - e0 : 64);
+        // the chunk's words, one per lane
+        const uint64_t w_beg = cw[c], w_end = c + 1 < nch ? cw[c + 1] : nwords;
+        const uint32_t word = w_beg + lane < w_end ? wd[w_beg + lane] : 0u;
+        // lane's row offset from r0: rows r0+1.. ending at or before the lane's edge
+        uint32_t rowoff = 0, adv = 0;
+        for (uint32_t base = r0 + 1;; base += 64) {
+            const uint32_t idx = base + lane;
+            const uint64_t end = idx <= n ? rowptr[idx] - eb : ~0ull;  // > e0 for every idx > r0
+            const uint32_t p = (uint32_t)(end - e0 < 65 ? end - e0 : 65);  // 1..65
+            const uint32_t p63 = __shfl(p, 63, 64);
+            uint32_t k = 0;
+#pragma unroll
+            for (uint32_t st = 32; st > 0; st >>= 1) {
+                const uint32_t v = __shfl(p, k + st - 1, 64);
+                if (v <= lane) k += st;
+            }
+            if (k == 63 && p63 <= lane) k = 64;
+            rowoff += k;
+            adv += (uint32_t)__popcll(__ballot(p <= 64));
+            if (p63 > 64 || base + 64 > n) break;
+        }
+        const bool valid = lane < cnt;
+        const uint32_t row = r0 + rowoff;
+        const uint32_t cd = valid ? code[row] : 0u;
+        const uint32_t prev_off = __shfl_up(rowoff, 1, 64);
+        const bool anchor = valid && (lane == 0 || rowoff != prev_off || cd == 0xFFu);
+        const uint32_t len = valid ? (anchor ? vbits : cd) : 0u;
+        const uint32_t pos = wave_incl_scan(len, lane) - len;
+        const uint32_t wi = pos >> 5;
+        const uint32_t lo_w = __shfl(word, wi, 64);
+        const uint32_t hi_w = __shfl(word, (wi + 1) & 63u, 64);
+        const uint64_t win = (uint64_t)lo_w | ((uint64_t)(wi + 1 < 64 ? hi_w : 0u) << 32);
+        const uint32_t v = (uint32_t)((win >> (pos & 31u)) & ((1ull << len) - 1));
+        const uint32_t incl = wave_incl_scan(v, lane);
+        const uint64_t heads = __ballot(anchor) & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+        const uint32_t head = 63u - (uint32_t)__clzll(heads);  // lane 0 is always a head
+        const uint32_t before = __shfl(incl, head > 0 ? head - 1 : 0, 64);
+        if (valid) col[e0 + lane] = incl - (head > 0 ? before : 0u);
+        r0 += adv;
+    }
+}
+
 }  // namespace lt

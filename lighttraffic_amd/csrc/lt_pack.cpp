@@ -1,0 +1,187 @@
+// lt_pack.cpp — packed partition images for the host->HBM partition copy.
+//
+// The reference copies a partition as its raw rowptr and col slices
+// (GPUGraph::loadAsync, src/GraphLoader.h:531-552).  Here the col slice may
+// instead travel gap-coded (format in lt_internal.h) and be unpacked in HBM by
+// unpack_kernel into exactly the bytes a raw copy would have written, so the
+// walk reads the same slot either way.  Sorted adjacency lists (what the bcsr
+// converter and the R-MAT generator write) have small gaps between
+// neighbours; a row whose neighbours are not sorted is stored absolute.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <thread>
+
+#include "lt_internal.h"
+
+namespace lt {
+
+static inline uint32_t bitlen(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
+
+uint32_t pack_value_bits(uint32_t V) { return V <= 2 ? 1u : bitlen(V - 1); }
+
+static inline bool is_anchor(uint64_t e, uint64_t row_begin, uint8_t code) {
+    return e == row_begin || e % kPackChunk == 0 || code == kPackRaw;
+}
+
+// Width code of a row of partition-local edges [a, b): the widest gap between
+// non-anchor neighbours, or kPackRaw if one of those gaps is negative.
+static uint8_t row_code(const uint32_t *c, uint64_t a, uint64_t b) {
+    uint32_t w = 0;
+    for (uint64_t e = a + 1; e < b; e++) {
+        if (e % kPackChunk == 0) continue;
+        if (c[e] < c[e - 1]) return kPackRaw;
+        w = std::max(w, bitlen(c[e] - c[e - 1]));
+    }
+    return (uint8_t)w;
+}
+
+uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, uint8_t *out) {
+    const uint64_t n = ne - nb;
+    const uint64_t eb = g.rowptr[nb], m = g.rowptr[ne] - eb;
+    const uint32_t *c = g.col + eb;
+    const uint64_t nch = (m + kPackChunk - 1) / kPackChunk;
+    const uint64_t code_bytes = (n + 3) & ~3ull;
+    std::vector<uint8_t> code(n);
+    std::vector<uint64_t> chunk_bits(nch, 0);
+    for (uint64_t r = 0; r < n; r++) {
+        const uint64_t a = g.rowptr[nb + r] - eb, b = g.rowptr[nb + r + 1] - eb;
+        code[r] = row_code(c, a, b);
+        for (uint64_t e = a; e < b; e++)
+            chunk_bits[e / kPackChunk] += is_anchor(e, a, code[r]) ? vbits : code[r];
+    }
+    uint64_t words = 0;
+    std::vector<uint64_t> chunk_word(nch);
+    for (uint64_t k = 0; k < nch; k++) {
+        chunk_word[k] = words;
+        words += (chunk_bits[k] + 31) / 32;
+    }
+    if (words > UINT32_MAX) return UINT64_MAX;  // chunk_word would not fit 32 bits
+    const uint64_t bytes = code_bytes + 4 * nch + 4 * (words + 2);
+    if (!out) return bytes;
+    memcpy(out, code.data(), n);
+    uint32_t *cw = (uint32_t *)(out + code_bytes);
+    for (uint64_t k = 0; k < nch; k++) cw[k] = (uint32_t)chunk_word[k];
+    uint32_t *wd = cw + nch;
+    uint64_t pos = 0;  // bit position within the current chunk
+    for (uint64_t r = 0; r < n; r++) {
+        const uint64_t a = g.rowptr[nb + r] - eb, b = g.rowptr[nb + r + 1] - eb;
+        for (uint64_t e = a; e < b; e++) {
+            if (e % kPackChunk == 0) pos = 0;
+            const bool anchor = is_anchor(e, a, code[r]);
+            const uint32_t len = anchor ? vbits : code[r];
+            const uint64_t v = anchor ? c[e] : (uint64_t)(c[e] - c[e - 1]);
+            uint32_t *w = wd + chunk_word[e / kPackChunk] + pos / 32;
+            const uint64_t sh = v << (pos % 32);
+            w[0] |= (uint32_t)sh;
+            if (pos % 32 + len > 32) w[1] |= (uint32_t)(sh >> 32);
+            pos += len;
+        }
+    }
+    return bytes;
+}
+
+PackedImage::~PackedImage() {
+    if (!host) return;
+    if (pinned) (void)hipHostFree(host);
+    else free(host);
+}
+
+// Runs f(i) for i in [0, n) on up to 16 threads.
+template <class F>
+static void parallel_for(uint32_t n, F f) {
+    const uint32_t nt = std::max(1u, std::min({n, 16u, std::thread::hardware_concurrency()}));
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < nt; t++)
+        th.emplace_back([&] {
+            for (uint32_t i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto &t : th) t.join();
+}
+
+lt_status packed_image(const Graph &g, const Partition &p, const PackedImage **out) {
+    for (auto &pk : g.packs)
+        if (pk->bytes == p.bytes && pk->node_off == p.node_off && pk->edge_off == p.edge_off) {
+            *out = pk.get();
+            return LT_OK;
+        }
+    auto pk = std::make_unique<PackedImage>();
+    const uint32_t P = p.P();
+    pk->bytes = p.bytes;
+    pk->node_off = p.node_off;
+    pk->edge_off = p.edge_off;
+    pk->vbits = pack_value_bits(g.V);
+    pk->off.assign(P, 0);
+    pk->size.assign(P, 0);
+    parallel_for(P, [&](uint32_t i) {
+        pk->size[i] = pack_encode(g, p.node_off[i], p.node_off[i + 1], pk->vbits, nullptr);
+    });
+    // a partition is copied packed only when that copies fewer bytes
+    for (uint32_t i = 0; i < P; i++) {
+        const uint64_t raw = 4 * (p.edge_off[i + 1] - p.edge_off[i]);
+        if (pk->size[i] >= raw) {
+            pk->size[i] = 0;
+            continue;
+        }
+        pk->off[i] = pk->total;
+        pk->total += (pk->size[i] + 255) & ~255ull;
+        pk->max_size = std::max(pk->max_size, pk->size[i]);
+    }
+    if (pk->total) {
+        if (g.pinned) {
+            if (hipHostMalloc((void **)&pk->host, pk->total, hipHostMallocPortable) != hipSuccess) {
+                (void)hipGetLastError();
+                pk->host = nullptr;
+                set_error("pinned host allocation of %llu bytes for packed partitions failed",
+                          (unsigned long long)pk->total);
+                return LT_E_NOMEM;
+            }
+            pk->pinned = true;
+        } else if (!(pk->host = (uint8_t *)malloc(pk->total))) {
+            set_error("host allocation of %llu bytes for packed partitions failed", (unsigned long long)pk->total);
+            return LT_E_NOMEM;
+        }
+        parallel_for(P, [&](uint32_t i) {
+            if (!pk->size[i]) return;
+            uint8_t *dst = pk->host + pk->off[i];
+            memset(dst, 0, pk->size[i]);
+            pack_encode(g, p.node_off[i], p.node_off[i + 1], pk->vbits, dst);
+        });
+    }
+    *out = pk.get();
+    g.packs.push_back(std::move(pk));
+    return LT_OK;
+}
+
+}  // namespace lt
+
+extern "C" {
+
+lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
+                            uint64_t *bytes_out) {
+    if (!g || !p || !bytes_out || part >= p->P() || p->node_off[part + 1] > g->V) {
+        lt::set_error("lt_partition_pack: null handle, or partition %u not in the graph", part);
+        return LT_E_ARG;
+    }
+    const uint32_t vbits = lt::pack_value_bits(g->V);
+    const uint64_t bytes = lt::pack_encode(*g, p->node_off[part], p->node_off[part + 1], vbits, nullptr);
+    if (bytes == UINT64_MAX) {
+        lt::set_error("lt_partition_pack: partition %u too large to pack", part);
+        return LT_E_ARG;
+    }
+    *bytes_out = bytes;
+    if (!out) return LT_OK;
+    if (cap < bytes) {
+        lt::set_error("lt_partition_pack: %llu bytes needed, %llu given", (unsigned long long)bytes,
+                      (unsigned long long)cap);
+        return LT_E_ARG;
+    }
+    memset(out, 0, bytes);
+    lt::pack_encode(*g, p->node_off[part], p->node_off[part + 1], vbits, (uint8_t *)out);
+    return LT_OK;
+}
+
+}  // extern "C"

@@ -56,6 +56,9 @@ def test_pagerank_cli_capped_cache(graph_file):
                   ["--zerocopy-threshold", 1 << 40]):
         out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, *flags)
         assert "result correct" in out
+    out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, "--pack", "--zerocopy-threshold", 0)
+    assert "result correct" in out
+    assert grep_num(r"H2D bytes: explicit (\d+)", out) < grep_num(r"H2D bytes unpacked: (\d+)", out)
     out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, "--zerocopy-threshold", 1 << 40)
     assert grep_num(r"graph loading time: [0-9.]+ ms, calls: (\d+)", out) == 0
     # -b below -w: walker queues spill to host memory and come back (the

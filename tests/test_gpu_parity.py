@@ -83,6 +83,13 @@ CONFIGS = [
                                      "zc_batch": 16, "rowptr_resident": 1}),
     ("rp-resident-device", 16 << 10, {"num_device_partitions": 6, "device_schedule": 1, "zerocopy_threshold": 3000,
                                       "span": "resident", "rowptr_resident": 1}),
+    # partitions copied packed and unpacked in HBM
+    ("packed-capped", 16 << 10, {"num_device_partitions": 3, "pack_partitions": 1}),
+    ("packed-learned-zc", 16 << 10, {"num_device_partitions": 3, "span": "resident", "policy": "learned",
+                                     "zerocopy_threshold": 3000, "pack_partitions": 1}),
+    ("packed-rp-resident", 16 << 10, {"num_device_partitions": 6, "zerocopy_threshold": 4000, "rowptr_resident": 1,
+                                      "pack_partitions": 1}),
+    ("packed-spill", 16 << 10, {"num_device_partitions": 4, "num_device_walkers": 5000, "pack_partitions": 1}),
 ]
 
 
