@@ -90,7 +90,8 @@ def measure(eng, wl, w, warmup, steps, label, timed_engine=None):
     ms = float(np.mean([s["running_ms"] for s in stats]))
     nominal = stats[0]["nominal_steps"]
     keys = ("moves", "visits", "residencies", "kernel_launches", "h2d_bytes_explicit", "h2d_bytes_zerocopy",
-            "graph_loading_calls", "zerocopy_calls", "iterations", "prefetch_issued", "prefetch_hits")
+            "graph_loading_calls", "zerocopy_calls", "iterations", "prefetch_issued", "prefetch_hits",
+            "h2d_bytes_unpacked", "packed_image_bytes", "packed_col_bytes")
     tot = {k: sum(s[k] for s in stats) for k in keys}
     h2d = (tot["h2d_bytes_explicit"] + tot["h2d_bytes_zerocopy"]) / len(stats)
     res = {"ms_per_step": ms, "steps_per_s": nominal / (ms * 1e-3), "h2d_bytes_per_step": h2d / nominal,

@@ -120,7 +120,10 @@ typedef struct {
     uint64_t prefetch_hits;         /* prefetched partitions picked next         */
     uint64_t h2d_bytes_unpacked;    /* the bytes the explicit copies stand for
                                        (= h2d_bytes_explicit without packing)   */
-    uint64_t reserved[3];
+    uint64_t packed_image_bytes;    /* packed column images copied (part of
+                                       h2d_bytes_explicit)                      */
+    uint64_t packed_col_bytes;      /* column bytes those images unpacked to    */
+    uint64_t reserved[1];
 } lt_stats;
 
 /* ---- errors ------------------------------------------------------------- */

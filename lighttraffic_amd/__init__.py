@@ -66,7 +66,8 @@ class Stats(ctypes.Structure):
                 ("teleports", _u64), ("visits", _u64), ("residencies", _u64), ("zerocopy_moves", _u64),
                 ("nominal_steps", _u64), ("kernel_launches", _u64), ("walk_kernel_ms", ctypes.c_double),
                 ("learned_decisions", _u64), ("fallback_decisions", _u64), ("prefetch_issued", _u64),
-                ("prefetch_hits", _u64), ("h2d_bytes_unpacked", _u64), ("reserved", _u64 * 3)]
+                ("prefetch_hits", _u64), ("h2d_bytes_unpacked", _u64), ("packed_image_bytes", _u64),
+                ("packed_col_bytes", _u64), ("reserved", _u64 * 1)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}

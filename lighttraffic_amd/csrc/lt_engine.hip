@@ -630,6 +630,8 @@ lt_status Engine::copy_partition(uint32_t p, uint8_t *base) {
             d_stage, (uint32_t)n, ee - eb, nwords, pk->vbits, (const uint64_t *)base, (uint32_t *)(base + rbytes));
         LT_HIP(hipGetLastError());
         st.h2d_bytes_explicit += pk->size[p];
+        st.packed_image_bytes += pk->size[p];
+        st.packed_col_bytes += cbytes;
     } else {
         if (cbytes) LT_HIP(hipMemcpyAsync(base + rbytes, g->col + eb, cbytes, hipMemcpyHostToDevice, s_copy));
         st.h2d_bytes_explicit += cbytes;

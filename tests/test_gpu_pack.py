@@ -76,4 +76,7 @@ def test_packed_byte_counters():
     assert sa["h2d_bytes_unpacked"] == sa["h2d_bytes_explicit"]
     assert sb["h2d_bytes_unpacked"] == sa["h2d_bytes_explicit"]
     assert sb["h2d_bytes_explicit"] < 0.75 * sa["h2d_bytes_explicit"]
+    assert sa["packed_image_bytes"] == sa["packed_col_bytes"] == 0
+    assert 0 < sb["packed_image_bytes"] < sb["packed_col_bytes"]
+    assert sb["h2d_bytes_unpacked"] - sb["h2d_bytes_explicit"] == sb["packed_col_bytes"] - sb["packed_image_bytes"]
     log(f"explicit bytes raw {sa['h2d_bytes_explicit']} packed {sb['h2d_bytes_explicit']}")
