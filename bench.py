@@ -321,7 +321,8 @@ def main():
         log(f"autotuner picked {best} in {tune_s:.1f} s")
         part = lt.Partition.build(g, best["partition_mb"] << 20)
         cfg = dict(num_device_partitions=best["slots"], span=best["span"], device_schedule=best["device_schedule"],
-                   slice_chunks=best["slice_chunks"], zc_batch=best["zc_batch"])
+                   slice_chunks=best["slice_chunks"], zc_batch=best["zc_batch"],
+                   pack_partitions=best["pack_partitions"])
         eng = lt.Engine(g, part, **cfg)
         # walk-kernel time for the roofline: the same partitioning, host-scheduled, event timers on
         teng = lt.Engine(g, part, **dict(cfg, device_schedule=0), event_timers=1)
@@ -332,14 +333,15 @@ def main():
                 f"p=0.15, 8 GB cache; autotuned: {best['partition_mb']} MB partitions (P={part.count}), "
                 f"{best['slots']} slots, span {best['span']}, "
                 f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}, "
-                f"zero-copy batch {best['zc_batch']}")
+                f"zero-copy batch {best['zc_batch']}, {'packed' if best['pack_partitions'] else 'raw'} copies")
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, "c4", best["span"]),
                     "stats_per_step": m["per_job"],
                     "autotune": {"seconds": tune_s, "sample": "every 16th query", "best": best,
                                  "trials": [{k: t[k] for k in ("partition_mb", "span", "device_schedule",
-                                                               "slice_chunks", "running_ms", "h2d_bytes")}
+                                                               "slice_chunks", "zc_batch", "pack_partitions",
+                                                               "running_ms", "h2d_bytes")}
                                             for t in trials]}})
     else:
         span = args.span or wl.get("span", "partition")
@@ -347,6 +349,9 @@ def main():
         teng = lt.Engine(g, part, span=span, event_timers=1, **base)
         run_job(teng, wl, w)
         m = measure(eng, wl, w, args.warmup, args.steps, args.workload, timed_engine=teng)
+        # at most one other engine alive while a variant is measured (hardware queues)
+        teng.close()
+        eng.close()
         if args.workload == "c2":  # the reference's span: walks stop at every partition boundary
             e2 = lt.Engine(g, part, span="partition", **base)
             v = measure(e2, wl, w, 1, 2, "c2/span partition")
@@ -363,11 +368,18 @@ def main():
                 e2.close()
             # same plain rules, walks continue through every cached partition and
             # sub-threshold queues are walked over PCIe 16 at a time
+            keep = ("steps_per_s", "ms_per_step", "h2d_bytes_per_step", "h2d_GB_per_job", "pcie_GBps")
             e2 = lt.Engine(g, part, span="resident", zc_batch=16, **base)
             v = measure(e2, wl, w, 1, 2, "c3/resident_zc_batch16")
-            variants["span_resident_zc_batch16"] = {k: v[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
-                                                                      "h2d_GB_per_job", "pcie_GBps")}
+            variants["span_resident_zc_batch16"] = {k: v[k] for k in keep}
             e2.close()
+            # the same two with packed partition copies
+            for name, kw in (("packed", dict(span=span)), ("span_resident_zc_batch16_packed",
+                                                           dict(span="resident", zc_batch=16))):
+                e2 = lt.Engine(g, part, pack_partitions=1, **kw, **base)
+                v = measure(e2, wl, w, 1, 2, f"c3/{name}")
+                variants[name] = {k: v[k] for k in keep}
+                e2.close()
             out["c3_variants"] = variants
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "span": span,

@@ -211,11 +211,12 @@ def test_autotuner_picks_a_trial():
     g = lt.Graph.rmat(14, 8)
     src, cnt = g.ppr_queries(256, lo=100, hi=10000, seed=5)
     best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=48 << 20, sample_every=4, sizes_mb=(16,))
-    # host scheduling: span partition x 2 slices + span resident x 2 slices x 2 batch sizes;
-    # device scheduling: 2 spans (whole queues, no batches)
-    assert len(trials) == 2 + 4 + 2
+    # host scheduling: (span partition x 2 slices + span resident x 2 slices x 2 batch
+    # sizes) x packed or raw copies; device scheduling: 2 spans (whole queues, raw copies)
+    assert len(trials) == 2 * (2 + 4) + 2
     assert best in trials and best["running_ms"] == min(t["running_ms"] for t in trials)
     assert {t["zc_batch"] for t in trials} == {1, 16}
+    assert {t["pack_partitions"] for t in trials} == {0, 1}
 
 
 def test_schedule_independence(g12):
