@@ -171,18 +171,19 @@ void lt_partition_free(lt_partition *p);
 /* Packed image of partition part's column slice, the form in which an engine
  * with pack_partitions = 1 copies it to HBM (replaces the raw col copy of
  * GPUGraph::loadAsync, src/GraphLoader.h:531-552; the rowptr slice is copied
- * as is).  For n rows and m edges, with vbits = bits of V-1 (at least 1):
- *   u8  code[n], zero-padded to a multiple of 4 bytes: 0..32 = bit width of
- *       the row's gaps, 255 = row stored absolute;
- *   u32 chunk_word[ceil(m/64)]: first word of chunk c (partition-local edges
- *       64c .. 64c+63) in words[];
+ * as is).  For m edges in nch = ceil(m/64) chunks of 64 (partition-local edges
+ * 64c .. 64c+63), each of four 16-edge blocks, with vbits = bits of V-1 (at
+ * least 1):
+ *   u32 meta[2*nch]: meta[2c] = first word of chunk c in words[]; byte j of
+ *       meta[2c+1] = code of block j of chunk c: 0..32 = bit width of the
+ *       block's gaps, 255 = block stored absolute;
  *   u32 words[]: each chunk's bit stream (LSB first, from its first word),
  *       then two zero words.
  * An edge is an anchor when it is its row's first, its chunk's first, or its
- * row's code is 255; an anchor holds its column in vbits bits, any other edge
- * holds the gap to the previous column in its row's width.  *bytes_out gets
- * the image size; the image is written when out is not NULL and cap is large
- * enough. */
+ * block's code is 255; an anchor holds its column in vbits bits, any other
+ * edge holds the gap to the previous column in its block's width.  *bytes_out
+ * gets the image size; the image is written when out is not NULL and cap is
+ * large enough. */
 lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
                             uint64_t *bytes_out);
 

@@ -623,7 +623,7 @@ lt_status Engine::copy_partition(uint32_t p, uint8_t *base) {
     st.h2d_bytes_unpacked += cbytes;
     if (pk && pk->size[p]) {
         const uint64_t n = ne - nb, nch = (ee - eb + 63) / 64;
-        const uint64_t nwords = (pk->size[p] - ((n + 3) & ~3ull) - 4 * nch) / 4 - 2;
+        const uint64_t nwords = (pk->size[p] - 8 * nch) / 4 - 2;
         const uint64_t waves = (nch + kUnpackChunks - 1) / kUnpackChunks;
         LT_HIP(hipMemcpyAsync(d_stage, pk->host + pk->off[p], pk->size[p], hipMemcpyHostToDevice, s_copy));
         unpack_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, s_copy>>>(

@@ -43,18 +43,21 @@ struct Partition {
 
 // Packed partition image: the column slice of a partition, gap-coded for the
 // host->HBM copy and unpacked in HBM by unpack_kernel.  Layout of one image
-// for a partition of n rows and m edges (partition-local edge index e):
-//   u8  code[n], zero-padded to a multiple of 4 bytes:
-//       0..32 = bit width of the row's gaps, kPackRaw = row stored absolute;
-//   u32 chunk_word[ceil(m/64)]: first 32-bit word of chunk c (edges 64c..64c+63);
+// for a partition of m edges (partition-local edge index e), nch = ceil(m/64)
+// chunks of 64 edges, each of 4 blocks of 16 edges:
+//   u32 meta[2*nch]: meta[2c] = first 32-bit word of chunk c in words[];
+//       meta[2c+1] byte j = code of block j of chunk c: 0..32 = bit width of
+//       its gaps, kPackRaw = block stored absolute;
 //   u32 words[]: the chunks' bit streams, each starting on a word, LSB first,
 //       followed by two zero words.
-// Edge e is an anchor if it is the first edge of its row, the first edge of
-// its chunk, or its row is kPackRaw; an anchor holds col[e] in vbits bits,
-// any other edge holds col[e] - col[e-1] in its row's width.  So every chunk
-// decodes without the others, and a row is gap-coded only when its
-// non-anchor neighbours are in non-decreasing order.
+// Edge e is an anchor if it is the first edge of its row or of its chunk, or
+// its block is kPackRaw; an anchor holds col[e] in vbits bits, any other edge
+// holds col[e] - col[e-1] in its block's width.  So every chunk decodes
+// without the others (the row starts come from the slot's rowptr), and a
+// block is gap-coded only when its non-anchor edges are in non-decreasing
+// order within their rows.
 constexpr uint32_t kPackChunk = 64;
+constexpr uint32_t kPackBlock = 16;
 constexpr uint8_t kPackRaw = 0xFF;
 
 struct PackedImage {

@@ -830,18 +830,18 @@ __global__ __launch_bounds__(256) void load_kernel(SchedState ss, Parts pt) {
 
 // Unpacks a packed partition image into the col slice of an HBM slot whose
 // rowptr slice is already in place (it gives the row boundaries; rowptr[0] is
-// the partition's first global edge).  Image layout: code u8[n] padded to 4
-// bytes, chunk_word u32[nch] (nch = ceil(m/64)), bit streams u32[nwords] + 2
-// zero words; an anchor (first edge of its row or of its chunk, or any edge of
-// a row coded 255) holds the column in vbits bits, any other edge the gap to
-// the previous column in its row's width.  The bytes written equal the raw
-// col slice.
+// the partition's first global edge).  Image layout: meta u32[2*nch] (nch =
+// ceil(m/64); per chunk its first word and the codes of its four 16-edge
+// blocks), bit streams u32[nwords] + 2 zero words; an anchor (first edge of
+// its row or of its chunk, or any edge of a block coded 255) holds the column
+// in vbits bits, any other edge the gap to the previous column in its block's
+// width.  The bytes written equal the raw col slice.
 //
 // One wave per kUnpackChunks consecutive 64-edge chunks, one lane per edge:
 //   - the row holding the wave's first edge comes from a 64-ary search of
 //     rowptr (one ballot per level); later chunks carry it forward;
-//   - each lane's row: rows ending inside the chunk are read 64 at a time
-//     (coalesced) and counted with a shuffle binary search;
+//   - whether a lane's edge starts a row: rows ending inside the chunk are
+//     read 64 at a time (coalesced) and counted with a shuffle binary search;
 //   - each lane's bit offset: exclusive scan of the field widths; the chunk's
 //     words (at most 64) sit one per lane and are fetched by shuffle;
 //   - columns: inclusive scan of the fields minus the scan just before the
@@ -869,9 +869,8 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *__restrict__
     if (cb >= nch) return;  // whole wave
     const uint64_t ce = cb + kUnpackChunks < nch ? cb + kUnpackChunks : nch;
     const uint64_t eb = rowptr[0];
-    const uint8_t *code = img;
-    const uint32_t *cw = (const uint32_t *)(img + (((uint64_t)n + 3) & ~3ull));
-    const uint32_t *wd = cw + nch;
+    const uint32_t *meta = (const uint32_t *)img;
+    const uint32_t *wd = meta + 2 * nch;
 
     // row r0 holding edge 64*cb: the last r < n with rowptr[r] - eb <= 64*cb
     uint64_t e0 = cb * 64;
@@ -892,8 +891,9 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *__restrict__
     for (uint64_t c = cb; c < ce; c++) {
         e0 = c * 64;
         const uint32_t cnt = (uint32_t)(m - e0 < 64 ? m - e0 : 64);
-        // the chunk's words, one per lane
-        const uint64_t w_beg = cw[c], w_end = c + 1 < nch ? cw[c + 1] : nwords;
+        // the chunk's words, one per lane, and its block codes
+        const uint64_t w_beg = meta[2 * c], w_end = c + 1 < nch ? meta[2 * c + 2] : nwords;
+        const uint32_t bw = (meta[2 * c + 1] >> ((lane >> 4) * 8)) & 0xFFu;
         const uint32_t word = w_beg + lane < w_end ? wd[w_beg + lane] : 0u;
         // lane's row offset from r0: rows r0+1.. ending at or before the lane's edge
         uint32_t rowoff = 0, adv = 0;
@@ -914,11 +914,9 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *__restrict__
             if (p63 > 64 || base + 64 > n) break;
         }
         const bool valid = lane < cnt;
-        const uint32_t row = r0 + rowoff;
-        const uint32_t cd = valid ? code[row] : 0u;
         const uint32_t prev_off = __shfl_up(rowoff, 1, 64);
-        const bool anchor = valid && (lane == 0 || rowoff != prev_off || cd == 0xFFu);
-        const uint32_t len = valid ? (anchor ? vbits : cd) : 0u;
+        const bool anchor = valid && (lane == 0 || rowoff != prev_off || bw == 0xFFu);
+        const uint32_t len = valid ? (anchor ? vbits : bw) : 0u;
         const uint32_t pos = wave_incl_scan(len, lane) - len;
         const uint32_t wi = pos >> 5;
         const uint32_t lo_w = __shfl(word, wi, 64);

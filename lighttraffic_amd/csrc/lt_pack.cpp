@@ -6,7 +6,7 @@
 // unpack_kernel into exactly the bytes a raw copy would have written, so the
 // walk reads the same slot either way.  Sorted adjacency lists (what the bcsr
 // converter and the R-MAT generator write) have small gaps between
-// neighbours; a row whose neighbours are not sorted is stored absolute.
+// neighbours; a block holding a descending pair is stored absolute.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,20 +22,10 @@ static inline uint32_t bitlen(uint32_t x) { return x ? 32u - (uint32_t)__builtin
 
 uint32_t pack_value_bits(uint32_t V) { return V <= 2 ? 1u : bitlen(V - 1); }
 
-static inline bool is_anchor(uint64_t e, uint64_t row_begin, uint8_t code) {
-    return e == row_begin || e % kPackChunk == 0 || code == kPackRaw;
-}
-
-// Width code of a row of partition-local edges [a, b): the widest gap between
-// non-anchor neighbours, or kPackRaw if one of those gaps is negative.
-static uint8_t row_code(const uint32_t *c, uint64_t a, uint64_t b) {
-    uint32_t w = 0;
-    for (uint64_t e = a + 1; e < b; e++) {
-        if (e % kPackChunk == 0) continue;
-        if (c[e] < c[e - 1]) return kPackRaw;
-        w = std::max(w, bitlen(c[e] - c[e - 1]));
-    }
-    return (uint8_t)w;
+// Edge e (partition-local) of a row starting at row_begin is an anchor if it
+// starts its row or its chunk, or its block is stored absolute.
+static inline bool is_anchor(uint64_t e, uint64_t row_begin, uint8_t block_code) {
+    return e == row_begin || e % kPackChunk == 0 || block_code == kPackRaw;
 }
 
 uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, uint8_t *out) {
@@ -43,14 +33,27 @@ uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, u
     const uint64_t eb = g.rowptr[nb], m = g.rowptr[ne] - eb;
     const uint32_t *c = g.col + eb;
     const uint64_t nch = (m + kPackChunk - 1) / kPackChunk;
-    const uint64_t code_bytes = (n + 3) & ~3ull;
-    std::vector<uint8_t> code(n);
+    const uint64_t nblk = nch * (kPackChunk / kPackBlock);
+    // block widths: the widest gap between non-anchor neighbours of the block,
+    // kPackRaw if one of those gaps is negative
+    std::vector<uint8_t> code(nblk, 0);
+    for (uint64_t r = 0; r < n; r++) {
+        const uint64_t a = g.rowptr[nb + r] - eb, b = g.rowptr[nb + r + 1] - eb;
+        for (uint64_t e = a + 1; e < b; e++) {
+            if (e % kPackChunk == 0) continue;
+            uint8_t &k = code[e / kPackBlock];
+            if (k == kPackRaw) continue;
+            if (c[e] < c[e - 1]) k = kPackRaw;
+            else k = (uint8_t)std::max<uint32_t>(k, bitlen(c[e] - c[e - 1]));
+        }
+    }
     std::vector<uint64_t> chunk_bits(nch, 0);
     for (uint64_t r = 0; r < n; r++) {
         const uint64_t a = g.rowptr[nb + r] - eb, b = g.rowptr[nb + r + 1] - eb;
-        code[r] = row_code(c, a, b);
-        for (uint64_t e = a; e < b; e++)
-            chunk_bits[e / kPackChunk] += is_anchor(e, a, code[r]) ? vbits : code[r];
+        for (uint64_t e = a; e < b; e++) {
+            const uint8_t k = code[e / kPackBlock];
+            chunk_bits[e / kPackChunk] += is_anchor(e, a, k) ? vbits : k;
+        }
     }
     uint64_t words = 0;
     std::vector<uint64_t> chunk_word(nch);
@@ -59,19 +62,25 @@ uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, u
         words += (chunk_bits[k] + 31) / 32;
     }
     if (words > UINT32_MAX) return UINT64_MAX;  // chunk_word would not fit 32 bits
-    const uint64_t bytes = code_bytes + 4 * nch + 4 * (words + 2);
+    const uint64_t bytes = 8 * nch + 4 * (words + 2);
     if (!out) return bytes;
-    memcpy(out, code.data(), n);
-    uint32_t *cw = (uint32_t *)(out + code_bytes);
-    for (uint64_t k = 0; k < nch; k++) cw[k] = (uint32_t)chunk_word[k];
-    uint32_t *wd = cw + nch;
+    uint32_t *meta = (uint32_t *)out;
+    for (uint64_t k = 0; k < nch; k++) {
+        meta[2 * k] = (uint32_t)chunk_word[k];
+        uint32_t wc = 0;
+        for (uint32_t j = 0; j < kPackChunk / kPackBlock; j++)
+            wc |= (uint32_t)code[k * (kPackChunk / kPackBlock) + j] << (8 * j);
+        meta[2 * k + 1] = wc;
+    }
+    uint32_t *wd = meta + 2 * nch;
     uint64_t pos = 0;  // bit position within the current chunk
     for (uint64_t r = 0; r < n; r++) {
         const uint64_t a = g.rowptr[nb + r] - eb, b = g.rowptr[nb + r + 1] - eb;
         for (uint64_t e = a; e < b; e++) {
             if (e % kPackChunk == 0) pos = 0;
-            const bool anchor = is_anchor(e, a, code[r]);
-            const uint32_t len = anchor ? vbits : code[r];
+            const uint8_t k = code[e / kPackBlock];
+            const bool anchor = is_anchor(e, a, k);
+            const uint32_t len = anchor ? vbits : k;
             const uint64_t v = anchor ? c[e] : (uint64_t)(c[e] - c[e - 1]);
             uint32_t *w = wd + chunk_word[e / kPackChunk] + pos / 32;
             const uint64_t sh = v << (pos % 32);

@@ -21,27 +21,26 @@ from oracle import ffi as oracle
 
 
 def decode(img, rowptr, eb, V):
-    """Column slice from a packed image, following the header's text: code
-    u8[n] padded to 4 bytes, chunk_word u32[ceil(m/64)], then the chunks' bit
-    streams (LSB first).  rowptr: the partition's n+1 global edge offsets."""
+    """Column slice from a packed image, following the header's text: meta
+    u32[2*ceil(m/64)] (first word of each chunk, codes of its four 16-edge
+    blocks), then the chunks' bit streams (LSB first).  rowptr: the
+    partition's n+1 global edge offsets."""
     n = len(rowptr) - 1
     m = int(rowptr[-1] - rowptr[0])
     vbits = max(1, int(V - 1).bit_length())
-    code = img[:n]
-    off = (n + 3) // 4 * 4
     nch = (m + 63) // 64
-    cw = img[off:off + 4 * nch].view(np.uint32)
-    words = img[off + 4 * nch:].view(np.uint32)
+    meta = img[:8 * nch].view(np.uint32)
+    words = img[8 * nch:].view(np.uint32)
     bits = np.unpackbits(words.view(np.uint8), bitorder="little")
-    row_of = np.repeat(np.arange(n), np.diff(rowptr).astype(np.int64))
+    row_start = np.zeros(m + 1, bool)
+    row_start[(rowptr[:-1] - eb).astype(np.int64)] = True
     out = np.zeros(m, np.uint32)
     for c in range(nch):
-        pos = int(cw[c]) * 32
+        pos = int(meta[2 * c]) * 32
         for e in range(64 * c, min(64 * c + 64, m)):
-            r = row_of[e]
-            first = e == int(rowptr[r] - eb)
-            anchor = first or e == 64 * c or code[r] == 255
-            ln = vbits if anchor else int(code[r])
+            code = (int(meta[2 * c + 1]) >> (8 * ((e - 64 * c) // 16))) & 0xFF
+            anchor = row_start[e] or e == 64 * c or code == 255
+            ln = vbits if anchor else code
             v = 0
             for k in range(ln):
                 v |= int(bits[pos + k]) << k
@@ -65,7 +64,7 @@ def check_graph(rowptr, col, part_bytes):
 
 
 def test_hand_computed_image():
-    # one row [5, 6, 8] in a 16-vertex graph: vbits = 4, gaps 1 and 2 -> width 2;
+    # one row [5, 6, 8] in a 16-vertex graph: vbits = 4; block 0's gaps 1 and 2 -> width 2;
     # bits: 5 (4 bits) | 1 (2 bits) | 2 (2 bits) = 0b10_01_0101 = 149
     V = 16
     rowptr = np.array([0, 3] + [3] * (V - 1), np.uint64)
@@ -74,15 +73,13 @@ def test_hand_computed_image():
     part = lt.Partition.build(g, 1 << 20)
     assert part.count == 1
     img = part.pack(g, 0)
-    # code u8[16] (row 0: width 2), one chunk word (0), words: [149, 0, 0]
-    expect = np.zeros(16 + 4 + 12, np.uint8)
-    expect[0] = 2
-    expect[20:24] = np.array([149], np.uint32).view(np.uint8)
+    # meta: chunk 0 starts at word 0, block codes (2, 0, 0, 0); words: [149, 0, 0]
+    expect = np.array([0, 2, 149, 0, 0], np.uint32).view(np.uint8)
     assert np.array_equal(img, expect)
 
 
-def test_hand_computed_unsorted_and_anchor():
-    # row 0 = [3, 1] (unsorted -> code 255, both absolute); row 1 = 65 edges
+def test_hand_computed_raw_block_and_anchors():
+    # row 0 = [3, 1] (descending -> block 0 stored absolute); row 1 = 65 edges
     # 0..64 in order: edge 64 of the partition starts chunk 1 and is an anchor
     V = 70
     rowptr = np.array([0, 2, 67] + [67] * (V - 2), np.uint64)
@@ -90,12 +87,14 @@ def test_hand_computed_unsorted_and_anchor():
     g = lt.Graph.from_csr(rowptr, col)
     part = lt.Partition.build(g, 1 << 20)
     img = part.pack(g, 0)
-    assert img[0] == 255 and img[1] == 1  # row 1's gaps are all 1: width 1
-    n, vbits = V, 7
-    cw = img[72:80].view(np.uint32)
-    # chunk 0: 3 anchors (3, 1, first of row 1) of 7 bits + 61 gaps of 1 bit = 82 bits -> 3 words
-    assert list(cw) == [0, 3]
-    assert np.array_equal(decode(img, rowptr[:n + 1], 0, V), col)
+    meta = img[:16].view(np.uint32)
+    vbits = 7
+    # chunk 0: block 0 = edges 0..15, all absolute (16 x 7 bits); blocks 1-3 gaps of 1
+    # -> 112 + 48 = 160 bits = 5 words; chunk 1: anchor 64 (7 bits) + gaps 1, 1 -> 1 word
+    assert list(meta) == [0, 255 | (1 << 8) | (1 << 16) | (1 << 24), 5, 1]
+    assert len(img) == 16 + 4 * (5 + 1 + 2)
+    assert np.array_equal(decode(img, rowptr[:V + 1], 0, V), col)
+    assert vbits == max(1, (V - 1).bit_length())
 
 
 @pytest.mark.parametrize("scale,ef,part_bytes", [(8, 4, 2048), (10, 8, 8192), (11, 16, 1 << 20)])
