@@ -201,6 +201,11 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    # Before the HIP runtime starts: at least eight hardware queues, so that a
+    # packed copy's DMA stream gets a queue of its own (the engine checks this
+    # value; HIP's default is 4).
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
     import lighttraffic_amd as lt
 
     wl = WORKLOADS[args.workload]

@@ -24,6 +24,11 @@ static int fail(lt_status s, const char *what) {
 }
 
 int main(int argc, char **argv) {
+    // Before the HIP runtime starts: at least eight hardware queues (HIP's
+    // default is 4), so that a packed copy's DMA stream does not share a queue
+    // with the walk stream.  A larger value the caller set is kept.
+    const char *hwq = getenv("GPU_MAX_HW_QUEUES");
+    if (!hwq || atoi(hwq) < 8) setenv("GPU_MAX_HW_QUEUES", "8", 1);
     const std::string app = LT_APP_NAME;
     CliConfig c;
     if (!cli_parse(argc, argv, c)) {
@@ -31,7 +36,7 @@ int main(int argc, char **argv) {
                 "usage: %s -f <graph.bcsr> [-m [-s MB]] [-p slots] [-b walkers] [-w walkers] [-l length]\n"
                 "          [--runs n] [--source v] [--prob p] [--scheduler greedy|fifo|learned]\n"
                 "          [--no-pipeline] [--zerocopy-threshold n] [--span partition|resident]\n"
-                "          [--seed n] [--timers] [--json] [--queries file]\n",
+                "          [--seed n] [--timers] [--json] [--queries file] [--pack] [--rowptr-resident]\n",
                 app.c_str());
         return 1;
     }

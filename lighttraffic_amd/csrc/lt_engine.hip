@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -172,8 +173,17 @@ struct Engine {
     uint64_t zc_threshold = 0;
     const uint64_t *zc_rowptr = nullptr;  // device view of the pinned host graph
     const uint32_t *zc_col = nullptr;
+    // Packed copies: images land in two HBM staging buffers by DMA on s_dma;
+    // s_copy waits for the DMA, then unpacks into the slot, so the next DMA
+    // overlaps this unpack.  Loads are complete when s_copy reaches them.
+    // With too few hardware queues s_dma is s_copy (DMA and unpack in turn).
     const PackedImage *pk = nullptr;      // packed partition images (pack_partitions)
-    uint8_t *d_stage = nullptr;           // HBM landing buffer of one packed image
+    uint8_t *d_stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_free[2] = {nullptr, nullptr};  // unpack of the buffer's last image done
+    bool stage_used[2] = {false, false};
+    uint32_t stage_next = 0;
+    hipStream_t s_dma = nullptr;          // == s_copy without packed copies
+    hipEvent_t ev_dma = nullptr;          // last DMA queued on s_dma
 
     // walker queues
     uint64_t wcap = 0;  // walkers the pool is sized for
@@ -296,6 +306,7 @@ struct Engine {
 Engine::~Engine() {
     if (s_compute) (void)hipStreamSynchronize(s_compute);
     if (s_copy) (void)hipStreamSynchronize(s_copy);
+    if (s_dma && s_dma != s_copy) (void)hipStreamSynchronize(s_dma);
     for (auto ev : pool_events) (void)hipEventDestroy(ev);
     if (ev_evicted) (void)hipEventDestroy(ev_evicted);
     dfree(d_node_off);
@@ -326,9 +337,14 @@ Engine::~Engine() {
     dfree(d_resident);
     dfree(d_zc_part);
     dfree(d_rowptr_all);
-    dfree(d_stage);
+    dfree(d_stage[0]);
+    dfree(d_stage[1]);
     if (s_compute) (void)hipStreamDestroy(s_compute);
     if (s_copy) (void)hipStreamDestroy(s_copy);
+    if (s_dma && s_dma != s_copy) (void)hipStreamDestroy(s_dma);
+    for (int k = 0; k < 2; k++)
+        if (stage_free[k]) (void)hipEventDestroy(stage_free[k]);
+    if (ev_dma) (void)hipEventDestroy(ev_dma);
 }
 
 lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_config *c) {
@@ -435,10 +451,26 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     LT_HIP(hipHostGetDevicePointer((void **)&zc_rowptr, g->rowptr, 0));
     LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
     // Packed copies: the images are built once per graph and partition table.
+    s_dma = s_copy;
     if (cfg.pack_partitions > 0 && !cfg.device_schedule) {
         LT_TRY(packed_image(*g, part, &pk));
-        if (pk->max_size) LT_TRY(dmalloc(&d_stage, pk->max_size));
-        else pk = nullptr;  // no partition gets smaller
+        if (pk->max_size) {
+            for (int k = 0; k < 2; k++) {
+                LT_TRY(dmalloc(&d_stage[k], pk->max_size));
+                LT_HIP(hipEventCreateWithFlags(&stage_free[k], hipEventDisableTiming));
+            }
+            // HIP spreads streams over GPU_MAX_HW_QUEUES hardware queues (4
+            // unless set).  A DMA stream that shares a queue with the walk
+            // stream waits behind walk kernels, so it is used only when the
+            // streams in play (null, walk, copy, DMA, a scorer's) get their own.
+            const char *hwq = getenv("GPU_MAX_HW_QUEUES");
+            if (hwq && atoi(hwq) >= 6) {
+                LT_HIP(hipEventCreateWithFlags(&ev_dma, hipEventDisableTiming));
+                LT_HIP(hipStreamCreateWithFlags(&s_dma, hipStreamNonBlocking));
+            }
+        } else {
+            pk = nullptr;  // no partition gets smaller
+        }
     }
     if (rp_resident) {
         LT_TRY(dmalloc(&d_rowptr_all, (size_t)V + 1));
@@ -598,7 +630,7 @@ lt_status Engine::load_async(uint32_t p) {
     const uint32_t s = free_slots.back();
     free_slots.pop_back();
     const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
-    hipEvent_t t = t_load.begin(s_copy);
+    hipEvent_t t = t_load.begin(s_dma);
     LT_TRY(copy_partition(p, d_slots + (size_t)s * slot_bytes));
     t_load.end(t, s_copy);
     slot_of[p] = (int32_t)s;
@@ -606,35 +638,54 @@ lt_status Engine::load_async(uint32_t p) {
     return LT_OK;
 }
 
-// Slot bytes of partition p, written on the copy stream: the rowptr slice,
-// then the col slice, either copied as is or copied packed into d_stage and
-// unpacked from there (the next copy into d_stage queues behind the unpack).
+// Slot bytes of partition p: the rowptr slice, then the col slice, either
+// copied as is or copied packed into a staging buffer and unpacked from there.
+// The DMAs run on s_dma; s_copy waits for them (and unpacks), so the load is
+// complete when s_copy reaches this point.  A staging buffer is refilled only
+// after the unpack of its previous image.
 lt_status Engine::copy_partition(uint32_t p, uint8_t *base) {
     const uint32_t nb = part.node_off[p], ne = part.node_off[p + 1];
     const uint64_t eb = part.edge_off[p], ee = part.edge_off[p + 1];
     const size_t rbytes = ((size_t)ne - nb + 1) * 8, cbytes = (size_t)(ee - eb) * 4;
+    const bool packed = pk && pk->size[p];
+    uint32_t k = 0;
+    if (packed) {
+        k = stage_next;
+        stage_next ^= 1u;
+        if (stage_used[k] && s_dma != s_copy) LT_HIP(hipStreamWaitEvent(s_dma, stage_free[k], 0));
+    }
     if (rp_resident) {
-        LT_HIP(hipMemcpyAsync(base, d_rowptr_all + nb, rbytes, hipMemcpyDeviceToDevice, s_copy));
+        LT_HIP(hipMemcpyAsync(base, d_rowptr_all + nb, rbytes, hipMemcpyDeviceToDevice, s_dma));
     } else {
-        LT_HIP(hipMemcpyAsync(base, g->rowptr + nb, rbytes, hipMemcpyHostToDevice, s_copy));
+        LT_HIP(hipMemcpyAsync(base, g->rowptr + nb, rbytes, hipMemcpyHostToDevice, s_dma));
         st.h2d_bytes_explicit += rbytes;
         st.h2d_bytes_unpacked += rbytes;
     }
     st.h2d_bytes_unpacked += cbytes;
-    if (pk && pk->size[p]) {
+    if (packed) {
         const uint64_t n = ne - nb, nch = (ee - eb + 63) / 64;
         const uint64_t nwords = (pk->size[p] - 8 * nch) / 4 - 2;
         const uint64_t waves = (nch + kUnpackChunks - 1) / kUnpackChunks;
-        LT_HIP(hipMemcpyAsync(d_stage, pk->host + pk->off[p], pk->size[p], hipMemcpyHostToDevice, s_copy));
+        LT_HIP(hipMemcpyAsync(d_stage[k], pk->host + pk->off[p], pk->size[p], hipMemcpyHostToDevice, s_dma));
+        if (s_dma != s_copy) {
+            LT_HIP(hipEventRecord(ev_dma, s_dma));
+            LT_HIP(hipStreamWaitEvent(s_copy, ev_dma, 0));
+        }
         unpack_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, s_copy>>>(
-            d_stage, (uint32_t)n, ee - eb, nwords, pk->vbits, (const uint64_t *)base, (uint32_t *)(base + rbytes));
+            d_stage[k], (uint32_t)n, ee - eb, nwords, pk->vbits, (const uint64_t *)base, (uint32_t *)(base + rbytes));
         LT_HIP(hipGetLastError());
+        LT_HIP(hipEventRecord(stage_free[k], s_copy));
+        stage_used[k] = true;
         st.h2d_bytes_explicit += pk->size[p];
         st.packed_image_bytes += pk->size[p];
         st.packed_col_bytes += cbytes;
     } else {
-        if (cbytes) LT_HIP(hipMemcpyAsync(base + rbytes, g->col + eb, cbytes, hipMemcpyHostToDevice, s_copy));
+        if (cbytes) LT_HIP(hipMemcpyAsync(base + rbytes, g->col + eb, cbytes, hipMemcpyHostToDevice, s_dma));
         st.h2d_bytes_explicit += cbytes;
+        if (s_dma != s_copy) {
+            LT_HIP(hipEventRecord(ev_dma, s_dma));
+            LT_HIP(hipStreamWaitEvent(s_copy, ev_dma, 0));
+        }
     }
     return LT_OK;
 }
@@ -644,6 +695,7 @@ lt_status Engine::copy_partition(uint32_t p, uint8_t *base) {
 lt_status Engine::fetch_partition(uint32_t p, uint64_t *rowptr_out, uint32_t *col_out) {
     LT_HIP(hipSetDevice(device));
     LT_HIP(hipStreamSynchronize(s_copy));
+    if (s_dma != s_copy) LT_HIP(hipStreamSynchronize(s_dma));
     const size_t rn = (size_t)part.node_off[p + 1] - part.node_off[p] + 1;
     const size_t m = part.edge_off[p + 1] - part.edge_off[p];
     uint8_t *buf = nullptr;
@@ -1282,6 +1334,7 @@ lt_status Engine::scheduler_loop_learned() {
                 if (!ev_evicted) LT_HIP(hipEventCreateWithFlags(&ev_evicted, hipEventDisableTiming));
                 LT_HIP(hipEventRecord(ev_evicted, s_compute));
                 LT_HIP(hipStreamWaitEvent(s_copy, ev_evicted, 0));
+                if (s_dma != s_copy) LT_HIP(hipStreamWaitEvent(s_dma, ev_evicted, 0));
             }
         }
         st.computing_calls++;

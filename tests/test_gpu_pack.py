@@ -80,3 +80,48 @@ def test_packed_byte_counters():
     assert 0 < sb["packed_image_bytes"] < sb["packed_col_bytes"]
     assert sb["h2d_bytes_unpacked"] - sb["h2d_bytes_explicit"] == sb["packed_col_bytes"] - sb["packed_image_bytes"]
     log(f"explicit bytes raw {sa['h2d_bytes_explicit']} packed {sb['h2d_bytes_explicit']}")
+
+
+CHILD = r"""
+import sys
+import numpy as np
+import lighttraffic_amd as lt
+from oracle import ffi as oracle
+g = lt.Graph.rmat(13, 8)
+part = lt.Partition.build(g, 16 << 10)
+no, eo = part.node_offsets(), part.edge_offsets()
+w = 2 * g.num_node
+for kw in (dict(num_device_partitions=3, zerocopy_threshold=0),
+           dict(num_device_partitions=4, span="resident", policy="learned", zerocopy_threshold=50)):
+    eng = lt.Engine(g, part, pack_partitions=1, **kw)
+    counts = eng.pagerank(w, 10)
+    rc, _, _ = oracle.pagerank(g.rowptr(), g.col(), w, 10)
+    assert np.array_equal(counts, rc), kw
+    st = eng.stats()
+    assert st["packed_image_bytes"] > 0 and st["graph_loading_calls"] > part.count, st
+    ends, moves = eng.genericwalk(w, 10, moves=True)
+    re, rm, _ = oracle.genericwalk(g.rowptr(), g.col(), w, 10)
+    assert np.array_equal(ends, re) and np.array_equal(moves, rm), kw
+    for p in range(part.count):
+        rp, c = eng.fetch_partition(p, no, eo)
+        assert np.array_equal(c, g.col()[eo[p]:eo[p + 1]]), p
+    eng.close()
+print("child ok", part.count)
+"""
+
+
+def test_packed_copies_on_their_own_dma_stream():
+    """With GPU_MAX_HW_QUEUES >= 6 (read when the HIP runtime starts, so in a
+    child process) the packed copies' DMAs run on a stream of their own and
+    two staging buffers alternate; results stay bit-exact."""
+    import os
+    import subprocess
+    import sys
+    log("packed copies with a DMA stream (child process, 8 hardware queues)")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, GPU_MAX_HW_QUEUES="8", PYTHONPATH=root)
+    r = subprocess.run([sys.executable, "-c", CHILD], env=env, cwd=root, capture_output=True, text=True,
+                       timeout=150)
+    log(r.stdout.strip())
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "child ok" in r.stdout
