@@ -199,6 +199,7 @@ __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, co
 // Per-thread event counts of one launch (far below 2^32), summed as u64.
 struct Counters {
     uint32_t moves = 0, tele = 0, visits = 0, resid = 0, zc_moves = 0, zc_reads = 0;
+    uint32_t *vkeys = nullptr, *vcnt = nullptr;  // workgroup visit table in LDS (walk kernels)
 };
 
 // Workgroup sum of the counters, added to slot blockIdx.x % kStatSlots, so
@@ -234,15 +235,51 @@ struct Span {
 
 __device__ __forceinline__ uint64_t walker_id(const Walker &w) { return ((uint64_t)w.id_hi << 32) | w.id_lo; }
 
-__device__ __forceinline__ void count_visit(const AppArgs &a, uint32_t v) {
+// Visits go to a workgroup table in LDS (open addressing, 4 probes) and reach
+// the global counters once per vertex and pass (flush_visits); a vertex that
+// finds no slot is counted in global memory directly.
+constexpr uint32_t kVisitSlots = 2048;
+constexpr uint32_t kVisitEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void count_visit(const AppArgs &a, uint32_t v, Counters &ct) {
+    if (ct.vkeys) {
+        const uint32_t h = (v * 2654435761u) >> 21;  // 11 bits: kVisitSlots
+#pragma unroll
+        for (uint32_t probe = 0; probe < 4; probe++) {
+            const uint32_t i = (h + probe) & (kVisitSlots - 1);
+            uint32_t k = ct.vkeys[i];
+            if (k == kVisitEmpty) k = atomicCAS(&ct.vkeys[i], kVisitEmpty, v);
+            if (k == kVisitEmpty || k == v) {
+                atomicAdd(&ct.vcnt[i], 1u);
+                return;
+            }
+        }
+    }
     atomicAdd(&a.counts[(size_t)(blockIdx.x % kCountCopies) * a.counts_stride + v], 1u);
+}
+
+// Adds the workgroup's visit table to the global counters and empties it.
+// Collective over the workgroup.
+__device__ __forceinline__ void flush_visits(const AppArgs &a, const Counters &ct) {
+    if (!ct.vkeys) return;
+    __syncthreads();
+    uint32_t *g = a.counts + (size_t)(blockIdx.x % kCountCopies) * a.counts_stride;
+    for (uint32_t i = threadIdx.x; i < kVisitSlots; i += blockDim.x) {
+        const uint32_t k = ct.vkeys[i];
+        if (k != kVisitEmpty) {
+            atomicAdd(&g[k], ct.vcnt[i]);
+            ct.vkeys[i] = kVisitEmpty;
+            ct.vcnt[i] = 0;
+        }
+    }
+    __syncthreads();
 }
 
 // One update of app/pagerank.cu:26-44: count the visit, then teleport with
 // probability 0.15 or move to a uniform neighbour.  A vertex without
 // neighbours teleports (the reference divides by zero there).
 __device__ __forceinline__ void step_pagerank(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
-    count_visit(a, w.cur);
+    count_visit(a, w.cur, ct);
     ct.visits++;
     w.step -= 1;
     if (w.step == 0) return;
@@ -270,7 +307,7 @@ __device__ __forceinline__ void step_pagerank(Walker &w, const Span &sp, const A
 // left; a vertex without neighbours is counted and ends the walk.
 __device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
     if (w.step == 1) {
-        if (a.counts) count_visit(a, w.cur);
+        if (a.counts) count_visit(a, w.cur, ct);
         ct.visits++;
     }
     w.step -= 1;
@@ -283,7 +320,7 @@ __device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArg
         ct.moves++;
         if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
     } else {
-        if (a.counts) count_visit(a, w.cur);
+        if (a.counts) count_visit(a, w.cur, ct);
         ct.visits++;
         w.step = 0;
         if (sp.zc) ct.zc_reads += a.rp_host;
@@ -377,6 +414,7 @@ __device__ __forceinline__ void walk_walkers(const WalkArgs &a, const AppendLds 
         if ((uint32_t)k * kBlock + threadIdx.x < nvalid) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
     }
     block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum);
+    flush_visits(a.app, ct);
 }
 
 // One workgroup pass: walkers [base, base + kBlock * WPT) of src's queue,
@@ -403,6 +441,18 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
     __shared__ uint32_t s_wsum[kBlock / 64];
     const AppendLds L = carve_lds(smem, a.pt.P, WPT);
     Counters ct;
+    if constexpr (APP != APP_GENERIC) {
+        __shared__ uint32_t s_vkeys[kVisitSlots], s_vcnt[kVisitSlots];
+        if (a.app.counts) {  // uniform
+            for (uint32_t i = threadIdx.x; i < kVisitSlots; i += blockDim.x) {
+                s_vkeys[i] = kVisitEmpty;
+                s_vcnt[i] = 0;
+            }
+            __syncthreads();
+            ct.vkeys = s_vkeys;
+            ct.vcnt = s_vcnt;
+        }
+    }
     if (a.nsrc) {  // zero-copy batch: every source is read over PCIe
         const uint64_t vend = a.voff[a.nsrc];
         for (uint64_t base = (uint64_t)blockIdx.x * BW; base < vend; base += (uint64_t)gridDim.x * BW) {

@@ -60,6 +60,32 @@ def test_packed_slots_unsorted_duplicates_empty_rows():
     slots_equal_raw(g, part, pack_partitions=1, num_device_partitions=3)
 
 
+def test_packed_copies_with_edgeless_partitions():
+    """Partitions holding only vertices without neighbours are copied raw (a
+    rowptr slice, no columns) beside packed ones; slots and walks are exact."""
+    from oracle import ffi as oracle
+    log("packed copies with edgeless partitions")
+    rng = np.random.default_rng(3)
+    V = 9000
+    deg = rng.integers(1, 20, V)
+    deg[2000:7000] = 0
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    col = np.concatenate([np.sort(rng.integers(0, V, d)) for d in deg]).astype(np.uint32)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, 8192)
+    eo = part.edge_offsets()
+    assert sum(eo[p + 1] == eo[p] for p in range(part.count)) >= 2
+    slots_equal_raw(g, part, pack_partitions=1, num_device_partitions=3)
+    eng = lt.Engine(g, part, pack_partitions=1, num_device_partitions=3, zerocopy_threshold=0)
+    w = 2 * V
+    counts = eng.pagerank(w, 10)
+    rc, _, _ = oracle.pagerank(rowptr, col, w, 10)
+    assert np.array_equal(counts, rc)
+    ends, moves = eng.genericwalk(w, 10, moves=True)
+    re, rm, _ = oracle.genericwalk(rowptr, col, w, 10)
+    assert np.array_equal(ends, re) and np.array_equal(moves, rm)
+
+
 def test_packed_byte_counters():
     log("packed byte counters")
     g = lt.Graph.rmat(14, 16)

@@ -143,3 +143,22 @@ def test_pack_arguments():
     assert L.lt_partition_pack(g._h, part._h, 0, None, 0, ctypes.byref(n)) == 0 and n.value > 0
     buf = np.zeros(n.value - 1, np.uint8)
     assert L.lt_partition_pack(g._h, part._h, 0, buf.ctypes.data, n.value - 1, ctypes.byref(n)) == 6  # LT_E_ARG
+
+
+def test_partitions_without_edges():
+    # 5000 vertices without neighbours between two R-MAT-like blocks: with 8 KB
+    # partitions several partitions hold no edge (their image is two zero words)
+    rng = np.random.default_rng(3)
+    V = 9000
+    deg = rng.integers(1, 20, V)
+    deg[2000:7000] = 0
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    col = np.concatenate([np.sort(rng.integers(0, V, d)) for d in deg]).astype(np.uint32)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, 8192)
+    eo = part.edge_offsets()
+    empty = [p for p in range(part.count) if eo[p + 1] == eo[p]]
+    assert len(empty) >= 2
+    for p in empty:
+        assert np.array_equal(part.pack(g, p), np.zeros(8, np.uint8))
+    check_graph(rowptr, col, 8192)
