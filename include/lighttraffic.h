@@ -186,6 +186,12 @@ void lt_partition_free(lt_partition *p);
  * large enough. */
 lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
                             uint64_t *bytes_out);
+/* An engine with pack_partitions = 1 builds the packed images of its
+ * partition table once (pinned host memory, about half the graph's column
+ * bytes) and caches them on the graph for later engines.  This drops the
+ * cache; engines already using an image keep it until they are destroyed.
+ * No counterpart in the reference. */
+void lt_graph_drop_packed(lt_graph *g);
 
 /* ---- engine (src/LightTraffic.h:19-44, src/Device.h:22-59) -------------- */
 lt_status lt_engine_create(const lt_graph *g, const lt_partition *p, const lt_config *cfg,

@@ -108,6 +108,7 @@ _sig("lt_partition_edge_offsets", _vp, _vp)
 _sig("lt_partition_find", _u32, _vp, _u32)
 _sig("lt_partition_free", None, _vp)
 _sig("lt_partition_pack", ctypes.c_int, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64))
+_sig("lt_graph_drop_packed", None, _vp)
 _sig("lt_engine_fetch_partition", ctypes.c_int, _vp, _u32, _vp, _vp)
 _sig("lt_engine_create", ctypes.c_int, _vp, _vp, ctypes.POINTER(Config), ctypes.POINTER(_vp))
 _sig("lt_engine_destroy", None, _vp)
@@ -218,6 +219,11 @@ class Graph:
 
     def save(self, path):
         _check(_lib.lt_graph_save(self._h, path.encode()), f"lt_graph_save({path})")
+
+    def drop_packed(self):
+        """Free the packed partition images cached on this graph (engines using
+        one keep it until they are closed)."""
+        _lib.lt_graph_drop_packed(self._h)
 
     def ppr_queries(self, nq, lo=1000, hi=1000000, seed=0x51525953):
         """Batched-PPR query table: sources proportional to degree, log-uniform counts."""

@@ -177,7 +177,8 @@ struct Engine {
     // s_copy waits for the DMA, then unpacks into the slot, so the next DMA
     // overlaps this unpack.  Loads are complete when s_copy reaches them.
     // With too few hardware queues s_dma is s_copy (DMA and unpack in turn).
-    const PackedImage *pk = nullptr;      // packed partition images (pack_partitions)
+    std::shared_ptr<const PackedImage> pk_hold;  // packed partition images (pack_partitions)
+    const PackedImage *pk = nullptr;
     uint8_t *d_stage[2] = {nullptr, nullptr};
     hipEvent_t stage_free[2] = {nullptr, nullptr};  // unpack of the buffer's last image done
     bool stage_used[2] = {false, false};
@@ -453,7 +454,8 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     // Packed copies: the images are built once per graph and partition table.
     s_dma = s_copy;
     if (cfg.pack_partitions > 0 && !cfg.device_schedule) {
-        LT_TRY(packed_image(*g, part, &pk));
+        LT_TRY(packed_image(*g, part, &pk_hold));
+        pk = pk_hold.get();
         if (pk->max_size) {
             for (int k = 0; k < 2; k++) {
                 LT_TRY(dmalloc(&d_stage[k], pk->max_size));
@@ -470,6 +472,7 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
             }
         } else {
             pk = nullptr;  // no partition gets smaller
+            pk_hold.reset();
         }
     }
     if (rp_resident) {

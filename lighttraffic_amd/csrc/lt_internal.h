@@ -24,8 +24,9 @@ struct Graph {
     uint32_t *col = nullptr;
     bool pinned = false;
     // Packed partition images built for this graph, one per partition table
-    // (built on first use by an engine that copies packed partitions).
-    mutable std::vector<std::unique_ptr<PackedImage>> packs;
+    // (built on first use by an engine that copies packed partitions; an
+    // engine shares ownership of the image it uses).
+    mutable std::vector<std::shared_ptr<const PackedImage>> packs;
     ~Graph();
 };
 
@@ -79,7 +80,7 @@ uint32_t pack_value_bits(uint32_t V);
 // zeroed bytes).
 uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, uint8_t *out);
 // The graph's image for partition table p, built on first call.
-lt_status packed_image(const Graph &g, const Partition &p, const PackedImage **out);
+lt_status packed_image(const Graph &g, const Partition &p, std::shared_ptr<const PackedImage> *out);
 
 }  // namespace lt
 

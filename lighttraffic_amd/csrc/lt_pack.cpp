@@ -111,13 +111,13 @@ static void parallel_for(uint32_t n, F f) {
     for (auto &t : th) t.join();
 }
 
-lt_status packed_image(const Graph &g, const Partition &p, const PackedImage **out) {
+lt_status packed_image(const Graph &g, const Partition &p, std::shared_ptr<const PackedImage> *out) {
     for (auto &pk : g.packs)
         if (pk->bytes == p.bytes && pk->node_off == p.node_off && pk->edge_off == p.edge_off) {
-            *out = pk.get();
+            *out = pk;
             return LT_OK;
         }
-    auto pk = std::make_unique<PackedImage>();
+    auto pk = std::make_shared<PackedImage>();
     const uint32_t P = p.P();
     pk->bytes = p.bytes;
     pk->node_off = p.node_off;
@@ -160,7 +160,7 @@ lt_status packed_image(const Graph &g, const Partition &p, const PackedImage **o
             pack_encode(g, p.node_off[i], p.node_off[i + 1], pk->vbits, dst);
         });
     }
-    *out = pk.get();
+    *out = pk;
     g.packs.push_back(std::move(pk));
     return LT_OK;
 }
@@ -168,6 +168,10 @@ lt_status packed_image(const Graph &g, const Partition &p, const PackedImage **o
 }  // namespace lt
 
 extern "C" {
+
+void lt_graph_drop_packed(lt_graph *g) {
+    if (g) g->packs.clear();
+}
 
 lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
                             uint64_t *bytes_out) {

@@ -51,6 +51,7 @@ def tune_ppr_batch(graph, sources, counts, prob=0.15, cache_bytes=8 << 30, sampl
             trials.append(rec)
             if log:
                 log(f"tune {rec}")
+        graph.drop_packed()  # the next partition size needs images of its own
         part.close()
     best = min(trials, key=lambda r: r["running_ms"])
     return best, trials

@@ -86,6 +86,25 @@ def test_packed_copies_with_edgeless_partitions():
     assert np.array_equal(ends, re) and np.array_equal(moves, rm)
 
 
+def test_drop_packed_images_keeps_running_engines():
+    log("drop packed images")
+    g = lt.Graph.rmat(12, 8)
+    part = lt.Partition.build(g, 16 << 10)
+    w = 2 * g.num_node
+    a = lt.Engine(g, part, pack_partitions=1, num_device_partitions=3, zerocopy_threshold=0, pipeline=0)
+    ref = a.pagerank(w, 10)
+    first = a.stats()["packed_image_bytes"]  # a cold cache: every partition is copied at least once
+    assert first > 0
+    g.drop_packed()  # a keeps its image
+    assert np.array_equal(a.pagerank(w, 10), ref) and a.stats()["packed_image_bytes"] > 0
+    b = lt.Engine(g, part, pack_partitions=1, num_device_partitions=3, zerocopy_threshold=0,
+                  pipeline=0)  # builds a new image
+    assert np.array_equal(b.pagerank(w, 10), ref)
+    assert b.stats()["packed_image_bytes"] == first  # same decisions, same images
+    a.close()
+    b.close()
+
+
 def test_packed_byte_counters():
     log("packed byte counters")
     g = lt.Graph.rmat(14, 16)
