@@ -4,7 +4,11 @@ A step is one complete walk job over one synthetic R-MAT graph: every walker
 created, walked to its end under the partition scheduler, results reduced.
 Timed region per step = the scheduler loop ("Running Time" of
 src/Scheduler.h:425-441): graph generation, walker creation and the copy of
-results back to the host are outside it, as in the reference.
+results back to the host are outside it, as in the reference.  Each measured
+configuration also reports the median of its steps and the end-to-end wall
+time of a job (walker creation and result copy included).  The pinned
+host-to-HBM copy rate is measured once and reported beside the link's
+theoretical rate.
 
 Workloads (R-MAT a,b,c,d = .57/.19/.19/.05, seeded, deduped, symmetrised,
 self loops and isolated vertices removed):
@@ -81,9 +85,11 @@ def measure(eng, wl, w, warmup, steps, label, timed_engine=None):
     for i in range(warmup):
         st = run_job(eng, wl, w)
         log(f"{label} warmup {i}: {st['running_ms']:.1f} ms")
-    stats = []
+    stats, walls = [], []
     for i in range(steps):
+        t0 = time.perf_counter()
         stats.append(run_job(eng, wl, w))
+        walls.append((time.perf_counter() - t0) * 1e3)
         s = stats[-1]
         log(f"{label} step {i}: {s['running_ms']:.1f} ms, loads {s['graph_loading_calls']}, "
             f"H2D {(s['h2d_bytes_explicit'] + s['h2d_bytes_zerocopy']) / 1e9:.2f} GB")
@@ -95,6 +101,8 @@ def measure(eng, wl, w, warmup, steps, label, timed_engine=None):
     tot = {k: sum(s[k] for s in stats) for k in keys}
     h2d = (tot["h2d_bytes_explicit"] + tot["h2d_bytes_zerocopy"]) / len(stats)
     res = {"ms_per_step": ms, "steps_per_s": nominal / (ms * 1e-3), "h2d_bytes_per_step": h2d / nominal,
+           "median_ms_per_step": float(np.median([s["running_ms"] for s in stats])),
+           "e2e_ms_per_step": float(np.mean(walls)),
            "h2d_GB_per_job": h2d / 1e9, "pcie_GBps": h2d / (ms * 1e-3) / 1e9,
            "per_job": {k: v / len(stats) for k, v in tot.items()}, "nominal": nominal}
     if timed_engine is not None:
@@ -185,7 +193,7 @@ def _cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--span", default=None, choices=["partition", "resident"],
@@ -213,6 +221,8 @@ def main():
     if args.copy_calibration:
         gbps = lt.device_copy_bandwidth(args.copy_calibration << 20, reps=1)
         log(f"copy probe {args.copy_calibration} MiB: {gbps:.0f} GB/s (read + write)")
+    h2d_peak = lt.device_h2d_bandwidth(256 << 20, reps=4)
+    log(f"pinned H2D copy: {h2d_peak:.1f} GB/s")
     t0 = time.perf_counter()
     g = lt.Graph.rmat(wl["scale"], wl["ef"])
     log(f"graph s{wl['scale']} ef{wl['ef']}: V={g.num_node} E={g.num_edge} in {time.perf_counter() - t0:.1f} s")
@@ -229,7 +239,9 @@ def main():
             ("all partitions" if not wl["slots"] else f"{wl['slots']} slots = {wl['slots'] * wl['mb'] // 1024} GB"))
     out = {"metric": METRIC, "unit": "walk steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
-           "data": "synthetic R-MAT (a,b,c,d=.57/.19/.19/.05), generated on the GPU, Philox walks"}
+           "data": "synthetic R-MAT (a,b,c,d=.57/.19/.19/.05), generated on the GPU, Philox walks",
+           "pcie": {"h2d_measured_peak_GBps": h2d_peak, "h2d_theoretical_GBps": 64.0,
+                    "note": "pinned hipMemcpyAsync of 256 MiB, 4 times; PCIe 5.0 x16 theoretical rate"}}
 
     if args.workload == "c5":
         # One engine at a time (plus the scorer's stream): HIP maps streams onto
@@ -238,7 +250,8 @@ def main():
         sc = lt.Scorer(part, seed=1)
         plain = {}
         prep_s = 0.0
-        keep = ("steps_per_s", "ms_per_step", "h2d_bytes_per_step", "h2d_GB_per_job", "pcie_GBps")
+        keep = ("steps_per_s", "ms_per_step", "median_ms_per_step", "e2e_ms_per_step", "h2d_bytes_per_step",
+                "h2d_GB_per_job", "pcie_GBps")
         for span in ("partition", "resident"):
             eng = lt.Engine(g, part, span=span, record_trace=1, **base)
             plain[span] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}")
@@ -290,6 +303,8 @@ def main():
         out.update({
             "value": learned["steps_per_s"],
             "ms_per_step": learned["ms_per_step"],
+            "median_ms_per_step": learned["median_ms_per_step"],
+            "e2e_ms_per_step": learned["e2e_ms_per_step"],
             "config": {"workload": desc, "scheduler": "learned (prefetch depth 2, span resident, zero-copy "
                                                       "threshold S/128, zero-copy batches of 16), partitions "
                                                       "copied packed (gap-coded columns, unpacked in HBM)",
@@ -340,6 +355,7 @@ def main():
                 f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}, "
                 f"zero-copy batch {best['zc_batch']}, {'packed' if best['pack_partitions'] else 'raw'} copies")
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
+                    "median_ms_per_step": m["median_ms_per_step"], "e2e_ms_per_step": m["e2e_ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, "c4", best["span"]),
                     "stats_per_step": m["per_job"],
@@ -373,7 +389,8 @@ def main():
                 e2.close()
             # same plain rules, walks continue through every cached partition and
             # sub-threshold queues are walked over PCIe 16 at a time
-            keep = ("steps_per_s", "ms_per_step", "h2d_bytes_per_step", "h2d_GB_per_job", "pcie_GBps")
+            keep = ("steps_per_s", "ms_per_step", "median_ms_per_step", "e2e_ms_per_step", "h2d_bytes_per_step",
+                "h2d_GB_per_job", "pcie_GBps")
             e2 = lt.Engine(g, part, span="resident", zc_batch=16, **base)
             v = measure(e2, wl, w, 1, 2, "c3/resident_zc_batch16")
             variants["span_resident_zc_batch16"] = {k: v[k] for k in keep}
@@ -387,6 +404,7 @@ def main():
                 e2.close()
             out["c3_variants"] = variants
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
+                    "median_ms_per_step": m["median_ms_per_step"], "e2e_ms_per_step": m["e2e_ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "span": span,
                                "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, args.workload, span),

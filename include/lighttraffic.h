@@ -323,6 +323,11 @@ lt_status lt_device_info(char *name, size_t cap, int *num_cu, uint64_t *hbm_byte
  * kernel copy_probe_kernel, reps times; gbps_out = bytes read + written per
  * second.  A kernel of known traffic, used to scale HBM traffic counters. */
 lt_status lt_device_copy_bandwidth(uint64_t bytes, uint32_t reps, double *gbps_out);
+/* Host-to-HBM DMA of `bytes` from pinned host memory (hipMemcpyAsync), reps
+ * times; gbps_out = bytes per second.  The achievable rate of the copies that
+ * load partitions (GPUGraph::loadAsync, src/GraphLoader.h:531-552), to set
+ * beside the link's theoretical rate. */
+lt_status lt_device_h2d_bandwidth(uint64_t bytes, uint32_t reps, double *gbps_out);
 
 #ifdef __cplusplus
 }

@@ -142,6 +142,7 @@ _sig("lt_sched_computed_partition", _i32, _u32, _vp, _vp, _i32, _u64)
 _sig("lt_device_info", ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
      ctypes.POINTER(_u64))
 _sig("lt_device_copy_bandwidth", ctypes.c_int, _u64, _u32, ctypes.POINTER(ctypes.c_double))
+_sig("lt_device_h2d_bandwidth", ctypes.c_int, _u64, _u32, ctypes.POINTER(ctypes.c_double))
 
 
 def lib():
@@ -184,6 +185,13 @@ def device_copy_bandwidth(nbytes, reps=5):
     """HBM copy bandwidth (read + written bytes per second) of a plain copy kernel."""
     gbps = ctypes.c_double()
     _check(_lib.lt_device_copy_bandwidth(nbytes, reps, ctypes.byref(gbps)), "lt_device_copy_bandwidth")
+    return gbps.value
+
+
+def device_h2d_bandwidth(nbytes, reps=5):
+    """Pinned host -> HBM DMA rate in GB/s (hipMemcpyAsync of nbytes, reps times)."""
+    gbps = ctypes.c_double()
+    _check(_lib.lt_device_h2d_bandwidth(nbytes, reps, ctypes.byref(gbps)), "lt_device_h2d_bandwidth")
     return gbps.value
 
 

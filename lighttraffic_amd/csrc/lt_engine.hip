@@ -1736,6 +1736,40 @@ lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, c
     return e->run(rs, nullptr, endpoints_out, nullptr);
 }
 
+lt_status lt_device_h2d_bandwidth(uint64_t bytes, uint32_t reps, double *gbps_out) {
+    if (bytes == 0 || reps == 0) return LT_E_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device visible");
+        return LT_E_NODEVICE;
+    }
+    uint8_t *h = nullptr, *d = nullptr;
+    LT_HIP(hipHostMalloc((void **)&h, bytes, hipHostMallocPortable));
+    std::unique_ptr<uint8_t, void (*)(uint8_t *)> gh(h, [](uint8_t *x) { (void)hipHostFree(x); });
+    memset(h, 1, bytes);
+    LT_TRY(dmalloc(&d, bytes));
+    std::unique_ptr<uint8_t, void (*)(uint8_t *)> gd(d, [](uint8_t *x) { (void)hipFree(x); });
+    hipStream_t s;
+    LT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<ihipStream_t, void (*)(hipStream_t)> gs(s, [](hipStream_t x) { (void)hipStreamDestroy(x); });
+    LT_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));  // warm
+    LT_HIP(hipStreamSynchronize(s));
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    LT_HIP(hipEventRecord(e0, s));
+    for (uint32_t r = 0; r < reps; r++) LT_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+    LT_HIP(hipEventRecord(e1, s));
+    LT_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (gbps_out) *gbps_out = (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return LT_OK;
+}
+
 lt_status lt_device_copy_bandwidth(uint64_t bytes, uint32_t reps, double *gbps_out) {
     if (bytes < 16 || reps == 0) return LT_E_ARG;
     int n = 0;

@@ -170,3 +170,14 @@ def test_packed_copies_on_their_own_dma_stream():
     log(r.stdout.strip())
     assert r.returncode == 0, r.stderr[-2000:]
     assert "child ok" in r.stdout
+
+
+def test_bandwidth_probes():
+    """The two rates bench.py reports beside its measurements: pinned host ->
+    HBM DMA (bounded by the PCIe link) and an HBM copy kernel."""
+    log("bandwidth probes")
+    h2d = lt.device_h2d_bandwidth(64 << 20, reps=2)
+    hbm = lt.device_copy_bandwidth(256 << 20, reps=2)
+    log(f"h2d {h2d:.1f} GB/s, hbm copy {hbm:.0f} GB/s")
+    assert 5 < h2d < 130
+    assert 500 < hbm < 16000
