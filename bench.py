@@ -18,7 +18,9 @@ self loops and isolated vertices removed):
       pagerank w=2|V| l=10
   c2  scale 22 ef 16, 32 MB partitions all HBM-resident, genericwalk w=2|V| l=10,
       walks continue through every partition (the reference's span is reported beside)
-  c3  scale 25 ef 44, 4 GB partition cache, 2^26 walkers at once, ppr from vertex 0
+  c3  scale 25 ef 44, 4 GB partition cache, 2^26 walkers at once, ppr from vertex 0;
+      value: span resident, zero-copy batches of 16, packed copies (the
+      reference's rules and its figure-14 ablations are reported beside)
   c4  same graph, 4,096 PPR queries (sources proportional to degree, walk counts
       log-uniform in [1e3, 1e6]), 8 GB cache; configuration chosen by the
       autotuner (partition size 16/32/64 MB, span, host or device scheduling)
@@ -45,7 +47,8 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10, zc=0),
     "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10, span="resident"),
-    "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26),
+    "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26,
+               eng=dict(span="resident", zc_batch=16, pack_partitions=1)),
     "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
     "c5": dict(scale=26, ef=27, mb=32, app="pagerank", slots=256, l=10),
 }
@@ -365,9 +368,11 @@ def main():
                                                                "running_ms", "h2d_bytes")}
                                             for t in trials]}})
     else:
-        span = args.span or wl.get("span", "partition")
-        eng = lt.Engine(g, part, span=span, **base)
-        teng = lt.Engine(g, part, span=span, event_timers=1, **base)
+        ekw = dict(wl.get("eng", {}))
+        ekw["span"] = args.span or ekw.get("span", wl.get("span", "partition"))
+        span = ekw["span"]
+        eng = lt.Engine(g, part, **ekw, **base)
+        teng = lt.Engine(g, part, event_timers=1, **ekw, **base)
         run_job(teng, wl, w)
         m = measure(eng, wl, w, args.warmup, args.steps, args.workload, timed_engine=teng)
         # at most one other engine alive while a variant is measured (hardware queues)
@@ -379,34 +384,24 @@ def main():
             out["c2_variants"] = {"span_partition": {k: v[k] for k in ("steps_per_s", "ms_per_step")}}
             e2.close()
         if args.workload == "c3":
-            # explicit copies vs zero-copy reads (the reference's figure-14 ablation)
-            variants = {}
-            for name, thr in (("explicit_only", 0), ("zerocopy_only", 1 << 62)):
-                e2 = lt.Engine(g, part, span=span, zerocopy_threshold=thr, **base)
-                v = measure(e2, wl, w, 1, 1, f"c3/{name}")
-                variants[name] = {k: v[k] for k in ("steps_per_s", "ms_per_step", "h2d_bytes_per_step",
-                                                    "h2d_GB_per_job", "pcie_GBps")}
-                e2.close()
-            # same plain rules, walks continue through every cached partition and
-            # sub-threshold queues are walked over PCIe 16 at a time
             keep = ("steps_per_s", "ms_per_step", "median_ms_per_step", "e2e_ms_per_step", "h2d_bytes_per_step",
-                "h2d_GB_per_job", "pcie_GBps")
-            e2 = lt.Engine(g, part, span="resident", zc_batch=16, **base)
-            v = measure(e2, wl, w, 1, 2, "c3/resident_zc_batch16")
-            variants["span_resident_zc_batch16"] = {k: v[k] for k in keep}
-            e2.close()
-            # the same two with packed partition copies
-            for name, kw in (("packed", dict(span=span)), ("span_resident_zc_batch16_packed",
-                                                           dict(span="resident", zc_batch=16))):
-                e2 = lt.Engine(g, part, pack_partitions=1, **kw, **base)
-                v = measure(e2, wl, w, 1, 2, f"c3/{name}")
-                variants[name] = {k: v[k] for k in keep}
+                    "h2d_GB_per_job", "pcie_GBps")
+            variants = {}
+            # the reference's rules: span partition, raw copies, default zero-copy
+            # threshold; then its figure-14 ablation (explicit copies only, zero-copy only)
+            for name, kw, k in (("reference_semantics", {}, args.steps), ("explicit_only", dict(zerocopy_threshold=0), 1),
+                                ("zerocopy_only", dict(zerocopy_threshold=1 << 62), 1),
+                                ("span_resident_zc_batch16", dict(span="resident", zc_batch=16), 2),
+                                ("packed", dict(pack_partitions=1), 2)):
+                e2 = lt.Engine(g, part, **kw, **base)
+                v = measure(e2, wl, w, 1, k, f"c3/{name}")
+                variants[name] = {x: v[x] for x in keep}
                 e2.close()
             out["c3_variants"] = variants
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "median_ms_per_step": m["median_ms_per_step"], "e2e_ms_per_step": m["e2e_ms_per_step"],
-                    "config": {"workload": desc, "scheduler": "greedy", "span": span,
-                               "device": info["name"]},
+                    "config": {"workload": desc, "scheduler": "greedy",
+                               **{k: v for k, v in ekw.items()}, "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, args.workload, span),
                     "stats_per_step": m["per_job"]})
     if not args.no_cpu_baseline:
