@@ -318,7 +318,8 @@ def test_c5_full_size_sampled_walkers():
     """The C5 workload at full size: R-MAT s26 ef27, 429 partitions of 32 MB,
     an 8 GB cache (256 slots), pagerank w = 2|V|, l = 10.  The reference
     scheduler's run matches the oracle on sampled walker ids (endpoints) and
-    the visit sum; the learned scheduler's run gives the same visit counts."""
+    the visit sum; the learned scheduler's run, with raw and with packed
+    partition copies, gives the same visit counts (and endpoints)."""
     log("C5 full size")
     g = lt.Graph.rmat(26, 27)
     part = lt.Partition.build(g, 32 << 20)
@@ -340,6 +341,14 @@ def test_c5_full_size_sampled_walkers():
     learned.set_scorer(sc)
     assert np.array_equal(learned.pagerank(w, l), counts)
     assert learned.stats()["prefetch_issued"] > 0
+    learned.close()
+    # the bench's headline configuration: packed partition copies
+    packed = lt.Engine(g, part, num_device_partitions=256, span="resident", policy="learned", pack_partitions=1)
+    packed.set_scorer(sc)
+    pc, pe = packed.pagerank(w, l, endpoints=True)
+    assert np.array_equal(pc, counts) and np.array_equal(pe, ends)
+    st = packed.stats()
+    assert st["packed_image_bytes"] > 0 and st["packed_col_bytes"] > 1.5 * st["packed_image_bytes"]
 
 
 def test_spill_moves_walkers_through_host_memory(g12):
