@@ -891,7 +891,8 @@ __global__ __launch_bounds__(256) void load_kernel(SchedState ss, Parts pt) {
 //   - the row holding the wave's first edge comes from a 64-ary search of
 //     rowptr (one ballot per level); later chunks carry it forward;
 //   - whether a lane's edge starts a row: rows ending inside the chunk are
-//     read 64 at a time (coalesced) and counted with a shuffle binary search;
+//     read 64 at a time (coalesced) and mark their end in the wave's LDS
+//     flags;
 //   - each lane's bit offset: exclusive scan of the field widths; the chunk's
 //     words (at most 64) sit one per lane and are fetched by shuffle;
 //   - columns: inclusive scan of the fields minus the scan just before the
@@ -899,12 +900,17 @@ __global__ __launch_bounds__(256) void load_kernel(SchedState ss, Parts pt) {
 //   - the 64 columns of a chunk are stored by consecutive lanes.
 constexpr uint32_t kUnpackChunks = 8;
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
-#pragma unroll
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: shifts by
+// 1, 2, 4 and 8 within each row of 16 lanes (lanes shifted in from outside
+// the row read 0), then row 0's total is added to row 1 and row 2's to row 3
+// (row_bcast:15), then lane 31's sum to rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); // row_bcast:15, rows 1, 3
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); // row_bcast:31, rows 2, 3
     return x;
 }
 
@@ -921,6 +927,8 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *__restrict__
     const uint64_t eb = rowptr[0];
     const uint32_t *meta = (const uint32_t *)img;
     const uint32_t *wd = meta + 2 * nch;
+    __shared__ uint8_t s_starts[4][64];  // per wave: chunk edges that start a row
+    volatile uint8_t *starts = s_starts[threadIdx.x >> 6];
 
     // row r0 holding edge 64*cb: the last r < n with rowptr[r] - eb <= 64*cb
     uint64_t e0 = cb * 64;
@@ -945,34 +953,29 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *__restrict__
         const uint64_t w_beg = meta[2 * c], w_end = c + 1 < nch ? meta[2 * c + 2] : nwords;
         const uint32_t bw = (meta[2 * c + 1] >> ((lane >> 4) * 8)) & 0xFFu;
         const uint32_t word = w_beg + lane < w_end ? wd[w_beg + lane] : 0u;
-        // lane's row offset from r0: rows r0+1.. ending at or before the lane's edge
-        uint32_t rowoff = 0, adv = 0;
+        // rows r0+1.. that end inside the chunk: a row ending at chunk edge i
+        // means edge i starts a row; mark those edges in the wave's LDS flags
+        uint32_t adv = 0;
+        starts[lane] = 0;
         for (uint32_t base = r0 + 1;; base += 64) {
             const uint32_t idx = base + lane;
             const uint64_t end = idx <= n ? rowptr[idx] - eb : ~0ull;  // > e0 for every idx > r0
             const uint32_t p = (uint32_t)(end - e0 < 65 ? end - e0 : 65);  // 1..65
-            const uint32_t p63 = __shfl(p, 63, 64);
-            uint32_t k = 0;
-#pragma unroll
-            for (uint32_t st = 32; st > 0; st >>= 1) {
-                const uint32_t v = __shfl(p, k + st - 1, 64);
-                if (v <= lane) k += st;
-            }
-            if (k == 63 && p63 <= lane) k = 64;
-            rowoff += k;
+            if (p < 64) starts[p] = 1;
+            const uint32_t p63 = (uint32_t)__builtin_amdgcn_readlane((int)p, 63);
             adv += (uint32_t)__popcll(__ballot(p <= 64));
             if (p63 > 64 || base + 64 > n) break;
         }
         const bool valid = lane < cnt;
-        const uint32_t prev_off = __shfl_up(rowoff, 1, 64);
-        const bool anchor = valid && (lane == 0 || rowoff != prev_off || bw == 0xFFu);
+        const bool anchor = valid && (lane == 0 || starts[lane] || bw == 0xFFu);
         const uint32_t len = valid ? (anchor ? vbits : bw) : 0u;
         const uint32_t pos = wave_incl_scan(len, lane) - len;
         const uint32_t wi = pos >> 5;
         const uint32_t lo_w = __shfl(word, wi, 64);
         const uint32_t hi_w = __shfl(word, (wi + 1) & 63u, 64);
-        const uint64_t win = (uint64_t)lo_w | ((uint64_t)(wi + 1 < 64 ? hi_w : 0u) << 32);
-        const uint32_t v = (uint32_t)((win >> (pos & 31u)) & ((1ull << len) - 1));
+        // bits [pos, pos + len) of the chunk: (hi:lo) >> (pos % 32), low len bits
+        const uint32_t bits = __builtin_amdgcn_alignbit(wi + 1 < 64 ? hi_w : 0u, lo_w, pos & 31u);
+        const uint32_t v = len >= 32 ? bits : bits & ((1u << len) - 1u);
         const uint32_t incl = wave_incl_scan(v, lane);
         const uint64_t heads = __ballot(anchor) & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
         const uint32_t head = 63u - (uint32_t)__clzll(heads);  // lane 0 is always a head
