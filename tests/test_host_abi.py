@@ -187,6 +187,16 @@ def test_engine_without_device_fails_loudly():
 
 
 @pytest.mark.skipif(not _no_device(), reason="a HIP device is visible")
+def test_bandwidth_probes_without_device():
+    import ctypes
+    L = lt.lib()
+    gbps = ctypes.c_double()
+    assert L.lt_device_h2d_bandwidth(0, 1, ctypes.byref(gbps)) == 6  # LT_E_ARG
+    assert L.lt_device_h2d_bandwidth(1 << 20, 1, ctypes.byref(gbps)) == 7  # LT_E_NODEVICE
+    assert L.lt_device_copy_bandwidth(1 << 20, 1, ctypes.byref(gbps)) == 7
+
+
+@pytest.mark.skipif(not _no_device(), reason="a HIP device is visible")
 def test_cli_materialises_partitions_then_needs_a_device(tmp_path):
     gpath = str(tmp_path / "g.bcsr")
     open(gpath, "wb").write(open(os.path.join(GOLDEN, "rmat_s10_ef8.bcsr"), "rb").read())
