@@ -196,3 +196,40 @@ def test_trained_scorer_gains_over_untrained_weights():
     assert res["trained"][0] >= res["untrained"][0] + 0.1
     assert res["trained"][1] < res["untrained"][1]
     assert res["trained"][1] <= 1.5 * plain_loads
+
+
+def test_weights_carry_over_to_another_graph(tmp_path):
+    """Weights trained on one R-MAT graph, saved and loaded into the scorer of
+    another graph of the same shape (other generator seed, its own partition
+    table), still prefetch partitions that are walked next; counts are
+    identical to the plain scheduler's."""
+    log("scorer weights on another graph")
+    common = dict(span="resident", zc_batch=16)
+    ga = lt.Graph.rmat(20, 16)
+    pa = lt.Partition.build(ga, 2 << 20)
+    wa = 2 * ga.num_node
+    plain = lt.Engine(ga, pa, num_device_partitions=pa.count // 2, record_trace=1,
+                      zerocopy_threshold=pa.bytes // 128, **common)
+    plain.pagerank(wa, 10)
+    sa = lt.Scorer(pa, seed=5)
+    sa.add_trace(plain, True, True)
+    sa.fit(epochs=20)
+    path = str(tmp_path / "weights.bin")
+    sa.save(path)
+    plain.close()
+    gb = lt.Graph.rmat(20, 16, seed=0x5EED4321)
+    pb = lt.Partition.build(gb, 2 << 20)
+    wb = 2 * gb.num_node
+    kw = dict(num_device_partitions=pb.count // 2, zerocopy_threshold=pb.bytes // 128, **common)
+    ref = lt.Engine(gb, pb, **kw).pagerank(wb, 10)
+    sb = lt.Scorer(pb, seed=9)
+    sb.load(path)
+    eng = lt.Engine(gb, pb, policy="learned", **kw)
+    eng.set_scorer(sb)
+    eng.pagerank(wb, 10)
+    counts = eng.pagerank(wb, 10)
+    assert np.array_equal(counts, ref)
+    st = eng.stats()
+    hit = st["prefetch_hits"] / max(1, st["prefetch_issued"])
+    log(f"P={pb.count}: prefetches {st['prefetch_issued']}, hit rate {hit:.2f}")
+    assert st["prefetch_issued"] > 20 and hit >= 0.5
