@@ -189,7 +189,11 @@ def test_dangling_vertices(kw):
 
 
 @pytest.mark.parametrize("kw", [{}, {"device_schedule": 1}, {"device_schedule": 1, "span": "resident"},
-                                {"span": "resident", "zerocopy_threshold": 5000, "zc_batch": 16}])
+                                {"span": "resident", "zerocopy_threshold": 5000, "zc_batch": 16},
+                                # the C4 autotuner's usual pick: packed copies, resident span, batches
+                                {"span": "resident", "zerocopy_threshold": 500, "zc_batch": 16,
+                                 "pack_partitions": 1},
+                                {"pack_partitions": 1, "zerocopy_threshold": 0, "slice_chunks": 1}])
 def test_ppr_batch_bit_exact(g12, kw):
     log(f"ppr batch {kw}")
     part = lt.Partition.build(g12, 16 << 10)
