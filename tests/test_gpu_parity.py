@@ -136,7 +136,8 @@ def test_genericwalk_bit_exact(g12, label, pbytes, kw):
 
 
 @pytest.mark.parametrize("label,pbytes,kw", [c for c in CONFIGS if c[0] in (
-    "one-partition", "capped", "all-zc", "span-resident", "device-sched-zc", "small-slices", "zc-batch")],
+    "one-partition", "capped", "all-zc", "span-resident", "device-sched-zc", "small-slices", "zc-batch",
+    "packed-capped", "packed-learned-zc")],
     ids=lambda x: x if isinstance(x, str) else "")
 def test_genericwalk_paths_bit_exact(g12, label, pbytes, kw):
     """Whole paths (layout of src/walk/NodeSampler.h:36-49) against the oracle."""
@@ -153,11 +154,15 @@ def test_genericwalk_paths_with_dead_ends():
     g = _dangling_graph()
     part = lt.Partition.build(g, 2048)
     eng = lt.Engine(g, part, num_device_partitions=3, zerocopy_threshold=100)
+    # unsorted neighbour lists: packed images hold blocks stored absolute
+    packed = lt.Engine(g, part, num_device_partitions=3, zerocopy_threshold=0, pack_partitions=1)
     for l in (0, 1, 7):
         paths = eng.genericwalk_paths(4000, l)
         ref = oracle.genericwalk_paths(g.rowptr(), g.col(), 4000, l)
         assert np.array_equal(paths, ref)
+        assert np.array_equal(packed.genericwalk_paths(4000, l), ref)
     assert (ref == 0xFFFFFFFF).any()
+    assert packed.stats()["packed_image_bytes"] > 0
 
 
 def _dangling_graph():
