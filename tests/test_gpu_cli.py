@@ -94,3 +94,8 @@ def test_learned_scheduler_from_saved_scorer(graph_file, tmp_path):
     assert "result correct" in out
     # same walkers and seed as the engine run above, so the same moves
     assert grep_num(r'"moves": (\d+)', out) == ref_moves
+    # the fastest capped-cache setup: the same with packed partition copies
+    out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 60000, "-l", 10, "--scheduler", "learned",
+              "--scorer", weights, "--span", "resident", "--pack", "--json")
+    assert "result correct" in out and "H2D bytes unpacked" in out
+    assert grep_num(r'"moves": (\d+)', out) == ref_moves
