@@ -26,9 +26,11 @@ self loops and isolated vertices removed):
       autotuner (partition size 16/32/64 MB, span, host or device scheduling)
 For c5 the learned scorer is trained here, on the GPU, from the traces of the
 plain scheduler's warm-up runs; `value` is the learned scheduler's steps/s with
-partitions copied packed (gap-coded columns unpacked in HBM).  Beside it: the
-plain scheduler as the reference runs it (raw copies), the plain scheduler with
-the same packed copies, and the learned scheduler with raw copies.
+partitions kept packed in HBM (16-edge frame-of-reference blocks, copied as
+they are and decoded by the walk, so the 8 GB cache holds more partitions).
+Beside it: the plain scheduler as the reference runs it (raw copies), the plain
+scheduler with the same block copies and with gap-coded copies unpacked in HBM,
+and the learned scheduler with gap-coded and with raw copies.
 """
 import argparse
 import json
@@ -266,17 +268,20 @@ def main():
                 ref = eng.pagerank(w, wl["l"])
             prep_s += time.perf_counter() - t1
             eng.close()
-            # the same rules with packed partition copies
-            eng = lt.Engine(g, part, span=span, pack_partitions=1, **base)
-            plain[span + "_packed"] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span} packed")
-            eng.close()
+            # the same rules with packed partition copies: gap-coded and unpacked
+            # in HBM (1), or blocks kept packed in the slots (2)
+            for mode, tag in ((1, "_packed"), (2, "_blocks")):
+                eng = lt.Engine(g, part, span=span, pack_partitions=mode, **base)
+                plain[span + tag] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}{tag}")
+                eng.close()
         # plain rules with all of the learned configuration's non-learned settings:
         # byte break-even zero-copy threshold, zero-copy batches, packed copies
-        eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
-                        pack_partitions=1, **base)
-        plain["tuned_packed"] = measure(eng, wl, w, args.warmup, args.steps,
-                                        "plain/resident S/128 zc-batch 16 packed")
-        eng.close()
+        for mode, tag in ((1, "_packed"), (2, "_blocks")):
+            eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
+                            pack_partitions=mode, **base)
+            plain["tuned" + tag] = measure(eng, wl, w, args.warmup, args.steps,
+                                           f"plain/resident S/128 zc-batch 16{tag}")
+            eng.close()
         t1 = time.perf_counter()
         loss = sc.fit(epochs=args.epochs, batch=32, lr=1e-3)
         train_s = time.perf_counter() - t1 + prep_s
@@ -287,15 +292,20 @@ def main():
         eng.close()
         eng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=1, **base)
         eng.set_scorer(sc)
-        learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident packed")
-        same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
+        learned_gap = measure(eng, wl, w, args.warmup, args.steps, "learned/resident packed")
         eng.close()
-        teng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=1, event_timers=1, **base)
+        eng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=2, **base)
+        eng.set_scorer(sc)
+        learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident blocks")
+        same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
+        slots = eng.stats()["cache_slots"]
+        eng.close()
+        teng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=2, event_timers=1, **base)
         teng.set_scorer(sc)
         learned.update(timed_walk(teng, wl, w))
         teng.close()
         ref_plain = plain["partition"]
-        same_copy = {k: v for k, v in plain.items() if k.endswith("packed")}
+        same_copy = {k: v for k, v in plain.items() if k.endswith("blocks")}
         best_name = min(same_copy, key=lambda k: same_copy[k]["ms_per_step"])
         best_plain = same_copy[best_name]
 
@@ -310,15 +320,20 @@ def main():
             "e2e_ms_per_step": learned["e2e_ms_per_step"],
             "config": {"workload": desc, "scheduler": "learned (prefetch depth 2, span resident, zero-copy "
                                                       "threshold S/128, zero-copy batches of 16), partitions "
-                                                      "copied packed (gap-coded columns, unpacked in HBM)",
+                                                      "kept packed in HBM (16-edge frame-of-reference blocks, "
+                                                      f"{slots} cache slots in the 8 GB)",
                        "device": info["name"]},
             "h2d_bytes_per_step": learned["h2d_bytes_per_step"],
             "plain": {"span_partition": {k: ref_plain[k] for k in keep},
                       "span_partition_packed": {k: plain["partition_packed"][k] for k in keep},
+                      "span_partition_blocks": {k: plain["partition_blocks"][k] for k in keep},
                       "span_resident": {k: plain["resident"][k] for k in keep},
                       "span_resident_packed": {k: plain["resident_packed"][k] for k in keep},
-                      "span_resident_S128_zcbatch16_packed": {k: plain["tuned_packed"][k] for k in keep}},
-            "learned": {"packed": {k: learned[k] for k in keep}, "unpacked": {k: learned_raw[k] for k in keep}},
+                      "span_resident_blocks": {k: plain["resident_blocks"][k] for k in keep},
+                      "span_resident_S128_zcbatch16_packed": {k: plain["tuned_packed"][k] for k in keep},
+                      "span_resident_S128_zcbatch16_blocks": {k: plain["tuned_blocks"][k] for k in keep}},
+            "learned": {"blocks": {k: learned[k] for k in keep}, "packed": {k: learned_gap[k] for k in keep},
+                        "unpacked": {k: learned_raw[k] for k in keep}},
             "learned_vs_plain": {
                 "speedup_vs_reference_plain": learned["steps_per_s"] / ref_plain["steps_per_s"],
                 "h2d_bytes_ratio_vs_reference_plain": learned["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"],
@@ -327,6 +342,10 @@ def main():
                 "unpacked_learned_vs_reference_plain": {
                     "speedup": learned_raw["steps_per_s"] / ref_plain["steps_per_s"],
                     "h2d_bytes_ratio": learned_raw["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"]},
+                "gap_coded_learned_vs_plain_gap_coded": {
+                    k: {"speedup": learned_gap["steps_per_s"] / plain[k]["steps_per_s"],
+                        "h2d_bytes_ratio": learned_gap["h2d_bytes_per_step"] / plain[k]["h2d_bytes_per_step"]}
+                    for k in plain if k.endswith("packed")},
                 "visit_counts_identical": same,
                 "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
                 "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],
@@ -354,9 +373,10 @@ def main():
         desc = (f"c4: R-MAT scale {wl['scale']} ef {wl['ef']} (V={g.num_node}, E={g.num_edge}), batched ppr "
                 f"{wl['nq']} queries (sources ~ degree, counts log-uniform [1e3,1e6), {int(cnt.sum())} walkers), "
                 f"p=0.15, 8 GB cache; autotuned: {best['partition_mb']} MB partitions (P={part.count}), "
-                f"{best['slots']} slots, span {best['span']}, "
+                f"{best['cache_slots']} cache slots, span {best['span']}, "
                 f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}, "
-                f"zero-copy batch {best['zc_batch']}, {'packed' if best['pack_partitions'] else 'raw'} copies")
+                f"zero-copy batch {best['zc_batch']}, "
+                f"{('raw', 'gap-coded', 'block')[best['pack_partitions']]} copies")
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "median_ms_per_step": m["median_ms_per_step"], "e2e_ms_per_step": m["e2e_ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "device": info["name"]},
@@ -365,7 +385,7 @@ def main():
                     "autotune": {"seconds": tune_s, "sample": "every 16th query", "best": best,
                                  "trials": [{k: t[k] for k in ("partition_mb", "span", "device_schedule",
                                                                "slice_chunks", "zc_batch", "pack_partitions",
-                                                               "running_ms", "h2d_bytes")}
+                                                               "cache_slots", "running_ms", "h2d_bytes")}
                                             for t in trials]}})
     else:
         ekw = dict(wl.get("eng", {}))
@@ -392,7 +412,10 @@ def main():
             for name, kw, k in (("reference_semantics", {}, args.steps), ("explicit_only", dict(zerocopy_threshold=0), 1),
                                 ("zerocopy_only", dict(zerocopy_threshold=1 << 62), 1),
                                 ("span_resident_zc_batch16", dict(span="resident", zc_batch=16), 2),
-                                ("packed", dict(pack_partitions=1), 2)):
+                                ("packed", dict(pack_partitions=1), 2),
+                                ("blocks", dict(pack_partitions=2), 2),
+                                ("span_resident_zc_batch16_blocks", dict(span="resident", zc_batch=16,
+                                                                         pack_partitions=2), 2)):
                 e2 = lt.Engine(g, part, **kw, **base)
                 v = measure(e2, wl, w, 1, k, f"c3/{name}")
                 variants[name] = {x: v[x] for x in keep}

@@ -88,8 +88,13 @@ typedef struct {
                                        off)                                     */
     int32_t  pack_partitions;       /* 1: copy partition columns gap-coded
                                        (lt_partition_pack) and unpack them in
-                                       HBM; a partition that does not get
-                                       smaller is copied as is (default 0: off;
+                                       HBM; 2: copy them as 16-edge
+                                       frame-of-reference blocks that stay
+                                       packed in the cache slot and are read in
+                                       place, so slots are smaller and the
+                                       cache holds more partitions.  A
+                                       partition that does not get smaller is
+                                       copied as is (default 0: off;
                                        device_schedule: unused)                 */
 } lt_config;
 
@@ -123,6 +128,7 @@ typedef struct {
     uint64_t packed_image_bytes;    /* packed column images copied (part of
                                        h2d_bytes_explicit)                      */
     uint64_t packed_col_bytes;      /* column bytes those images unpacked to    */
+    uint64_t cache_slots;           /* partitions the HBM cache holds at once   */
     uint64_t reserved[1];
 } lt_stats;
 
@@ -186,7 +192,23 @@ void lt_partition_free(lt_partition *p);
  * large enough. */
 lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
                             uint64_t *bytes_out);
-/* An engine with pack_partitions = 1 builds the packed images of its
+/* Block image of partition part's column slice, the form in which an engine
+ * with pack_partitions = 2 copies it to HBM and keeps it there, read in
+ * place by the walk (replaces the raw col slice of the cache slot,
+ * GPUGraph::loadAsync, src/GraphLoader.h:531-552).  For m edges in
+ * nblk = ceil(m/16) blocks of 16 (partition-local edges 16k .. 16k+15; the
+ * last block padded with its base):
+ *   u32 hdr[2*(nblk+1)]: hdr[2k] = base of block k (its smallest column),
+ *       hdr[2k+1] = its first bit in words[]; hdr[2*nblk] = 0,
+ *       hdr[2*nblk+1] = total bits;
+ *   u32 words[]: the blocks' bit streams back to back (LSB first), then two
+ *       zero words.
+ * Block k stores column - base of each of its 16 edges in
+ * w = (hdr[2k+3] - hdr[2k+1]) / 16 bits.  Same size query and write rules as
+ * lt_partition_pack.  No counterpart in the reference. */
+lt_status lt_partition_pack_blocks(const lt_graph *g, const lt_partition *p, uint32_t part, void *out,
+                                   uint64_t cap, uint64_t *bytes_out);
+/* An engine with pack_partitions = 1 or 2 builds the packed images of its
  * partition table once (pinned host memory, about half the graph's column
  * bytes) and caches them on the graph for later engines.  This drops the
  * cache; engines already using an image keep it until they are destroyed.

@@ -67,7 +67,8 @@ class Stats(ctypes.Structure):
                 ("nominal_steps", _u64), ("kernel_launches", _u64), ("walk_kernel_ms", ctypes.c_double),
                 ("learned_decisions", _u64), ("fallback_decisions", _u64), ("prefetch_issued", _u64),
                 ("prefetch_hits", _u64), ("h2d_bytes_unpacked", _u64), ("packed_image_bytes", _u64),
-                ("packed_col_bytes", _u64), ("reserved", _u64 * 1)]
+                ("packed_col_bytes", _u64), ("cache_slots", _u64),
+                ("reserved", _u64 * 1)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
@@ -108,6 +109,7 @@ _sig("lt_partition_edge_offsets", _vp, _vp)
 _sig("lt_partition_find", _u32, _vp, _u32)
 _sig("lt_partition_free", None, _vp)
 _sig("lt_partition_pack", ctypes.c_int, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64))
+_sig("lt_partition_pack_blocks", ctypes.c_int, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64))
 _sig("lt_graph_drop_packed", None, _vp)
 _sig("lt_engine_fetch_partition", ctypes.c_int, _vp, _u32, _vp, _vp)
 _sig("lt_engine_create", ctypes.c_int, _vp, _vp, ctypes.POINTER(Config), ctypes.POINTER(_vp))
@@ -310,14 +312,16 @@ class Partition:
     def find(self, v):
         return _lib.lt_partition_find(self._h, v)
 
-    def pack(self, graph, part):
+    def pack(self, graph, part, blocks=False):
         """Packed image (u8 array) of partition `part`'s column slice, the
-        form an engine with pack_partitions=1 copies to HBM."""
+        form an engine with pack_partitions=1 copies to HBM (blocks=True: the
+        block image that pack_partitions=2 keeps in HBM)."""
+        name = "lt_partition_pack_blocks" if blocks else "lt_partition_pack"
+        fn = getattr(_lib, name)
         n = _u64()
-        _check(_lib.lt_partition_pack(graph._h, self._h, part, None, 0, ctypes.byref(n)), "lt_partition_pack")
+        _check(fn(graph._h, self._h, part, None, 0, ctypes.byref(n)), name)
         out = np.zeros(n.value, np.uint8)
-        _check(_lib.lt_partition_pack(graph._h, self._h, part, out.ctypes.data, n.value, ctypes.byref(n)),
-               "lt_partition_pack")
+        _check(fn(graph._h, self._h, part, out.ctypes.data, n.value, ctypes.byref(n)), name)
         return out
 
     def close(self):

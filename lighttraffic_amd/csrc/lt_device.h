@@ -69,7 +69,8 @@ struct Parts {
     const uint64_t *edge_off;  // [P+1]
     const uint32_t *lut;       // partition of vertex (j << lut_shift)
     const uint64_t *const *rowptr_tab;  // [P] HBM slot rowptr, nullptr if not resident
-    const uint32_t *const *col_tab;     // [P]
+    const uint32_t *const *col_tab;     // [P] raw col slice, or the block headers of a packed slot
+    const uint32_t *const *words_tab;   // [P] bit streams of a packed slot, nullptr for raw columns
     const uint8_t *zc_part;             // [P] 1 while the table points p at host memory (zero-copy batch)
     uint32_t lut_shift;
     uint32_t P;
@@ -134,7 +135,8 @@ struct WalkArgs {
     uint32_t chunk_lo;    // first chunk of the slice
     uint32_t chunk_hi;    // one past the last chunk of the slice
     const uint64_t *rowptr;  // rowptr slice of src (HBM slot or mapped host)
-    const uint32_t *col;     // col slice of src
+    const uint32_t *col;     // col slice of src (block headers if words is set)
+    const uint32_t *words;   // bit streams of src's packed slot, nullptr for raw columns
     int zerocopy;         // src slice is read over PCIe
     int span_resident;    // keep walking through HBM-resident partitions
     unsigned long long *stats;

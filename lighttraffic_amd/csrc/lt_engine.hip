@@ -162,9 +162,12 @@ struct Engine {
     uint32_t lut_shift = 0;
     const uint64_t **d_rowptr_tab = nullptr;
     const uint32_t **d_col_tab = nullptr;
+    const uint32_t **d_words_tab = nullptr;  // packed slots (pack_partitions = 2): bit streams
 
     // HBM partition cache
     uint32_t nslots = 0;
+    uint64_t slot_sz = 0;        // bytes per slot (256-aligned)
+    bool slot_packed = false;    // slots keep columns packed (pack_partitions = 2)
     uint8_t *d_slots = nullptr;
     std::vector<int32_t> slot_of;      // partition -> slot or -1
     std::vector<int32_t> part_in_slot; // slot -> partition or -1
@@ -284,6 +287,13 @@ struct Engine {
     lt_status load_async(uint32_t p);
     lt_status copy_partition(uint32_t p, uint8_t *base);
     lt_status fetch_partition(uint32_t p, uint64_t *rowptr_out, uint32_t *col_out);
+    // Bit streams of p's slot whose column part starts at cp: past the block
+    // headers when the slot holds p's block image, nullptr for raw columns.
+    const uint32_t *slot_words(uint32_t p, const uint32_t *cp) const {
+        if (!slot_packed || !pk->size[p]) return nullptr;
+        const uint64_t nblk = (part.edge_off[p + 1] - part.edge_off[p] + kBlockEdges - 1) / kBlockEdges;
+        return cp + 2 * (nblk + 1);
+    }
     lt_status set_table(uint32_t p, bool present);
     lt_status scheduler_loop();
     lt_status scheduler_loop_learned();
@@ -315,6 +325,7 @@ Engine::~Engine() {
     dfree(d_lut);
     dfree(d_rowptr_tab);
     dfree(d_col_tab);
+    dfree(d_words_tab);
     dfree(d_slots);
     dfree(q.slab);
     dfree(q.frame);
@@ -378,6 +389,10 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
         set_error("empty graph");
         return LT_E_ARG;
     }
+    if (cfg.pack_partitions < 0 || cfg.pack_partitions > 2) {
+        set_error("pack_partitions must be 0, 1 or 2 (got %d)", cfg.pack_partitions);
+        return LT_E_ARG;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         (void)hipGetLastError();
@@ -421,10 +436,38 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     LT_TRY(dmalloc(&d_col_tab, P));
     LT_HIP(hipMemset(d_rowptr_tab, 0, sizeof(void *) * P));
     LT_HIP(hipMemset(d_col_tab, 0, sizeof(void *) * P));
+    LT_TRY(dmalloc(&d_words_tab, P));
+    LT_HIP(hipMemset(d_words_tab, 0, sizeof(void *) * P));
+
+    // Packed copies: the images are built once per graph and partition table.
+    // Format 1 travels gap-coded and is unpacked into raw slots; format 2 is
+    // read in place, so slots are as large as the largest packed partition and
+    // the same cache bytes hold more of them.
+    s_dma = s_copy;
+    if (cfg.pack_partitions > 0 && !cfg.device_schedule) {
+        LT_TRY(packed_image(*g, part, cfg.pack_partitions >= 2 ? 2 : 1, &pk_hold));
+        pk = pk_hold.get();
+        if (!pk->max_size) {
+            pk = nullptr;  // no partition gets smaller
+            pk_hold.reset();
+        }
+    }
+    slot_packed = pk && pk->format == 2;
 
     // HBM cache slots (GPUGraph, src/GraphLoader.h:513-520)
     nslots = cfg.num_device_partitions == 0 ? P : std::min(cfg.num_device_partitions, P);
-    const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
+    const uint64_t raw_slot = (part.bytes + 255) & ~255ull;
+    uint64_t slot_bytes = raw_slot;
+    if (slot_packed) {
+        uint64_t need = 0;
+        for (uint32_t i = 0; i < P; i++) {
+            const uint64_t rb = ((uint64_t)part.node_off[i + 1] - part.node_off[i] + 1) * 8;
+            need = std::max(need, rb + (pk->size[i] ? pk->size[i] : 4 * (part.edge_off[i + 1] - part.edge_off[i])));
+        }
+        slot_bytes = (need + 255) & ~255ull;
+        nslots = (uint32_t)std::min<uint64_t>(P, (uint64_t)nslots * raw_slot / slot_bytes);
+    }
+    slot_sz = slot_bytes;
     // The whole rowptr array in HBM, paid for with slots of the same cache:
     // a zero-copied move then reads only its column entry over PCIe.
     const uint64_t rp_bytes = 8ull * ((uint64_t)V + 1);
@@ -451,28 +494,19 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     else zc_threshold = cfg.policy == LT_SCHED_LEARNED ? part.bytes / 128 : part.bytes / 128 / 2;
     LT_HIP(hipHostGetDevicePointer((void **)&zc_rowptr, g->rowptr, 0));
     LT_HIP(hipHostGetDevicePointer((void **)&zc_col, g->col, 0));
-    // Packed copies: the images are built once per graph and partition table.
-    s_dma = s_copy;
-    if (cfg.pack_partitions > 0 && !cfg.device_schedule) {
-        LT_TRY(packed_image(*g, part, &pk_hold));
-        pk = pk_hold.get();
-        if (pk->max_size) {
-            for (int k = 0; k < 2; k++) {
-                LT_TRY(dmalloc(&d_stage[k], pk->max_size));
-                LT_HIP(hipEventCreateWithFlags(&stage_free[k], hipEventDisableTiming));
-            }
-            // HIP spreads streams over GPU_MAX_HW_QUEUES hardware queues (4
-            // unless set).  A DMA stream that shares a queue with the walk
-            // stream waits behind walk kernels, so it is used only when the
-            // streams in play (null, walk, copy, DMA, a scorer's) get their own.
-            const char *hwq = getenv("GPU_MAX_HW_QUEUES");
-            if (hwq && atoi(hwq) >= 6) {
-                LT_HIP(hipEventCreateWithFlags(&ev_dma, hipEventDisableTiming));
-                LT_HIP(hipStreamCreateWithFlags(&s_dma, hipStreamNonBlocking));
-            }
-        } else {
-            pk = nullptr;  // no partition gets smaller
-            pk_hold.reset();
+    if (pk && pk->format == 1) {  // staging buffers for the images unpacked in HBM
+        for (int k = 0; k < 2; k++) {
+            LT_TRY(dmalloc(&d_stage[k], pk->max_size));
+            LT_HIP(hipEventCreateWithFlags(&stage_free[k], hipEventDisableTiming));
+        }
+        // HIP spreads streams over GPU_MAX_HW_QUEUES hardware queues (4
+        // unless set).  A DMA stream that shares a queue with the walk
+        // stream waits behind walk kernels, so it is used only when the
+        // streams in play (null, walk, copy, DMA, a scorer's) get their own.
+        const char *hwq = getenv("GPU_MAX_HW_QUEUES");
+        if (hwq && atoi(hwq) >= 6) {
+            LT_HIP(hipEventCreateWithFlags(&ev_dma, hipEventDisableTiming));
+            LT_HIP(hipStreamCreateWithFlags(&s_dma, hipStreamNonBlocking));
         }
     }
     if (rp_resident) {
@@ -599,14 +633,14 @@ lt_status Engine::read_meta() {
 
 lt_status Engine::set_table(uint32_t p, bool present) {
     const uint64_t *rp = nullptr;
-    const uint32_t *cp = nullptr;
+    const uint32_t *cp = nullptr, *wp = nullptr;
     if (present) {
-        const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
-        uint8_t *base = d_slots + (size_t)slot_of[p] * slot_bytes;
+        uint8_t *base = d_slots + (size_t)slot_of[p] * slot_sz;
         rp = (const uint64_t *)base;
         cp = (const uint32_t *)(base + ((uint64_t)part.node_off[p + 1] - part.node_off[p] + 1) * 8);
+        wp = slot_words(p, cp);
     }
-    set_table_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, p, rp, cp);
+    set_table_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, d_words_tab, p, rp, cp, wp);
     LT_HIP(hipGetLastError());
     return LT_OK;
 }
@@ -632,9 +666,8 @@ lt_status Engine::load_async(uint32_t p) {
     }
     const uint32_t s = free_slots.back();
     free_slots.pop_back();
-    const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
     hipEvent_t t = t_load.begin(s_dma);
-    LT_TRY(copy_partition(p, d_slots + (size_t)s * slot_bytes));
+    LT_TRY(copy_partition(p, d_slots + (size_t)s * slot_sz));
     t_load.end(t, s_copy);
     slot_of[p] = (int32_t)s;
     part_in_slot[s] = (int32_t)p;
@@ -652,6 +685,21 @@ lt_status Engine::copy_partition(uint32_t p, uint8_t *base) {
     const size_t rbytes = ((size_t)ne - nb + 1) * 8, cbytes = (size_t)(ee - eb) * 4;
     const bool packed = pk && pk->size[p];
     uint32_t k = 0;
+    if (packed && slot_packed) {  // the block image is the slot's column part
+        if (rp_resident) {
+            LT_HIP(hipMemcpyAsync(base, d_rowptr_all + nb, rbytes, hipMemcpyDeviceToDevice, s_copy));
+        } else {
+            LT_HIP(hipMemcpyAsync(base, g->rowptr + nb, rbytes, hipMemcpyHostToDevice, s_copy));
+            st.h2d_bytes_explicit += rbytes;
+            st.h2d_bytes_unpacked += rbytes;
+        }
+        LT_HIP(hipMemcpyAsync(base + rbytes, pk->host + pk->off[p], pk->size[p], hipMemcpyHostToDevice, s_copy));
+        st.h2d_bytes_unpacked += cbytes;
+        st.h2d_bytes_explicit += pk->size[p];
+        st.packed_image_bytes += pk->size[p];
+        st.packed_col_bytes += cbytes;
+        return LT_OK;
+    }
     if (packed) {
         k = stage_next;
         stage_next ^= 1u;
@@ -696,6 +744,10 @@ lt_status Engine::copy_partition(uint32_t p, uint8_t *base) {
 // A partition's slot bytes, built by copy_partition in a scratch buffer and
 // copied back; the cache and the statistics are left as they were.
 lt_status Engine::fetch_partition(uint32_t p, uint64_t *rowptr_out, uint32_t *col_out) {
+    if (slot_packed) {
+        set_error("fetch of partition %u: slots keep columns packed (pack_partitions = 2)", p);
+        return LT_E_ARG;
+    }
     LT_HIP(hipSetDevice(device));
     LT_HIP(hipStreamSynchronize(s_copy));
     if (s_dma != s_copy) LT_HIP(hipStreamSynchronize(s_dma));
@@ -760,6 +812,7 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
     a.pt.lut = d_lut;
     a.pt.rowptr_tab = d_rowptr_tab;
     a.pt.col_tab = d_col_tab;
+    a.pt.words_tab = d_words_tab;
     a.pt.zc_part = d_zc_part;
     a.pt.lut_shift = lut_shift;
     a.pt.P = P;
@@ -772,10 +825,10 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
         a.rowptr = zc_rowptr + part.node_off[p];
         a.col = zc_col + part.edge_off[p];
     } else {
-        const uint64_t slot_bytes = (part.bytes + 255) & ~255ull;
-        const uint8_t *base = d_slots + (size_t)slot_of[p] * slot_bytes;
+        const uint8_t *base = d_slots + (size_t)slot_of[p] * slot_sz;
         a.rowptr = (const uint64_t *)base;
         a.col = (const uint32_t *)(base + ((uint64_t)part.node_off[p + 1] - part.node_off[p] + 1) * 8);
+        a.words = slot_words(p, a.col);
     }
     EventTimer &phase = zerocopy ? t_zc : t_compute;
     hipEvent_t tp = phase.begin(s_compute);
@@ -1033,6 +1086,7 @@ lt_status Engine::process_zc_batch(const std::vector<uint32_t> &parts) {
     a.pt.lut = d_lut;
     a.pt.rowptr_tab = d_rowptr_tab;
     a.pt.col_tab = d_col_tab;
+    a.pt.words_tab = d_words_tab;
     a.pt.zc_part = d_zc_part;
     a.pt.lut_shift = lut_shift;
     a.pt.P = P;
@@ -1383,7 +1437,7 @@ SchedState Engine::sched_state(bool device) const {
     ss.rowptr_tab = d_rowptr_tab;
     ss.col_tab = d_col_tab;
     ss.slots = d_slots;
-    ss.slot_bytes = (part.bytes + 255) & ~255ull;
+    ss.slot_bytes = slot_sz;
     ss.nslots = nslots;
     ss.zc_rowptr = zc_rowptr;
     ss.rp_host = rp_resident ? 0 : 1;
@@ -1421,6 +1475,7 @@ lt_status Engine::scheduler_loop_device() {
     a.pt.lut = d_lut;
     a.pt.rowptr_tab = d_rowptr_tab;
     a.pt.col_tab = d_col_tab;
+    a.pt.words_tab = d_words_tab;
     a.pt.zc_part = d_zc_part;
     a.pt.lut_shift = lut_shift;
     a.pt.P = P;
@@ -1488,6 +1543,7 @@ lt_status Engine::scheduler_loop_device() {
 lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoints_out, uint32_t *moves_out) {
     LT_HIP(hipSetDevice(device));
     memset(&st, 0, sizeof(st));
+    st.cache_slots = nslots;
     spill_h2d_bytes = 0;
     st.h2d_bytes_explicit = pending_h2d;  // the rowptr upload, once
     st.h2d_bytes_unpacked = pending_h2d;
@@ -1563,6 +1619,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         ia.pt.lut = d_lut;
         ia.pt.rowptr_tab = d_rowptr_tab;
         ia.pt.col_tab = d_col_tab;
+        ia.pt.words_tab = d_words_tab;
         ia.pt.zc_part = d_zc_part;
         ia.pt.lut_shift = lut_shift;
         ia.pt.P = P;

@@ -61,7 +61,19 @@ constexpr uint32_t kPackChunk = 64;
 constexpr uint32_t kPackBlock = 16;
 constexpr uint8_t kPackRaw = 0xFF;
 
+// Block image (pack_partitions = 2): the column slice in a form the walk
+// kernel reads in place, kept packed in the HBM slot.  For m edges in
+// nblk = ceil(m/16) blocks of 16 (the last padded with its base):
+//   u32x2 hdr[nblk + 1]: {base, bit offset} of block k; hdr[nblk] = {0, total bits};
+//   u32 words[]: the blocks' bit streams back to back (LSB first), then two
+//       zero words.
+// Block k holds col[e] - base for its 16 edges in w = (off[k+1] - off[k]) / 16
+// bits each (base = the block's smallest column), so column e is
+// hdr[e/16].base + bits(off[e/16] + (e % 16) * w, w).
+constexpr uint32_t kBlockEdges = 16;
+
 struct PackedImage {
+    int format = 1;                      // 1 gap-coded chunks, 2 blocks read in place
     uint64_t bytes = 0;                  // partition budget S the image is for
     std::vector<uint32_t> node_off;      // partition table it was built for
     std::vector<uint64_t> edge_off;
@@ -79,8 +91,12 @@ uint32_t pack_value_bits(uint32_t V);
 // exceed 2^32 words; writes it when out is not null (out must hold that many
 // zeroed bytes).
 uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, uint8_t *out);
-// The graph's image for partition table p, built on first call.
-lt_status packed_image(const Graph &g, const Partition &p, std::shared_ptr<const PackedImage> *out);
+// Bytes of the block image of rows [nb, ne), UINT64_MAX if its bit offsets
+// would exceed 32 bits; writes it when out is not null (zeroed, that size).
+uint64_t pack_blocks_encode(const Graph &g, uint32_t nb, uint32_t ne, uint8_t *out);
+// The graph's image for partition table p in `format` (1 or 2), built on
+// first call.
+lt_status packed_image(const Graph &g, const Partition &p, int format, std::shared_ptr<const PackedImage> *out);
 
 }  // namespace lt
 

@@ -227,11 +227,26 @@ __device__ void flush_counters(const Counters &c, unsigned long long *stats) {
 // The graph span a walker is currently walking in.
 struct Span {
     const uint64_t *rowptr;  // rowptr[v - nb] = global offset of v's first edge
-    const uint32_t *col;     // col[e - eb]
+    const uint32_t *col;     // col[e - eb], or the block headers of a packed slot
+    const uint32_t *words;   // bit streams of a packed slot, nullptr for raw columns
     uint64_t eb;
     uint32_t nb, ne;
     int zc;
 };
+
+// Column of partition-local edge e.  A packed slot holds, per block of 16
+// edges, {base, bit offset} (the next block's offset gives the width) and
+// the bit streams: column = base + bits(offset + (e % 16) * width, width).
+__device__ __forceinline__ uint32_t span_col(const Span &sp, uint64_t e) {
+    if (!sp.words) return sp.col[e];
+    const uint64_t k = e >> 4;
+    const uint2 h = ((const uint2 *)sp.col)[k];
+    const uint32_t w = (sp.col[2 * k + 3] - h.y) >> 4;
+    const uint32_t pos = h.y + (uint32_t)(e & 15) * w;
+    const uint32_t lo = sp.words[pos >> 5], hi = sp.words[(pos >> 5) + 1];
+    const uint32_t bits = __builtin_amdgcn_alignbit(hi, lo, pos & 31u);
+    return h.x + (w >= 32 ? bits : bits & ((1u << w) - 1u));
+}
 
 __device__ __forceinline__ uint64_t walker_id(const Walker &w) { return ((uint64_t)w.id_hi << 32) | w.id_lo; }
 
@@ -292,7 +307,7 @@ __device__ __forceinline__ void step_pagerank(Walker &w, const Span &sp, const A
             tele = true;
             if (sp.zc) ct.zc_reads += a.rp_host;
         } else {
-            w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
+            w.cur = span_col(sp, b - sp.eb + bounded(x.y, x.z, d));
             ct.moves++;
             if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
         }
@@ -316,7 +331,7 @@ __device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArg
     const uint64_t b = r[0], d = r[1] - b;
     if (d > 0) {
         const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_PPR);
-        w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
+        w.cur = span_col(sp, b - sp.eb + bounded(x.y, x.z, d));
         ct.moves++;
         if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
     } else {
@@ -336,7 +351,7 @@ __device__ __forceinline__ void step_generic(Walker &w, const Span &sp, const Ap
     if (path) path[a.length - w.step] = w.cur;  // the vertex after (length - step) moves
     if (d > 0) {
         const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_GENERIC);
-        w.cur = sp.col[b - sp.eb + bounded(x.y, x.z, d)];
+        w.cur = span_col(sp, b - sp.eb + bounded(x.y, x.z, d));
         ct.moves++;
         if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
         w.step -= 1;
@@ -384,6 +399,7 @@ __device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, uint32_t src, c
         if (!rp) return p;
         sp.rowptr = rp;
         sp.col = a.pt.col_tab[p];
+        sp.words = a.pt.words_tab[p];
         sp.nb = a.pt.node_off[p];
         sp.ne = a.pt.node_off[p + 1];
         sp.eb = a.pt.edge_off[p];
@@ -462,6 +478,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
             Span sp;
             sp.rowptr = a.ss.zc_rowptr + a.pt.node_off[src];
             sp.col = a.ss.zc_col + a.pt.edge_off[src];
+            sp.words = nullptr;
             sp.nb = a.pt.node_off[src];
             sp.ne = a.pt.node_off[src + 1];
             sp.eb = a.pt.edge_off[src];
@@ -476,6 +493,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
     Span src_sp;
     src_sp.rowptr = a.rowptr;
     src_sp.col = a.col;
+    src_sp.words = a.words;
     src_sp.zc = a.zerocopy;
     if (a.nhost) {  // walkers of src spilled to host memory, read over PCIe
         src_sp.nb = a.pt.node_off[src];
@@ -498,6 +516,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
         chunk_lo = 0;
         chunk_hi = 0xFFFFFFFFu;
         src_sp.zc = a.ss.ds->zerocopy;
+        src_sp.words = nullptr;  // device scheduling copies raw columns
         if (src_sp.zc) {
             src_sp.rowptr = a.ss.zc_rowptr + a.pt.node_off[src];
             src_sp.col = a.ss.zc_col + a.pt.edge_off[src];
@@ -733,11 +752,12 @@ __global__ void set_zc_tables_kernel(const uint64_t **rowptr_tab, const uint32_t
     }
 }
 
-__global__ void set_table_kernel(const uint64_t **rowptr_tab, const uint32_t **col_tab, uint32_t p,
-                                 const uint64_t *rp, const uint32_t *cp) {
+__global__ void set_table_kernel(const uint64_t **rowptr_tab, const uint32_t **col_tab, const uint32_t **words_tab,
+                                 uint32_t p, const uint64_t *rp, const uint32_t *cp, const uint32_t *wp) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         rowptr_tab[p] = rp;
         col_tab[p] = cp;
+        words_tab[p] = wp;
     }
 }
 

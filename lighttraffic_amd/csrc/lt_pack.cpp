@@ -92,6 +92,49 @@ uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, u
     return bytes;
 }
 
+uint64_t pack_blocks_encode(const Graph &g, uint32_t nb, uint32_t ne, uint8_t *out) {
+    const uint64_t eb = g.rowptr[nb], m = g.rowptr[ne] - eb;
+    const uint32_t *c = g.col + eb;
+    const uint64_t nblk = (m + kBlockEdges - 1) / kBlockEdges;
+    uint64_t bits = 0;
+    std::vector<uint32_t> base(nblk), width(nblk);
+    for (uint64_t k = 0; k < nblk; k++) {
+        const uint64_t a = k * kBlockEdges, b = std::min<uint64_t>(a + kBlockEdges, m);
+        uint32_t lo = c[a], hi = c[a];
+        for (uint64_t e = a + 1; e < b; e++) {
+            lo = std::min(lo, c[e]);
+            hi = std::max(hi, c[e]);
+        }
+        base[k] = lo;
+        width[k] = bitlen(hi - lo);
+        bits += (uint64_t)width[k] * kBlockEdges;
+    }
+    if (bits > UINT32_MAX) return UINT64_MAX;
+    const uint64_t words = (bits + 31) / 32;
+    const uint64_t bytes = 8 * (nblk + 1) + 4 * (words + 2);
+    if (!out) return bytes;
+    uint32_t *hdr = (uint32_t *)out;
+    uint32_t *wd = hdr + 2 * (nblk + 1);
+    uint64_t pos = 0;
+    for (uint64_t k = 0; k < nblk; k++) {
+        hdr[2 * k] = base[k];
+        hdr[2 * k + 1] = (uint32_t)pos;
+        const uint64_t a = k * kBlockEdges;
+        for (uint64_t j = 0; j < kBlockEdges; j++) {
+            const uint64_t e = a + j;
+            const uint64_t v = e < m ? (uint64_t)(c[e] - base[k]) : 0;  // padding: the base itself
+            const uint64_t sh = v << (pos % 32);
+            uint32_t *w = wd + pos / 32;
+            w[0] |= (uint32_t)sh;
+            if (pos % 32 + width[k] > 32) w[1] |= (uint32_t)(sh >> 32);
+            pos += width[k];
+        }
+    }
+    hdr[2 * nblk] = 0;
+    hdr[2 * nblk + 1] = (uint32_t)pos;
+    return bytes;
+}
+
 PackedImage::~PackedImage() {
     if (!host) return;
     if (pinned) (void)hipHostFree(host);
@@ -111,23 +154,26 @@ static void parallel_for(uint32_t n, F f) {
     for (auto &t : th) t.join();
 }
 
-lt_status packed_image(const Graph &g, const Partition &p, std::shared_ptr<const PackedImage> *out) {
+lt_status packed_image(const Graph &g, const Partition &p, int format, std::shared_ptr<const PackedImage> *out) {
     for (auto &pk : g.packs)
-        if (pk->bytes == p.bytes && pk->node_off == p.node_off && pk->edge_off == p.edge_off) {
+        if (pk->format == format && pk->bytes == p.bytes && pk->node_off == p.node_off && pk->edge_off == p.edge_off) {
             *out = pk;
             return LT_OK;
         }
     auto pk = std::make_shared<PackedImage>();
     const uint32_t P = p.P();
+    pk->format = format;
     pk->bytes = p.bytes;
     pk->node_off = p.node_off;
     pk->edge_off = p.edge_off;
     pk->vbits = pack_value_bits(g.V);
     pk->off.assign(P, 0);
     pk->size.assign(P, 0);
-    parallel_for(P, [&](uint32_t i) {
-        pk->size[i] = pack_encode(g, p.node_off[i], p.node_off[i + 1], pk->vbits, nullptr);
-    });
+    auto encode = [&](uint32_t i, uint8_t *dst) {
+        return format == 2 ? pack_blocks_encode(g, p.node_off[i], p.node_off[i + 1], dst)
+                           : pack_encode(g, p.node_off[i], p.node_off[i + 1], pk->vbits, dst);
+    };
+    parallel_for(P, [&](uint32_t i) { pk->size[i] = encode(i, nullptr); });
     // a partition is copied packed only when that copies fewer bytes
     for (uint32_t i = 0; i < P; i++) {
         const uint64_t raw = 4 * (p.edge_off[i + 1] - p.edge_off[i]);
@@ -157,7 +203,7 @@ lt_status packed_image(const Graph &g, const Partition &p, std::shared_ptr<const
             if (!pk->size[i]) return;
             uint8_t *dst = pk->host + pk->off[i];
             memset(dst, 0, pk->size[i]);
-            pack_encode(g, p.node_off[i], p.node_off[i + 1], pk->vbits, dst);
+            encode(i, dst);
         });
     }
     *out = pk;
@@ -173,28 +219,41 @@ void lt_graph_drop_packed(lt_graph *g) {
     if (g) g->packs.clear();
 }
 
-lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
-                            uint64_t *bytes_out) {
+static lt_status pack_one(const char *fn, int format, const lt_graph *g, const lt_partition *p, uint32_t part,
+                          void *out, uint64_t cap, uint64_t *bytes_out) {
     if (!g || !p || !bytes_out || part >= p->P() || p->node_off[part + 1] > g->V) {
-        lt::set_error("lt_partition_pack: null handle, or partition %u not in the graph", part);
+        lt::set_error("%s: null handle, or partition %u not in the graph", fn, part);
         return LT_E_ARG;
     }
+    const uint32_t nb = p->node_off[part], ne = p->node_off[part + 1];
     const uint32_t vbits = lt::pack_value_bits(g->V);
-    const uint64_t bytes = lt::pack_encode(*g, p->node_off[part], p->node_off[part + 1], vbits, nullptr);
+    auto encode = [&](uint8_t *dst) {
+        return format == 2 ? lt::pack_blocks_encode(*g, nb, ne, dst) : lt::pack_encode(*g, nb, ne, vbits, dst);
+    };
+    const uint64_t bytes = encode(nullptr);
     if (bytes == UINT64_MAX) {
-        lt::set_error("lt_partition_pack: partition %u too large to pack", part);
+        lt::set_error("%s: partition %u too large to pack", fn, part);
         return LT_E_ARG;
     }
     *bytes_out = bytes;
     if (!out) return LT_OK;
     if (cap < bytes) {
-        lt::set_error("lt_partition_pack: %llu bytes needed, %llu given", (unsigned long long)bytes,
-                      (unsigned long long)cap);
+        lt::set_error("%s: %llu bytes needed, %llu given", fn, (unsigned long long)bytes, (unsigned long long)cap);
         return LT_E_ARG;
     }
     memset(out, 0, bytes);
-    lt::pack_encode(*g, p->node_off[part], p->node_off[part + 1], vbits, (uint8_t *)out);
+    encode((uint8_t *)out);
     return LT_OK;
+}
+
+lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
+                            uint64_t *bytes_out) {
+    return pack_one("lt_partition_pack", 1, g, p, part, out, cap, bytes_out);
+}
+
+lt_status lt_partition_pack_blocks(const lt_graph *g, const lt_partition *p, uint32_t part, void *out,
+                                   uint64_t cap, uint64_t *bytes_out) {
+    return pack_one("lt_partition_pack_blocks", 2, g, p, part, out, cap, bytes_out);
 }
 
 }  // extern "C"

@@ -6,7 +6,8 @@ bytes), span (partition | resident), scheduling on the host or on the GPU
 (hipGraph-captured device loop), the number of 4096-walker chunks one walk
 launch consumes, (span resident, host scheduling) how many sub-threshold
 partitions one zero-copy launch walks, and (host scheduling) whether partitions
-are copied packed.  Every configuration produces the same endpoints, so the
+are copied raw, gap-coded and unpacked in HBM, or kept packed in HBM as
+16-edge blocks (the same cache bytes then hold more partitions).  Every configuration produces the same endpoints, so the
 choice only affects speed and PCIe bytes.
 """
 import itertools
@@ -19,7 +20,7 @@ from . import Engine, Partition
 
 def tune_ppr_batch(graph, sources, counts, prob=0.15, cache_bytes=8 << 30, sample_every=16,
                    sizes_mb=(16, 32, 64), spans=("partition", "resident"), device_schedule=(0, 1),
-                   slices=(2048, 8192), zc_batches=(1, 16), packs=(0, 1), log=None):
+                   slices=(2048, 8192), zc_batches=(1, 16), packs=(0, 1, 2), log=None):
     """Returns (best, trials): best is a dict of Partition/Engine settings, trials
     one dict per configuration with its time on the sample."""
     sel = np.arange(0, len(sources), sample_every)
@@ -45,7 +46,7 @@ def tune_ppr_batch(graph, sources, counts, prob=0.15, cache_bytes=8 << 30, sampl
             eng.close()
             rec = {"partition_mb": mb, "slots": slots, "partitions": part.count, "span": span,
                    "device_schedule": dev, "slice_chunks": sl, "zc_batch": zb, "pack_partitions": pk,
-                   "running_ms": st["running_ms"],
+                   "cache_slots": st["cache_slots"], "running_ms": st["running_ms"],
                    "wall_ms": wall * 1e3, "h2d_bytes": st["h2d_bytes_explicit"] + st["h2d_bytes_zerocopy"],
                    "iterations": st["iterations"]}
             trials.append(rec)

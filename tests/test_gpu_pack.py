@@ -127,6 +127,71 @@ def test_packed_byte_counters():
     log(f"explicit bytes raw {sa['h2d_bytes_explicit']} packed {sb['h2d_bytes_explicit']}")
 
 
+def test_block_slots_hold_more_partitions():
+    """pack_partitions=2 keeps 16-edge blocks packed in the slots: the same
+    cache bytes hold more partitions, fewer partition loads are needed, and
+    every walk equals the oracle's."""
+    from oracle import ffi as oracle
+    log("block slots")
+    g = lt.Graph.rmat(14, 16)
+    part = lt.Partition.build(g, 256 << 10)
+    kw = dict(num_device_partitions=part.count // 3, pipeline=0, zerocopy_threshold=0)
+    raw = lt.Engine(g, part, **kw)
+    blk = lt.Engine(g, part, pack_partitions=2, **kw)
+    w = 2 * g.num_node
+    rc, re, _ = oracle.pagerank(g.rowptr(), g.col(), w, 10, endpoints=True)
+    for eng in (raw, blk):
+        c, e = eng.pagerank(w, 10, endpoints=True)
+        assert np.array_equal(c, rc) and np.array_equal(e, re)
+    sa, sb = raw.stats(), blk.stats()
+    log(f"slots raw {sa['cache_slots']} blocks {sb['cache_slots']}; loads {sa['graph_loading_calls']} "
+        f"{sb['graph_loading_calls']}; explicit bytes {sa['h2d_bytes_explicit']} {sb['h2d_bytes_explicit']}")
+    assert sb["cache_slots"] > 1.3 * sa["cache_slots"]
+    assert sb["graph_loading_calls"] < sa["graph_loading_calls"]
+    assert 0 < sb["packed_image_bytes"] < 0.8 * sb["packed_col_bytes"]
+    ends, moves = blk.genericwalk(w, 10, moves=True)
+    re, rm, _ = oracle.genericwalk(g.rowptr(), g.col(), w, 10)
+    assert np.array_equal(ends, re) and np.array_equal(moves, rm)
+    counts, ends = blk.ppr(w, source=3, endpoints=True)
+    rc, re, _ = oracle.ppr(g.rowptr(), g.col(), w, source=3, endpoints=True)
+    assert np.array_equal(counts, rc) and np.array_equal(ends, re)
+    with pytest.raises(lt.LTError):
+        blk.fetch_partition(0, part.node_offsets(), part.edge_offsets())
+    raw.close()
+    blk.close()
+
+
+def test_block_slots_edgeless_and_unsorted_partitions():
+    from oracle import ffi as oracle
+    log("block slots: edgeless partitions, unsorted rows, hub row")
+    rng = np.random.default_rng(4)
+    V = 20000
+    deg = rng.integers(0, 40, V)
+    deg[123] = 5000
+    deg[1000:13000] = 0  # edgeless partitions (copied as a rowptr slice only)
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    col = np.empty(int(rowptr[-1]), np.uint32)
+    for r in range(V):
+        a, b = int(rowptr[r]), int(rowptr[r + 1])
+        x = np.sort(rng.integers(0, V, b - a))
+        if r % 5 == 0:
+            rng.shuffle(x)
+        col[a:b] = x
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, 32 << 10)
+    eo = part.edge_offsets()
+    assert sum(eo[p + 1] == eo[p] for p in range(part.count)) >= 2
+    eng = lt.Engine(g, part, pack_partitions=2, num_device_partitions=4, zerocopy_threshold=0)
+    w = 2 * V
+    counts, ends = eng.pagerank(w, 10, endpoints=True)
+    rc, re, _ = oracle.pagerank(rowptr, col, w, 10, endpoints=True)
+    assert np.array_equal(counts, rc) and np.array_equal(ends, re)
+    ends, moves = eng.genericwalk(w, 10, moves=True)
+    re, rm, _ = oracle.genericwalk(rowptr, col, w, 10)
+    assert np.array_equal(ends, re) and np.array_equal(moves, rm)
+    assert eng.stats()["packed_image_bytes"] > 0
+
+
 CHILD = r"""
 import sys
 import numpy as np

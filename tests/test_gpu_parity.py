@@ -90,6 +90,15 @@ CONFIGS = [
     ("packed-rp-resident", 16 << 10, {"num_device_partitions": 6, "zerocopy_threshold": 4000, "rowptr_resident": 1,
                                       "pack_partitions": 1}),
     ("packed-spill", 16 << 10, {"num_device_partitions": 4, "num_device_walkers": 5000, "pack_partitions": 1}),
+    # partitions kept packed in HBM as 16-edge blocks, decoded per read
+    ("blocks-capped", 16 << 10, {"num_device_partitions": 3, "pack_partitions": 2}),
+    ("blocks-learned-zc", 16 << 10, {"num_device_partitions": 3, "span": "resident", "policy": "learned",
+                                     "zerocopy_threshold": 3000, "pack_partitions": 2}),
+    ("blocks-batch", 16 << 10, {"num_device_partitions": 4, "span": "resident", "zerocopy_threshold": 3000,
+                                "zc_batch": 16, "pack_partitions": 2}),
+    ("blocks-rp-resident", 16 << 10, {"num_device_partitions": 6, "zerocopy_threshold": 4000, "rowptr_resident": 1,
+                                      "pack_partitions": 2}),
+    ("blocks-spill", 16 << 10, {"num_device_partitions": 4, "num_device_walkers": 5000, "pack_partitions": 2}),
 ]
 
 
@@ -156,13 +165,16 @@ def test_genericwalk_paths_with_dead_ends():
     eng = lt.Engine(g, part, num_device_partitions=3, zerocopy_threshold=100)
     # unsorted neighbour lists: packed images hold blocks stored absolute
     packed = lt.Engine(g, part, num_device_partitions=3, zerocopy_threshold=0, pack_partitions=1)
+    blocks = lt.Engine(g, part, num_device_partitions=3, zerocopy_threshold=0, pack_partitions=2)
     for l in (0, 1, 7):
         paths = eng.genericwalk_paths(4000, l)
         ref = oracle.genericwalk_paths(g.rowptr(), g.col(), 4000, l)
         assert np.array_equal(paths, ref)
         assert np.array_equal(packed.genericwalk_paths(4000, l), ref)
+        assert np.array_equal(blocks.genericwalk_paths(4000, l), ref)
     assert (ref == 0xFFFFFFFF).any()
     assert packed.stats()["packed_image_bytes"] > 0
+    assert blocks.stats()["packed_image_bytes"] > 0
 
 
 def _dangling_graph():
@@ -198,7 +210,9 @@ def test_dangling_vertices(kw):
                                 # the C4 autotuner's usual pick: packed copies, resident span, batches
                                 {"span": "resident", "zerocopy_threshold": 500, "zc_batch": 16,
                                  "pack_partitions": 1},
-                                {"pack_partitions": 1, "zerocopy_threshold": 0, "slice_chunks": 1}])
+                                {"pack_partitions": 1, "zerocopy_threshold": 0, "slice_chunks": 1},
+                                {"span": "resident", "zerocopy_threshold": 500, "zc_batch": 16,
+                                 "pack_partitions": 2}])
 def test_ppr_batch_bit_exact(g12, kw):
     log(f"ppr batch {kw}")
     part = lt.Partition.build(g12, 16 << 10)
@@ -221,11 +235,12 @@ def test_autotuner_picks_a_trial():
     src, cnt = g.ppr_queries(256, lo=100, hi=10000, seed=5)
     best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=48 << 20, sample_every=4, sizes_mb=(16,))
     # host scheduling: (span partition x 2 slices + span resident x 2 slices x 2 batch
-    # sizes) x packed or raw copies; device scheduling: 2 spans (whole queues, raw copies)
-    assert len(trials) == 2 * (2 + 4) + 2
+    # sizes) x raw, gap-coded or block copies; device scheduling: 2 spans (whole
+    # queues, raw copies)
+    assert len(trials) == 3 * (2 + 4) + 2
     assert best in trials and best["running_ms"] == min(t["running_ms"] for t in trials)
     assert {t["zc_batch"] for t in trials} == {1, 16}
-    assert {t["pack_partitions"] for t in trials} == {0, 1}
+    assert {t["pack_partitions"] for t in trials} == {0, 1, 2}
 
 
 def test_schedule_independence(g12):
@@ -327,8 +342,9 @@ def test_c5_full_size_sampled_walkers():
     """The C5 workload at full size: R-MAT s26 ef27, 429 partitions of 32 MB,
     an 8 GB cache (256 slots), pagerank w = 2|V|, l = 10.  The reference
     scheduler's run matches the oracle on sampled walker ids (endpoints) and
-    the visit sum; the learned scheduler's run, with raw and with packed
-    partition copies, gives the same visit counts (and endpoints)."""
+    the visit sum; the learned scheduler's run, with raw copies, gap-coded
+    copies and blocks kept packed in HBM, gives the same visit counts (and
+    endpoints)."""
     log("C5 full size")
     g = lt.Graph.rmat(26, 27)
     part = lt.Partition.build(g, 32 << 20)
@@ -351,13 +367,20 @@ def test_c5_full_size_sampled_walkers():
     assert np.array_equal(learned.pagerank(w, l), counts)
     assert learned.stats()["prefetch_issued"] > 0
     learned.close()
-    # the bench's headline configuration: packed partition copies
     packed = lt.Engine(g, part, num_device_partitions=256, span="resident", policy="learned", pack_partitions=1)
     packed.set_scorer(sc)
     pc, pe = packed.pagerank(w, l, endpoints=True)
     assert np.array_equal(pc, counts) and np.array_equal(pe, ends)
     st = packed.stats()
     assert st["packed_image_bytes"] > 0 and st["packed_col_bytes"] > 1.5 * st["packed_image_bytes"]
+    packed.close()
+    # the bench's headline configuration: blocks kept packed in the slots
+    blocks = lt.Engine(g, part, num_device_partitions=256, span="resident", policy="learned", pack_partitions=2)
+    blocks.set_scorer(sc)
+    bc, be = blocks.pagerank(w, l, endpoints=True)
+    assert np.array_equal(bc, counts) and np.array_equal(be, ends)
+    st = blocks.stats()
+    assert st["cache_slots"] > 256 and st["packed_image_bytes"] > 0
 
 
 def test_spill_moves_walkers_through_host_memory(g12):

@@ -186,6 +186,16 @@ def test_engine_without_device_fails_loudly():
     assert e.value.status == 7  # LT_E_NODEVICE
 
 
+def test_engine_rejects_unknown_pack_mode():
+    # checked before any device call, so on every host
+    g = lt.Graph.open(os.path.join(GOLDEN, "rmat_s8_ef4.bcsr"))
+    p = lt.Partition.build(g, 1 << 20)
+    for mode in (-1, 3):
+        with pytest.raises(lt.LTError) as e:
+            lt.Engine(g, p, pack_partitions=mode)
+        assert e.value.status == 6 and "pack_partitions" in str(e.value)  # LT_E_ARG
+
+
 @pytest.mark.skipif(not _no_device(), reason="a HIP device is visible")
 def test_bandwidth_probes_without_device():
     import ctypes

@@ -49,6 +49,25 @@ def decode(img, rowptr, eb, V):
     return out
 
 
+def decode_blocks(img, m):
+    """Column slice from a block image, following the header's text:
+    hdr u32[2*(nblk+1)] ({base, first bit} per 16-edge block, then {0, total
+    bits}), then the bit streams (LSB first) and two zero words."""
+    nblk = (m + 15) // 16
+    hdr = img[:8 * (nblk + 1)].view(np.uint32)
+    words = img[8 * (nblk + 1):].view(np.uint32)
+    assert len(words) >= 2 and not words[-2:].any()
+    bits = np.unpackbits(words.view(np.uint8), bitorder="little")
+    out = np.zeros(m, np.uint32)
+    for e in range(m):
+        k = e // 16
+        base, off = int(hdr[2 * k]), int(hdr[2 * k + 1])
+        w = (int(hdr[2 * k + 3]) - off) // 16
+        pos = off + (e % 16) * w
+        out[e] = base + sum(int(bits[pos + j]) << j for j in range(w))
+    return out
+
+
 def check_graph(rowptr, col, part_bytes):
     g = lt.Graph.from_csr(rowptr, col)
     part = lt.Partition.build(g, part_bytes)
@@ -58,6 +77,9 @@ def check_graph(rowptr, col, part_bytes):
         img = part.pack(g, p)
         rp = rowptr[no[p]:no[p + 1] + 1]
         got = decode(img, rp, eo[p], g.num_node)
+        assert np.array_equal(got, col[eo[p]:eo[p + 1]]), p
+        blk = part.pack(g, p, blocks=True)
+        got = decode_blocks(blk, int(eo[p + 1] - eo[p]))
         assert np.array_equal(got, col[eo[p]:eo[p + 1]]), p
         sizes.append((len(img), 4 * int(eo[p + 1] - eo[p])))
     return sizes
@@ -76,6 +98,32 @@ def test_hand_computed_image():
     # meta: chunk 0 starts at word 0, block codes (2, 0, 0, 0); words: [149, 0, 0]
     expect = np.array([0, 2, 149, 0, 0], np.uint32).view(np.uint8)
     assert np.array_equal(img, expect)
+
+
+def test_hand_computed_block_image():
+    # one row [5, 6, 8]: one block, base 5, values 0, 1, 3 (+ 13 padding zeros)
+    # in bitlen(8 - 5) = 2 bits each: 0 | 1 << 2 | 3 << 4 = 52, 32 bits in all
+    V = 16
+    rowptr = np.array([0, 3] + [3] * (V - 1), np.uint64)
+    col = np.array([5, 6, 8], np.uint32)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, 1 << 20)
+    img = part.pack(g, 0, blocks=True)
+    assert np.array_equal(img, np.array([5, 0, 0, 32, 52, 0, 0], np.uint32).view(np.uint8))
+
+
+def test_block_image_unsorted_and_zero_width():
+    # block 0 = [999, 0, 7 x 14]: base 0, width bitlen(999) = 10, unsorted is
+    # fine; block 1 = [3 x 16]: width 0, no bits at all
+    V = 1000
+    col = np.array([999, 0] + [7] * 14 + [3] * 16, np.uint32)
+    rowptr = np.array([0, 32] + [32] * (V - 1), np.uint64)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, 1 << 20)
+    img = part.pack(g, 0, blocks=True)
+    assert list(img[:24].view(np.uint32)) == [0, 0, 3, 160, 0, 160]
+    assert len(img) == 24 + 4 * (5 + 2)
+    assert np.array_equal(decode_blocks(img, 32), col)
 
 
 def test_hand_computed_raw_block_and_anchors():
