@@ -19,11 +19,12 @@ self loops and isolated vertices removed):
   c2  scale 22 ef 16, 32 MB partitions all HBM-resident, genericwalk w=2|V| l=10,
       walks continue through every partition (the reference's span is reported beside)
   c3  scale 25 ef 44, 4 GB partition cache, 2^26 walkers at once, ppr from vertex 0;
-      value: span resident, zero-copy batches of 16, packed copies (the
-      reference's rules and its figure-14 ablations are reported beside)
+      value: span resident, zero-copy batches of 16, partitions kept packed in
+      HBM (the reference's rules and its figure-14 ablations are reported beside)
   c4  same graph, 4,096 PPR queries (sources proportional to degree, walk counts
       log-uniform in [1e3, 1e6]), 8 GB cache; configuration chosen by the
-      autotuner (partition size 16/32/64 MB, span, host or device scheduling)
+      autotuner (partition size 16/32/64 MB, span, host or device scheduling,
+      zero-copy batches, raw / gap-coded / block copies)
 For c5 the learned scorer is trained here, on the GPU, from the traces of the
 plain scheduler's warm-up runs; `value` is the learned scheduler's steps/s with
 partitions kept packed in HBM (16-edge frame-of-reference blocks, copied as
@@ -50,7 +51,7 @@ WORKLOADS = {
     "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10, zc=0),
     "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10, span="resident"),
     "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26,
-               eng=dict(span="resident", zc_batch=16, pack_partitions=1)),
+               eng=dict(span="resident", zc_batch=16, pack_partitions=2)),
     "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
     "c5": dict(scale=26, ef=27, mb=32, app="pagerank", slots=256, l=10),
 }
@@ -414,8 +415,8 @@ def main():
                                 ("span_resident_zc_batch16", dict(span="resident", zc_batch=16), 2),
                                 ("packed", dict(pack_partitions=1), 2),
                                 ("blocks", dict(pack_partitions=2), 2),
-                                ("span_resident_zc_batch16_blocks", dict(span="resident", zc_batch=16,
-                                                                         pack_partitions=2), 2)):
+                                ("span_resident_zc_batch16_packed", dict(span="resident", zc_batch=16,
+                                                                         pack_partitions=1), 2)):
                 e2 = lt.Engine(g, part, **kw, **base)
                 v = measure(e2, wl, w, 1, k, f"c3/{name}")
                 variants[name] = {x: v[x] for x in keep}

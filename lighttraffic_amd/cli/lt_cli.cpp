@@ -36,7 +36,7 @@ int main(int argc, char **argv) {
                 "usage: %s -f <graph.bcsr> [-m [-s MB]] [-p slots] [-b walkers] [-w walkers] [-l length]\n"
                 "          [--runs n] [--source v] [--prob p] [--scheduler greedy|fifo|learned]\n"
                 "          [--no-pipeline] [--zerocopy-threshold n] [--span partition|resident]\n"
-                "          [--seed n] [--timers] [--json] [--queries file] [--pack] [--rowptr-resident]\n",
+                "          [--seed n] [--timers] [--json] [--queries file] [--pack | --pack-blocks] [--rowptr-resident]\n",
                 app.c_str());
         return 1;
     }
@@ -104,7 +104,7 @@ int main(int argc, char **argv) {
     if (c.prefetch_depth >= 0) cfg.prefetch_depth = c.prefetch_depth;
     cfg.zc_batch = c.zc_batch;
     cfg.device_schedule = c.device_schedule ? 1 : 0;
-    cfg.pack_partitions = c.pack ? 1 : 0;
+    cfg.pack_partitions = c.pack;
     cfg.rowptr_resident = c.rowptr_resident ? 1 : 0;
     lt_engine *e = nullptr;
     s = lt_engine_create(g, part, &cfg, &e);

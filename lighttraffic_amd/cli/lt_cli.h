@@ -16,7 +16,8 @@
 //   --scheduler greedy|fifo|learned, --scorer <weights file> (learned),
 //   --no-pipeline, --zerocopy-threshold <n>, --span partition|resident,
 //   --prefetch-depth <n>, --zc-batch <n>, --device-schedule, --pack,
-//   --rowptr-resident, --seed <n>, --timers, --json, --queries <file>
+//   --pack-blocks, --rowptr-resident, --seed <n>, --timers, --json,
+//   --queries <file>
 #pragma once
 #include <stdint.h>
 #include <stdio.h>
@@ -42,7 +43,7 @@ struct CliConfig {
     int prefetch_depth = -1;  // -1: library default
     int zc_batch = 0;         // 0: library default
     bool device_schedule = false;
-    bool pack = false;           // copy partitions packed (lt_config.pack_partitions)
+    int pack = 0;                // lt_config.pack_partitions: 1 --pack, 2 --pack-blocks
     bool rowptr_resident = false;
     bool pipeline = true;
     bool has_zc = false;
@@ -98,7 +99,8 @@ static bool cli_parse(int argc, char **argv, CliConfig &c) {
         else if (a == "--prefetch-depth") c.prefetch_depth = cli_num<int>(next("prefetch-depth"));
         else if (a == "--zc-batch") c.zc_batch = cli_num<int>(next("zc-batch"));
         else if (a == "--device-schedule") c.device_schedule = true;
-        else if (a == "--pack") c.pack = true;
+        else if (a == "--pack") c.pack = 1;
+        else if (a == "--pack-blocks") c.pack = 2;
         else if (a == "--rowptr-resident") c.rowptr_resident = true;
         else if (a == "--no-pipeline") c.pipeline = false;
         else if (a == "--zerocopy-threshold") { c.zc = cli_num<uint64_t>(next("zerocopy-threshold")); c.has_zc = true; }

@@ -59,6 +59,9 @@ def test_pagerank_cli_capped_cache(graph_file):
     out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, "--pack", "--zerocopy-threshold", 0)
     assert "result correct" in out
     assert grep_num(r"H2D bytes: explicit (\d+)", out) < grep_num(r"H2D bytes unpacked: (\d+)", out)
+    out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, "--pack-blocks", "--zerocopy-threshold", 0)
+    assert "result correct" in out
+    assert grep_num(r"H2D bytes: explicit (\d+)", out) < grep_num(r"H2D bytes unpacked: (\d+)", out)
     out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, "--zerocopy-threshold", 1 << 40)
     assert grep_num(r"graph loading time: [0-9.]+ ms, calls: (\d+)", out) == 0
     # -b below -w: walker queues spill to host memory and come back (the
@@ -97,5 +100,10 @@ def test_learned_scheduler_from_saved_scorer(graph_file, tmp_path):
     # the fastest capped-cache setup: the same with packed partition copies
     out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 60000, "-l", 10, "--scheduler", "learned",
               "--scorer", weights, "--span", "resident", "--pack", "--json")
+    assert "result correct" in out and "H2D bytes unpacked" in out
+    assert grep_num(r'"moves": (\d+)', out) == ref_moves
+    # and with partitions kept packed in HBM (the bench's headline setup)
+    out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 60000, "-l", 10, "--scheduler", "learned",
+              "--scorer", weights, "--span", "resident", "--pack-blocks", "--json")
     assert "result correct" in out and "H2D bytes unpacked" in out
     assert grep_num(r'"moves": (\d+)', out) == ref_moves
