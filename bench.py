@@ -383,8 +383,10 @@ def main():
                     "config": {"workload": desc, "scheduler": "greedy", "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, "c4", best["span"]),
                     "stats_per_step": m["per_job"],
-                    "autotune": {"seconds": tune_s, "sample": "every 16th query", "best": best,
-                                 "trials": [{k: t[k] for k in ("partition_mb", "span", "device_schedule",
+                    "autotune": {"seconds": tune_s, "sample": "every 16th query; the 3 fastest and the fastest of each copy mode "
+                                           "again on every 4th",
+                                 "best": best,
+                                 "trials": [{k: t[k] for k in ("stage", "partition_mb", "span", "device_schedule",
                                                                "slice_chunks", "zc_batch", "pack_partitions",
                                                                "cache_slots", "running_ms", "h2d_bytes")}
                                             for t in trials]}})
