@@ -208,6 +208,17 @@ lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t p
  * lt_partition_pack.  No counterpart in the reference. */
 lt_status lt_partition_pack_blocks(const lt_graph *g, const lt_partition *p, uint32_t part, void *out,
                                    uint64_t cap, uint64_t *bytes_out);
+/* HBM cache layout an engine without device_schedule gets for partition table
+ * p: *slot_bytes per slot and *slots slots for num_device_partitions raw slots
+ * (0: one per partition), before rowptr_resident takes its share.  With
+ * pack_partitions 0 or 1 a slot is S rounded up to 256 bytes (the reference's
+ * slot, src/GraphLoader.h:513-520); with 2 it is the largest rowptr slice +
+ * block image over the partitions (256-aligned) and the cache keeps the same
+ * bytes, so it holds more slots (at most one per partition).  Lets a caller
+ * size the cache before creating an engine; engines report the same count in
+ * lt_stats.cache_slots.  No counterpart in the reference. */
+lt_status lt_partition_slot_layout(const lt_graph *g, const lt_partition *p, int32_t pack_partitions,
+                                   uint32_t num_device_partitions, uint64_t *slot_bytes, uint32_t *slots);
 /* An engine with pack_partitions = 1 or 2 builds the packed images of its
  * partition table once (pinned host memory, about half the graph's column
  * bytes) and caches them on the graph for later engines.  This drops the

@@ -110,6 +110,8 @@ _sig("lt_partition_find", _u32, _vp, _u32)
 _sig("lt_partition_free", None, _vp)
 _sig("lt_partition_pack", ctypes.c_int, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64))
 _sig("lt_partition_pack_blocks", ctypes.c_int, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64))
+_sig("lt_partition_slot_layout", ctypes.c_int, _vp, _vp, ctypes.c_int32, _u32, ctypes.POINTER(_u64),
+     ctypes.POINTER(_u32))
 _sig("lt_graph_drop_packed", None, _vp)
 _sig("lt_engine_fetch_partition", ctypes.c_int, _vp, _u32, _vp, _vp)
 _sig("lt_engine_create", ctypes.c_int, _vp, _vp, ctypes.POINTER(Config), ctypes.POINTER(_vp))
@@ -323,6 +325,14 @@ class Partition:
         out = np.zeros(n.value, np.uint8)
         _check(fn(graph._h, self._h, part, out.ctypes.data, n.value, ctypes.byref(n)), name)
         return out
+
+    def slot_layout(self, graph, pack_partitions=0, num_device_partitions=0):
+        """(slot bytes, slots) of the HBM cache an engine with these settings
+        gets (before rowptr_resident takes its share)."""
+        sb, n = _u64(), _u32()
+        _check(_lib.lt_partition_slot_layout(graph._h, self._h, pack_partitions, num_device_partitions,
+                                             ctypes.byref(sb), ctypes.byref(n)), "lt_partition_slot_layout")
+        return sb.value, n.value
 
     def close(self):
         if self._h and _lib is not None:

@@ -455,18 +455,8 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     slot_packed = pk && pk->format == 2;
 
     // HBM cache slots (GPUGraph, src/GraphLoader.h:513-520)
-    nslots = cfg.num_device_partitions == 0 ? P : std::min(cfg.num_device_partitions, P);
-    const uint64_t raw_slot = (part.bytes + 255) & ~255ull;
-    uint64_t slot_bytes = raw_slot;
-    if (slot_packed) {
-        uint64_t need = 0;
-        for (uint32_t i = 0; i < P; i++) {
-            const uint64_t rb = ((uint64_t)part.node_off[i + 1] - part.node_off[i] + 1) * 8;
-            need = std::max(need, rb + (pk->size[i] ? pk->size[i] : 4 * (part.edge_off[i + 1] - part.edge_off[i])));
-        }
-        slot_bytes = (need + 255) & ~255ull;
-        nslots = (uint32_t)std::min<uint64_t>(P, (uint64_t)nslots * raw_slot / slot_bytes);
-    }
+    uint64_t slot_bytes = 0;
+    slot_layout(part, slot_packed ? pk->size.data() : nullptr, cfg.num_device_partitions, &slot_bytes, &nslots);
     slot_sz = slot_bytes;
     // The whole rowptr array in HBM, paid for with slots of the same cache:
     // a zero-copied move then reads only its column entry over PCIe.

@@ -94,6 +94,13 @@ uint64_t pack_encode(const Graph &g, uint32_t nb, uint32_t ne, uint32_t vbits, u
 // Bytes of the block image of rows [nb, ne), UINT64_MAX if its bit offsets
 // would exceed 32 bits; writes it when out is not null (zeroed, that size).
 uint64_t pack_blocks_encode(const Graph &g, uint32_t nb, uint32_t ne, uint8_t *out);
+// Slot bytes (256-aligned) and number of slots of an HBM partition cache of
+// `raw_slots` raw slots (0: one per partition) for partition table p.  sizes:
+// per partition, the bytes of the packed column image its slot keeps (0: raw
+// columns), or nullptr for raw slots; the cache keeps raw_slots raw slots'
+// bytes, and holds at most one slot per partition.
+void slot_layout(const Partition &p, const uint64_t *sizes, uint32_t raw_slots, uint64_t *slot_bytes,
+                 uint32_t *slots);
 // The graph's image for partition table p in `format` (1 or 2), built on
 // first call.
 lt_status packed_image(const Graph &g, const Partition &p, int format, std::shared_ptr<const PackedImage> *out);

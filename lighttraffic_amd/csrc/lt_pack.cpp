@@ -135,6 +135,25 @@ uint64_t pack_blocks_encode(const Graph &g, uint32_t nb, uint32_t ne, uint8_t *o
     return bytes;
 }
 
+void slot_layout(const Partition &p, const uint64_t *sizes, uint32_t raw_slots, uint64_t *slot_bytes,
+                 uint32_t *slots) {
+    const uint32_t P = p.P();
+    const uint64_t raw = (p.bytes + 255) & ~255ull;
+    uint32_t n = raw_slots == 0 ? P : std::min(raw_slots, P);
+    uint64_t sb = raw;
+    if (sizes) {
+        uint64_t need = 0;
+        for (uint32_t i = 0; i < P; i++) {
+            const uint64_t rb = ((uint64_t)p.node_off[i + 1] - p.node_off[i] + 1) * 8;
+            need = std::max(need, rb + (sizes[i] ? sizes[i] : 4 * (p.edge_off[i + 1] - p.edge_off[i])));
+        }
+        sb = std::max<uint64_t>(256, (need + 255) & ~255ull);
+        n = (uint32_t)std::min<uint64_t>(P, (uint64_t)n * raw / sb);
+    }
+    *slot_bytes = sb;
+    *slots = n;
+}
+
 PackedImage::~PackedImage() {
     if (!host) return;
     if (pinned) (void)hipHostFree(host);
@@ -249,6 +268,29 @@ static lt_status pack_one(const char *fn, int format, const lt_graph *g, const l
 lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t part, void *out, uint64_t cap,
                             uint64_t *bytes_out) {
     return pack_one("lt_partition_pack", 1, g, p, part, out, cap, bytes_out);
+}
+
+lt_status lt_partition_slot_layout(const lt_graph *g, const lt_partition *p, int32_t pack_partitions,
+                                   uint32_t num_device_partitions, uint64_t *slot_bytes, uint32_t *slots) {
+    if (!g || !p || !slot_bytes || !slots || pack_partitions < 0 || pack_partitions > 2 ||
+        p->node_off.back() > g->V) {
+        lt::set_error("lt_partition_slot_layout: null argument, pack_partitions not 0-2, or partition table "
+                      "not of this graph");
+        return LT_E_ARG;
+    }
+    std::vector<uint64_t> sizes;
+    bool any = false;
+    if (pack_partitions == 2) {  // the sizes an engine's block image gets (lt_pack.cpp packed_image)
+        const uint32_t P = p->P();
+        sizes.assign(P, 0);
+        lt::parallel_for(P, [&](uint32_t i) {
+            const uint64_t s = lt::pack_blocks_encode(*g, p->node_off[i], p->node_off[i + 1], nullptr);
+            sizes[i] = s < 4 * (p->edge_off[i + 1] - p->edge_off[i]) ? s : 0;
+        });
+        for (uint64_t s : sizes) any = any || s;
+    }
+    lt::slot_layout(*p, any ? sizes.data() : nullptr, num_device_partitions, slot_bytes, slots);
+    return LT_OK;
 }
 
 lt_status lt_partition_pack_blocks(const lt_graph *g, const lt_partition *p, uint32_t part, void *out,

@@ -147,6 +147,10 @@ def test_block_slots_hold_more_partitions():
     log(f"slots raw {sa['cache_slots']} blocks {sb['cache_slots']}; loads {sa['graph_loading_calls']} "
         f"{sb['graph_loading_calls']}; explicit bytes {sa['h2d_bytes_explicit']} {sb['h2d_bytes_explicit']}")
     assert sb["cache_slots"] > 1.3 * sa["cache_slots"]
+    # the host-side layout query gives the engines' slot counts
+    nd = kw["num_device_partitions"]
+    assert part.slot_layout(g, 0, nd)[1] == sa["cache_slots"] == nd
+    assert part.slot_layout(g, 2, nd)[1] == sb["cache_slots"]
     assert sb["graph_loading_calls"] < sa["graph_loading_calls"]
     assert 0 < sb["packed_image_bytes"] < 0.8 * sb["packed_col_bytes"]
     ends, moves = blk.genericwalk(w, 10, moves=True)

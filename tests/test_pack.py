@@ -210,3 +210,42 @@ def test_partitions_without_edges():
     for p in empty:
         assert np.array_equal(part.pack(g, p), np.zeros(8, np.uint8))
     check_graph(rowptr, col, 8192)
+
+
+def test_slot_layout_hand_computed():
+    # one row 0..63 in a 64-vertex graph: 4 blocks of width bitlen(15) = 4,
+    # image = 5 header pairs (40 B) + 8 stream words + 2 zero words (40 B) =
+    # 80 B < 256 B raw; slot = rowptr 65 * 8 + 80 = 600 -> 768 bytes
+    V = 64
+    rowptr = np.array([0, 64] + [64] * (V - 1), np.uint64)
+    col = np.arange(64, dtype=np.uint32)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, 1 << 20)
+    assert len(part.pack(g, 0, blocks=True)) == 80
+    assert part.slot_layout(g, 2) == (768, 1)
+    assert part.slot_layout(g, 0) == part.slot_layout(g, 1) == (1 << 20, 1)
+    # [5, 6, 8]: the block image (28 B) is not smaller than the columns (12 B),
+    # so no partition is kept packed and slots stay raw
+    rowptr = np.array([0, 3] + [3] * 15, np.uint64)
+    g = lt.Graph.from_csr(rowptr, np.array([5, 6, 8], np.uint32))
+    part = lt.Partition.build(g, 1 << 20)
+    assert part.slot_layout(g, 2) == (1 << 20, 1)
+
+
+def test_slot_layout_rmat():
+    rowptr, col = oracle.rmat_graph(11, 16, seed=11)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, 16 << 10)
+    no, eo = part.node_offsets(), part.edge_offsets()
+    need = 0
+    for p in range(part.count):
+        m = int(eo[p + 1] - eo[p])
+        b = len(part.pack(g, p, blocks=True))
+        need = max(need, 8 * int(no[p + 1] - no[p] + 1) + (b if b < 4 * m else 4 * m))
+    sb, n = part.slot_layout(g, 2, 4)
+    assert sb == (need + 255) // 256 * 256 and sb < 16 << 10
+    assert n == min(part.count, 4 * (16 << 10) // sb) and n > 4
+    assert part.slot_layout(g, 2, 0)[1] == part.count  # every partition has a slot either way
+    assert part.slot_layout(g, 1, 4) == (16 << 10, 4)
+    with pytest.raises(lt.LTError):
+        part.slot_layout(g, 3)
