@@ -665,10 +665,11 @@ lt_status Engine::load_async(uint32_t p) {
 }
 
 // Slot bytes of partition p: the rowptr slice, then the col slice, either
-// copied as is or copied packed into a staging buffer and unpacked from there.
+// copied as is, or copied gap-coded into a staging buffer and unpacked from
+// there, or (block slots) its block image copied into the slot, where it stays.
 // The DMAs run on s_dma; s_copy waits for them (and unpacks), so the load is
 // complete when s_copy reaches this point.  A staging buffer is refilled only
-// after the unpack of its previous image.
+// after the unpack of its previous image.  Block slots copy on s_copy alone.
 lt_status Engine::copy_partition(uint32_t p, uint8_t *base) {
     const uint32_t nb = part.node_off[p], ne = part.node_off[p + 1];
     const uint64_t eb = part.edge_off[p], ee = part.edge_off[p + 1];
