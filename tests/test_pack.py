@@ -249,3 +249,18 @@ def test_slot_layout_rmat():
     assert part.slot_layout(g, 1, 4) == (16 << 10, 4)
     with pytest.raises(lt.LTError):
         part.slot_layout(g, 3)
+
+
+def test_pack_blocks_arguments():
+    rowptr, col = oracle.rmat_graph(8, 4, seed=1)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, 2048)
+    with pytest.raises(lt.LTError):
+        part.pack(g, part.count, blocks=True)
+    L = lt.lib()
+    n = ctypes.c_uint64()
+    assert L.lt_partition_pack_blocks(g._h, part._h, 0, None, 0, ctypes.byref(n)) == 0 and n.value > 0
+    assert n.value == len(part.pack(g, 0, blocks=True))
+    buf = np.zeros(n.value - 1, np.uint8)
+    assert L.lt_partition_pack_blocks(g._h, part._h, 0, buf.ctypes.data, n.value - 1, ctypes.byref(n)) == 6
+    assert L.lt_partition_pack_blocks(None, part._h, 0, None, 0, ctypes.byref(n)) == 6
