@@ -220,8 +220,9 @@ lt_status lt_partition_pack_blocks(const lt_graph *g, const lt_partition *p, uin
 lt_status lt_partition_slot_layout(const lt_graph *g, const lt_partition *p, int32_t pack_partitions,
                                    uint32_t num_device_partitions, uint64_t *slot_bytes, uint32_t *slots);
 /* An engine with pack_partitions = 1 or 2 builds the packed images of its
- * partition table once (pinned host memory, about half the graph's column
- * bytes) and caches them on the graph for later engines.  This drops the
+ * partition table once (pinned host memory; on R-MAT about half the graph's
+ * column bytes for 1, 0.6 for 2) and caches them on the graph for later
+ * engines.  This drops the
  * cache; engines already using an image keep it until they are destroyed.
  * No counterpart in the reference. */
 void lt_graph_drop_packed(lt_graph *g);
