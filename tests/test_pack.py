@@ -264,3 +264,38 @@ def test_pack_blocks_arguments():
     buf = np.zeros(n.value - 1, np.uint8)
     assert L.lt_partition_pack_blocks(g._h, part._h, 0, buf.ctypes.data, n.value - 1, ctypes.byref(n)) == 6
     assert L.lt_partition_pack_blocks(None, part._h, 0, None, 0, ctypes.byref(n)) == 6
+
+
+def span_col_emulated(img, m, e):
+    """The walk kernel's decode of edge indices e (u64 array) from a block
+    image, in the kernel's 32-bit operations: the header pair of block e/16,
+    the next block's offset, two stream words joined by alignbit, a mask."""
+    nblk = (m + 15) // 16
+    hdr = img.view(np.uint32)
+    words = hdr[2 * (nblk + 1):]
+    k = (e >> np.uint64(4)).astype(np.int64)
+    hx, hy = hdr[2 * k].astype(np.uint64), hdr[2 * k + 1].astype(np.uint64)
+    w = ((hdr[2 * k + 3].astype(np.uint64) - hy) & np.uint64(0xFFFFFFFF)) >> np.uint64(4)
+    pos = (hy + (e & np.uint64(15)) * w) & np.uint64(0xFFFFFFFF)
+    lo = words[(pos >> np.uint64(5)).astype(np.int64)].astype(np.uint64)
+    hi = words[((pos >> np.uint64(5)) + np.uint64(1)).astype(np.int64)].astype(np.uint64)
+    bits = (((hi << np.uint64(32)) | lo) >> (pos & np.uint64(31))) & np.uint64(0xFFFFFFFF)
+    mask = np.where(w >= 32, np.uint64(0xFFFFFFFF), (np.uint64(1) << np.minimum(w, np.uint64(31))) - np.uint64(1))
+    return ((hx + (bits & mask)) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("scale,ef,part_bytes", [(12, 16, 64 << 10), (10, 8, 4096)])
+def test_kernel_block_decode_emulated(scale, ef, part_bytes):
+    # every edge of every partition, through the kernel's arithmetic; reads stay
+    # inside the image (the two zero words cover the last window)
+    rowptr, col = oracle.rmat_graph(scale, ef, seed=scale + 1)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build(g, part_bytes)
+    eo = part.edge_offsets()
+    for p in range(part.count):
+        m = int(eo[p + 1] - eo[p])
+        if m == 0:
+            continue
+        img = part.pack(g, p, blocks=True)
+        got = span_col_emulated(img, m, np.arange(m, dtype=np.uint64))
+        assert np.array_equal(got, col[eo[p]:eo[p + 1]]), p
