@@ -220,6 +220,14 @@ for kw in (dict(num_device_partitions=3, zerocopy_threshold=0),
         rp, c = eng.fetch_partition(p, no, eo)
         assert np.array_equal(c, g.col()[eo[p]:eo[p + 1]]), p
     eng.close()
+# block slots under the same queue setting (the bench's): no DMA stream, results exact
+eng = lt.Engine(g, part, pack_partitions=2, num_device_partitions=4, span="resident", policy="learned",
+                zerocopy_threshold=50)
+rc, re, _ = oracle.pagerank(g.rowptr(), g.col(), w, 10, endpoints=True)
+counts, ends = eng.pagerank(w, 10, endpoints=True)
+assert np.array_equal(counts, rc) and np.array_equal(ends, re)
+assert eng.stats()["packed_image_bytes"] > 0
+eng.close()
 print("child ok", part.count)
 """
 
