@@ -299,3 +299,13 @@ def test_kernel_block_decode_emulated(scale, ef, part_bytes):
         img = part.pack(g, p, blocks=True)
         got = span_col_emulated(img, m, np.arange(m, dtype=np.uint64))
         assert np.array_equal(got, col[eo[p]:eo[p + 1]]), p
+
+
+def test_slot_layout_rejects_foreign_partition_table():
+    big_rp, big_col = oracle.rmat_graph(10, 8, seed=2)
+    small_rp, small_col = oracle.rmat_graph(8, 4, seed=2)
+    big, small = lt.Graph.from_csr(big_rp, big_col), lt.Graph.from_csr(small_rp, small_col)
+    part = lt.Partition.build(big, 4096)
+    with pytest.raises(lt.LTError) as e:
+        part.slot_layout(small, 2)
+    assert e.value.status == 6  # LT_E_ARG
