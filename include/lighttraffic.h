@@ -176,7 +176,7 @@ uint32_t lt_partition_find(const lt_partition *p, uint32_t v);
 void lt_partition_free(lt_partition *p);
 /* Packed image of partition part's column slice, the form in which an engine
  * with pack_partitions = 1 copies it to HBM (replaces the raw col copy of
- * GPUGraph::loadAsync, src/GraphLoader.h:531-552; the rowptr slice is copied
+ * GPUGraph::loadAsync, src/GraphLoader.h:193-215; the rowptr slice is copied
  * as is).  For m edges in nch = ceil(m/64) chunks of 64 (partition-local edges
  * 64c .. 64c+63), each of four 16-edge blocks, with vbits = bits of V-1 (at
  * least 1):
@@ -195,7 +195,7 @@ lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t p
 /* Block image of partition part's column slice, the form in which an engine
  * with pack_partitions = 2 copies it to HBM and keeps it there, read in
  * place by the walk (replaces the raw col slice of the cache slot,
- * GPUGraph::loadAsync, src/GraphLoader.h:531-552).  For m edges in
+ * GPUGraph::loadAsync, src/GraphLoader.h:193-215).  For m edges in
  * nblk = ceil(m/16) blocks of 16 (partition-local edges 16k .. 16k+15; the
  * last block padded with its base):
  *   u32 hdr[2*(nblk+1)]: hdr[2k] = base of block k (its smallest column),
@@ -212,7 +212,7 @@ lt_status lt_partition_pack_blocks(const lt_graph *g, const lt_partition *p, uin
  * p: *slot_bytes per slot and *slots slots for num_device_partitions raw slots
  * (0: one per partition), before rowptr_resident takes its share.  With
  * pack_partitions 0 or 1 a slot is S rounded up to 256 bytes (the reference's
- * slot, src/GraphLoader.h:513-520); with 2 it is the largest rowptr slice +
+ * slot, src/GraphLoader.h:175-176); with 2 it is the largest rowptr slice +
  * block image over the partitions (256-aligned) and the cache keeps the same
  * bytes, so it holds more slots (at most one per partition).  Lets a caller
  * size the cache before creating an engine; engines report the same count in
@@ -359,7 +359,7 @@ lt_status lt_device_info(char *name, size_t cap, int *num_cu, uint64_t *hbm_byte
 lt_status lt_device_copy_bandwidth(uint64_t bytes, uint32_t reps, double *gbps_out);
 /* Host-to-HBM DMA of `bytes` from pinned host memory (hipMemcpyAsync), reps
  * times; gbps_out = bytes per second.  The achievable rate of the copies that
- * load partitions (GPUGraph::loadAsync, src/GraphLoader.h:531-552), to set
+ * load partitions (GPUGraph::loadAsync, src/GraphLoader.h:193-215), to set
  * beside the link's theoretical rate. */
 lt_status lt_device_h2d_bandwidth(uint64_t bytes, uint32_t reps, double *gbps_out);
 

@@ -365,22 +365,7 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     cfg = *c;
     V = g->V;
     P = part.P();
-    if (part.node_off[P] != g->V || part.edge_off[P] != g->E) {
-        set_error("partition covers %u vertices / %llu edges, graph has %u / %llu", part.node_off[P],
-                  (unsigned long long)part.edge_off[P], g->V, (unsigned long long)g->E);
-        return LT_E_FORMAT;
-    }
-    for (uint32_t i = 0; i < P; i++) {
-        if (part.edge_off[i] != g->rowptr[part.node_off[i]]) {
-            set_error("partition %u edge offset does not match the graph", i);
-            return LT_E_FORMAT;
-        }
-        if (partition_bytes(i) > part.bytes) {
-            set_error("partition %u needs %llu bytes > partition size %llu", i,
-                      (unsigned long long)partition_bytes(i), (unsigned long long)part.bytes);
-            return LT_E_FORMAT;
-        }
-    }
+    LT_TRY(check_partition_of(*g, part));
     if (P > kMaxPartitions) {
         set_error("%u partitions exceed the engine limit %u; use a larger partition size", P, kMaxPartitions);
         return LT_E_ARG;
@@ -454,7 +439,7 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     }
     slot_packed = pk && pk->format == 2;
 
-    // HBM cache slots (GPUGraph, src/GraphLoader.h:513-520)
+    // HBM cache slots (GPUGraph, src/GraphLoader.h:175-176)
     uint64_t slot_bytes = 0;
     slot_layout(part, slot_packed ? pk->size.data() : nullptr, cfg.num_device_partitions, &slot_bytes, &nslots);
     slot_sz = slot_bytes;
@@ -646,7 +631,7 @@ lt_status Engine::evict(uint32_t p) {
     return LT_OK;
 }
 
-// GPUGraph::loadAsync (src/GraphLoader.h:531-552): rowptr slice at slot offset
+// GPUGraph::loadAsync (src/GraphLoader.h:193-215): rowptr slice at slot offset
 // 0, col slice right after it, both on the copy stream.  The compute stream
 // must not be reading the slot (the caller evicted it after a sync).
 lt_status Engine::load_async(uint32_t p) {
@@ -1194,7 +1179,6 @@ lt_status Engine::run_partition(uint32_t p, int32_t &prev_loaded) {
             }
             LT_TRY(evict((uint32_t)victim));
             evicted = victim;
-            sched.on_evict((uint32_t)victim);
         }
         record(p, 3, (int32_t)p, evicted);
         LT_TRY(load_async(p));
@@ -1232,7 +1216,7 @@ lt_status Engine::scheduler_loop() {
     sched.begin_run();
     while (true) {
         LT_TRY(read_meta());
-        const int32_t c = sched.next(counts.data(), resident.data(), prev, sync);
+        const int32_t c = sched.next(counts.data(), prev, sync);
         if (c == -2) return LT_E_INTERNAL;
         if (c < 0) {
             if (sync) break;
@@ -1301,7 +1285,7 @@ lt_status Engine::scheduler_loop_learned() {
     while (true) {
         LT_TRY(read_meta());
         LT_TRY(retire(-1));
-        int32_t c = sched.next(counts.data(), resident.data(), prev, sync);
+        int32_t c = sched.next(counts.data(), prev, sync);
         if (c < 0) {
             if (sync) break;
             sync = true;

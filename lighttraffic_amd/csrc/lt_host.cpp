@@ -82,6 +82,45 @@ uint32_t Partition::find(uint32_t v) const {
     return low;
 }
 
+
+// Partition table p describes graph g: it covers [0, V) and [0, E) with
+// non-decreasing offsets, each edge offset is the rowptr of its first vertex,
+// and every partition fits the partition size.  The engine and the cache
+// sizing read slices of g through these offsets without further checks.
+lt_status check_partition_of(const Graph &g, const Partition &p) {
+    const size_t n = p.node_off.size();
+    if (n < 2 || p.edge_off.size() != n) {
+        set_error("partition table holds no partition");
+        return LT_E_FORMAT;
+    }
+    const uint32_t P = (uint32_t)n - 1;
+    if (p.node_off[0] != 0 || p.node_off[P] != g.V || p.edge_off[P] != g.E) {
+        set_error("partition covers %u vertices / %llu edges, graph has %u / %llu", p.node_off[P],
+                  (unsigned long long)p.edge_off[P], g.V, (unsigned long long)g.E);
+        return LT_E_FORMAT;
+    }
+    for (uint32_t i = 0; i < P; i++)
+        if (p.node_off[i + 1] < p.node_off[i]) {
+            set_error("partition offsets decrease at partition %u", i);
+            return LT_E_FORMAT;
+        }
+    for (uint32_t i = 0; i <= P; i++)
+        if (p.edge_off[i] != g.rowptr[p.node_off[i]]) {
+            set_error("partition %u edge offset does not match the graph", i);
+            return LT_E_FORMAT;
+        }
+    for (uint32_t i = 0; i < P; i++) {
+        const uint64_t need =
+            ((uint64_t)p.node_off[i + 1] - p.node_off[i] + 1) * 8 + (p.edge_off[i + 1] - p.edge_off[i]) * 4;
+        if (need > p.bytes) {
+            set_error("partition %u needs %llu bytes > partition size %llu", i, (unsigned long long)need,
+                      (unsigned long long)p.bytes);
+            return LT_E_FORMAT;
+        }
+    }
+    return LT_OK;
+}
+
 }  // namespace lt
 
 using namespace lt;
@@ -121,6 +160,28 @@ void lt_config_default(lt_config *c) {
 }
 
 // ---- bcsr (src/GraphLoader.h:30-48; tools/convert_from_txt.cc:211-218) ----
+// A CSR the walk kernels may read without bounds checks: rowptr starts at 0,
+// never decreases and ends at E (so every rowptr[v] <= E), and every column
+// is a vertex id below V.
+static lt_status check_csr(uint32_t V, uint64_t E, const uint64_t *rowptr, const uint32_t *col, const char *what) {
+    if (rowptr[0] != 0 || rowptr[V] != E) {
+        set_error("%s: rowptr[0]=%llu rowptr[V]=%llu E=%llu", what, (unsigned long long)rowptr[0],
+                  (unsigned long long)rowptr[V], (unsigned long long)E);
+        return LT_E_FORMAT;
+    }
+    for (uint32_t v = 0; v < V; v++)
+        if (rowptr[v + 1] < rowptr[v]) {
+            set_error("%s: rowptr decreases at %u", what, v);
+            return LT_E_FORMAT;
+        }
+    for (uint64_t e = 0; e < E; e++)
+        if (col[e] >= V) {
+            set_error("%s: col[%llu]=%u >= V", what, (unsigned long long)e, col[e]);
+            return LT_E_FORMAT;
+        }
+    return LT_OK;
+}
+
 // Layout: u32 num_node | u64 num_edge | u64 rowptr[V+1] | u32 col[E], packed.
 lt_status lt_graph_open(const char *path, lt_graph **out) {
     if (!path || !out) return LT_E_ARG;
@@ -157,36 +218,16 @@ lt_status lt_graph_open(const char *path, lt_graph **out) {
         return LT_E_IO;
     }
     fclose(f);
-    if (g->rowptr[0] != 0 || g->rowptr[V] != E) {
-        set_error("%s: rowptr[0]=%llu rowptr[V]=%llu E=%llu", path, (unsigned long long)g->rowptr[0],
-                  (unsigned long long)g->rowptr[V], (unsigned long long)E);
-        return LT_E_FORMAT;
-    }
-    for (uint32_t v = 0; v < V; v++)
-        if (g->rowptr[v + 1] < g->rowptr[v]) {
-            set_error("%s: rowptr decreases at %u", path, v);
-            return LT_E_FORMAT;
-        }
-    for (uint64_t e = 0; e < E; e++)
-        if (g->col[e] >= V) {
-            set_error("%s: col[%llu]=%u >= V", path, (unsigned long long)e, g->col[e]);
-            return LT_E_FORMAT;
-        }
+    st = check_csr(V, E, g->rowptr, g->col, path);
+    if (st != LT_OK) return st;
     *out = g.release();
     return LT_OK;
 }
 
 lt_status lt_graph_from_csr(uint32_t V, uint64_t E, const uint64_t *rowptr, const uint32_t *col, lt_graph **out) {
     if (!rowptr || (E && !col) || !out) return LT_E_ARG;
-    if (rowptr[0] != 0 || rowptr[V] != E) {
-        set_error("rowptr must start at 0 and end at E");
-        return LT_E_FORMAT;
-    }
-    for (uint64_t e = 0; e < E; e++)
-        if (col[e] >= V) {
-            set_error("col[%llu]=%u >= V", (unsigned long long)e, col[e]);
-            return LT_E_FORMAT;
-        }
+    lt_status cs = check_csr(V, E, rowptr, col, "csr");
+    if (cs != LT_OK) return cs;
     std::unique_ptr<lt_graph> g(new lt_graph);
     lt_status st = graph_alloc(g.get(), V, E);
     if (st != LT_OK) return st;

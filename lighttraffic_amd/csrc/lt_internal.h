@@ -42,6 +42,9 @@ struct Partition {
     uint32_t find(uint32_t v) const;
 };
 
+// Partition table p describes graph g (coverage, offsets, sizes); LT_E_FORMAT otherwise.
+lt_status check_partition_of(const Graph &g, const Partition &p);
+
 // Packed partition image: the column slice of a partition, gap-coded for the
 // host->HBM copy and unpacked in HBM by unpack_kernel.  Layout of one image
 // for a partition of m edges (partition-local edge index e), nch = ceil(m/64)

@@ -1,7 +1,7 @@
 // lt_pack.cpp — packed partition images for the host->HBM partition copy.
 //
 // The reference copies a partition as its raw rowptr and col slices
-// (GPUGraph::loadAsync, src/GraphLoader.h:531-552).  Here the col slice may
+// (GPUGraph::loadAsync, src/GraphLoader.h:193-215).  Here the col slice may
 // instead travel gap-coded (format in lt_internal.h) and be unpacked in HBM by
 // unpack_kernel into exactly the bytes a raw copy would have written, so the
 // walk reads the same slot either way.  Sorted adjacency lists (what the bcsr
@@ -272,12 +272,11 @@ lt_status lt_partition_pack(const lt_graph *g, const lt_partition *p, uint32_t p
 
 lt_status lt_partition_slot_layout(const lt_graph *g, const lt_partition *p, int32_t pack_partitions,
                                    uint32_t num_device_partitions, uint64_t *slot_bytes, uint32_t *slots) {
-    if (!g || !p || !slot_bytes || !slots || pack_partitions < 0 || pack_partitions > 2 ||
-        p->node_off.back() > g->V) {
-        lt::set_error("lt_partition_slot_layout: null argument, pack_partitions not 0-2, or partition table "
-                      "not of this graph");
+    if (!g || !p || !slot_bytes || !slots || pack_partitions < 0 || pack_partitions > 2) {
+        lt::set_error("lt_partition_slot_layout: null argument or pack_partitions not 0-2");
         return LT_E_ARG;
     }
+    if (lt::check_partition_of(*g, *p) != LT_OK) return LT_E_ARG;  // the message says what does not match
     std::vector<uint64_t> sizes;
     bool any = false;
     if (pack_partitions == 2) {  // the sizes an engine's block image gets (lt_pack.cpp packed_image)

@@ -2,12 +2,12 @@
 //
 // GREEDY and FIFO restate Scheduler::nextPartition and the eviction /
 // pipelined-compute choices of GPUScheduler (src/Scheduler.h:84-186,
-// 368-416).  LEARNED asks the scorer for a choice and falls back to GREEDY
-// when the scorer has none.
+// 368-416).  LEARNED takes GREEDY's choices here; the learned scheduler loop
+// (Engine::scheduler_loop_learned) adds its scorer-driven prefetch and victim
+// choice on top of them.
 #pragma once
 #include <stdint.h>
 
-#include <functional>
 #include <vector>
 
 #include "../../include/lighttraffic.h"
@@ -19,11 +19,7 @@ struct Scheduler {
     int policy = LT_SCHED_GREEDY;
     uint32_t fifo_cursor = 0;  // the reference keeps it in a function-local static (Scheduler.h:394)
     std::vector<int32_t> history;  // loaded partitions, most recent last
-    uint64_t learned_decisions = 0, fallback_decisions = 0;
-
-    // Learned scorer: returns a partition or -1 to fall back.
-    std::function<int32_t(const uint64_t *counts, const uint8_t *resident, int32_t prev, bool sync)> scorer;
-    std::function<int32_t(const uint64_t *counts, const uint8_t *resident, int32_t except)> evict_scorer;
+    uint64_t learned_decisions = 0, fallback_decisions = 0;  // victims chosen with / without scores
 
     void init(uint32_t P_, const lt_config &cfg) {
         P = P_;
@@ -34,18 +30,9 @@ struct Scheduler {
         learned_decisions = fallback_decisions = 0;
     }
     void on_load(uint32_t p) { history.push_back((int32_t)p); }
-    void on_evict(uint32_t) {}
 
-    int32_t next(const uint64_t *counts, const uint8_t *resident, int32_t prev, bool sync) {
+    int32_t next(const uint64_t *counts, int32_t prev, bool sync) {
         if (policy == LT_SCHED_FIFO) return fifo_next(counts);
-        if (policy == LT_SCHED_LEARNED && scorer) {
-            const int32_t c = scorer(counts, resident, prev, sync);
-            if (c >= 0 && (uint32_t)c < P && counts[c] > 0) {
-                learned_decisions++;
-                return c;
-            }
-            fallback_decisions++;
-        }
         return lt_sched_greedy_next(P, counts, prev, sync ? 1 : 0);
     }
 
@@ -74,10 +61,6 @@ struct Scheduler {
                 if (resident[v] && (int32_t)v != prev_loaded) return (int32_t)v;
             }
             return -1;
-        }
-        if (policy == LT_SCHED_LEARNED && evict_scorer) {
-            const int32_t v = evict_scorer(counts, resident, prev_loaded);
-            if (v >= 0 && (uint32_t)v < P && resident[v] && v != prev_loaded) return v;
         }
         return lt_sched_evict_min(P, counts, resident, prev_loaded);
     }

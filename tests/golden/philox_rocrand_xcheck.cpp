@@ -1,6 +1,8 @@
 // Writes Philox4x32-10 vectors computed by ROCm's open-source rocrand header
 // (rocrand_philox4x32_10.h, ten_rounds) for seeded pseudo-random counters and
-// keys.  Host-only; build and run with make_golden.sh.
+// keys.  Host code only; it writes philox_rocrand.json:
+//   hipcc -std=c++17 -O1 -o xcheck tests/golden/philox_rocrand_xcheck.cpp
+//   ./xcheck > tests/golden/philox_rocrand.json
 #include <cstdio>
 #include <cstdint>
 #include <rocrand/rocrand_philox4x32_10.h>

@@ -1,6 +1,6 @@
 """Packed partition copies on the GPU: unpack_kernel writes exactly the bytes
 of a raw copy (the rowptr and col slices of GPUGraph::loadAsync,
-src/GraphLoader.h:531-552) into an HBM slot, for sorted R-MAT graphs and for
+src/GraphLoader.h:193-215) into an HBM slot, for sorted R-MAT graphs and for
 rows stored absolute, and the engine's byte counters report what crossed
 PCIe.  Walk parity with packing on is covered by the packed-* cases of
 test_gpu_parity.py."""

@@ -83,6 +83,22 @@ def test_bcsr_errors(tmp_path):
     assert e.value.status == 2
 
 
+def test_from_csr_refuses_what_the_kernels_cannot_read():
+    """A rowptr that decreases, or whose interior offsets pass E, would make a
+    walk kernel read far outside the column slice: from_csr refuses it as
+    lt_graph_open does (LT_E_FORMAT)."""
+    col = np.zeros(8, np.uint32)
+    for rowptr in ([0, 5, 3, 8], [0, 100, 100, 8]):
+        with pytest.raises(lt.LTError) as e:
+            lt.Graph.from_csr(np.array(rowptr, np.uint64), col)
+        assert e.value.status == 2, rowptr
+    with pytest.raises(lt.LTError) as e:  # a column id >= V
+        lt.Graph.from_csr(np.array([0, 1, 2, 8], np.uint64), np.array([0, 1, 2, 3, 0, 0, 0, 0], np.uint32))
+    assert e.value.status == 2
+    g = lt.Graph.from_csr(np.array([0, 3, 3, 8], np.uint64), col)  # an empty row is fine
+    assert g.num_node == 3 and g.num_edge == 8
+
+
 def _random_csr(rng, V, maxdeg):
     deg = rng.integers(0, maxdeg, V)
     rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)

@@ -1,7 +1,7 @@
 """Packed partition images (lt_partition_pack) on the host, no GPU needed.
 
 The reference copies a partition's rowptr and col slices as they are
-(GPUGraph::loadAsync, src/GraphLoader.h:531-552); an engine with
+(GPUGraph::loadAsync, src/GraphLoader.h:193-215); an engine with
 pack_partitions=1 copies the col slice in the packed form that
 include/lighttraffic.h specifies and unpacks it in HBM.  Here the format is
 checked on its own terms:
@@ -309,3 +309,15 @@ def test_slot_layout_rejects_foreign_partition_table():
     with pytest.raises(lt.LTError) as e:
         part.slot_layout(small, 2)
     assert e.value.status == 6  # LT_E_ARG
+    # the other direction: a table built for the smaller graph, asked about the larger one
+    part_small = lt.Partition.build(small, 4096)
+    with pytest.raises(lt.LTError) as e:
+        part_small.slot_layout(big, 2)
+    assert e.value.status == 6
+    # same vertex count, other edges: offsets no longer match the rowptr
+    other_rp, other_col = oracle.rmat_graph(10, 8, seed=3)
+    other = lt.Graph.from_csr(other_rp, other_col)
+    if other.num_node == big.num_node:
+        with pytest.raises(lt.LTError):
+            part.slot_layout(other, 0)
+    assert part.slot_layout(big, 0)[1] == part.count
