@@ -113,18 +113,21 @@ def measure(eng, wl, w, warmup, steps, label, timed_engine=None):
            "per_job": {k: v / len(stats) for k, v in tot.items()}, "nominal": nominal}
     if timed_engine is not None:
         tst = run_job(timed_engine, wl, w)
-        res.update({"kernel_ms": tst["walk_kernel_ms"], "launches": tst["kernel_launches"],
-                    "alg_bytes": algorithmic_bytes(tst), "timed_job_ms": tst["running_ms"]})
+        res.update(_timed(tst))
     return res
+
+
+def _timed(tst):
+    return {"kernel_ms": tst["walk_kernel_ms"], "launches": tst["kernel_launches"],
+            "alg_bytes": algorithmic_bytes(tst), "timed_job_ms": tst["running_ms"],
+            "zc_launches": tst["zerocopy_launches"], "zc_kernel_ms": tst["walk_kernel_zc_ms"]}
 
 
 def timed_walk(eng, wl, w):
     """Walk-kernel time of one job for the roofline: a warm-up job, then one
     job on an engine with HIP event timers around every walk launch."""
     run_job(eng, wl, w)
-    tst = run_job(eng, wl, w)
-    return {"kernel_ms": tst["walk_kernel_ms"], "launches": tst["kernel_launches"],
-            "alg_bytes": algorithmic_bytes(tst), "timed_job_ms": tst["running_ms"]}
+    return _timed(run_job(eng, wl, w))
 
 
 def measured_traffic(workload, span, alg_bytes_per_launch):
@@ -143,11 +146,47 @@ def measured_traffic(workload, span, alg_bytes_per_launch):
 def roofline(m, workload=None, span=None):
     achieved = m["alg_bytes"] / (m["kernel_ms"] * 1e-3) / 1e9 if m["kernel_ms"] > 0 else 0.0
     per_launch = m["alg_bytes"] / max(m["launches"], 1)
+    traffic = measured_traffic(workload, span, per_launch)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, span, per_launch),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": None if traffic is None else
+            (f"derived, not measured in this run: the FETCH_SIZE + WRITE_SIZE / algorithmic-byte ratio of "
+             f"profiles/walk_kernel_{workload}_{span}_pmc.json times this run's algorithmic bytes per launch"),
             "kernel": "walk_kernel",
             "launches": m["launches"], "avg_launch_us": m["kernel_ms"] * 1e3 / max(m["launches"], 1),
-            "algorithmic_bytes_per_launch": m["alg_bytes"] / max(m["launches"], 1)}
+            "algorithmic_bytes_per_launch": m["alg_bytes"] / max(m["launches"], 1),
+            "zerocopy_launches": m.get("zc_launches"), "zerocopy_kernel_ms": m.get("zc_kernel_ms")}
+
+
+def roofline_pcie(m, peak_gbps):
+    """Capped-cache workloads are bound by the host link: H2D bytes (explicit
+    partition copies + zero-copied reads + spilled walkers) per second of the
+    timed region against the measured pinned copy rate, and the steps/s that
+    rate allows at this run's H2D bytes per step."""
+    achieved = m["pcie_GBps"]
+    return {"bound": "pcie", "achieved": achieved, "peak": peak_gbps, "unit": "GB/s", "frac": achieved / peak_gbps,
+            "peak_kind": "measured pinned hipMemcpyAsync rate", "theoretical": 64.0,
+            "h2d_bytes_per_step": m["h2d_bytes_per_step"],
+            "bound_steps_per_s": peak_gbps * 1e9 / m["h2d_bytes_per_step"] if m["h2d_bytes_per_step"] > 0 else None}
+
+
+def belady_bound(eng, part):
+    """The engine's last job replayed with Belady's optimal eviction: the
+    partitions it walked from HBM (trace records of kind 0, 2 and 3), in
+    order, through the same number of cache slots, starting from the job's
+    first residency.  Its loads are the fewest any victim rule needs for this
+    processing order; the H2D floor scales the job's explicit bytes by them."""
+    tr = eng.trace()
+    _, resident, _ = eng.trace_state()
+    st = eng.stats()
+    seq = [r[0] for r in tr if r[1] in (0, 2, 3)]
+    loads = st["graph_loading_calls"]
+    best, _, _ = lt.sched_belady(seq, part.count, st["cache_slots"], init_resident=resident[0])
+    explicit = st["h2d_bytes_explicit"] * (best / loads if loads else 1.0)
+    return {"demands": len(seq), "loads": loads, "loads_belady": best,
+            "load_saving_bound": 1.0 - best / loads if loads else 0.0,
+            "h2d_bytes_per_step": (st["h2d_bytes_explicit"] + st["h2d_bytes_zerocopy"]) / st["nominal_steps"],
+            "h2d_bytes_per_step_floor": (explicit + st["h2d_bytes_zerocopy"]) / st["nominal_steps"]}
 
 
 def cpu_baseline(g, wl, w, budget_s=20.0):
@@ -277,12 +316,20 @@ def main():
                 eng.close()
         # plain rules with all of the learned configuration's non-learned settings:
         # byte break-even zero-copy threshold, zero-copy batches, packed copies
+        belady = {}
         for mode, tag in ((1, "_packed"), (2, "_blocks")):
             eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
-                            pack_partitions=mode, **base)
+                            pack_partitions=mode, record_trace=1, **base)
             plain["tuned" + tag] = measure(eng, wl, w, args.warmup, args.steps,
                                            f"plain/resident S/128 zc-batch 16{tag}")
+            if mode == 2:
+                belady["plain_same_settings"] = belady_bound(eng, part)
             eng.close()
+        # the learned loop without its network: prefetch the largest queues
+        # that need host data, evict by the plain rule (what the network adds)
+        eng = lt.Engine(g, part, span="resident", policy="learned", prefetch_plain=1, pack_partitions=2, **base)
+        plain["prefetch_rule_blocks"] = measure(eng, wl, w, args.warmup, args.steps, "plain-rule prefetch blocks")
+        eng.close()
         t1 = time.perf_counter()
         loss = sc.fit(epochs=args.epochs, batch=32, lr=1e-3)
         train_s = time.perf_counter() - t1 + prep_s
@@ -295,9 +342,10 @@ def main():
         eng.set_scorer(sc)
         learned_gap = measure(eng, wl, w, args.warmup, args.steps, "learned/resident packed")
         eng.close()
-        eng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=2, **base)
+        eng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=2, record_trace=1, **base)
         eng.set_scorer(sc)
         learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident blocks")
+        belady["learned"] = belady_bound(eng, part)
         same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
         slots = eng.stats()["cache_slots"]
         eng.close()
@@ -307,8 +355,6 @@ def main():
         teng.close()
         ref_plain = plain["partition"]
         same_copy = {k: v for k, v in plain.items() if k.endswith("blocks")}
-        best_name = min(same_copy, key=lambda k: same_copy[k]["ms_per_step"])
-        best_plain = same_copy[best_name]
 
         def vs(m):
             return {"speedup": learned["steps_per_s"] / m["steps_per_s"],
@@ -332,14 +378,20 @@ def main():
                       "span_resident_packed": {k: plain["resident_packed"][k] for k in keep},
                       "span_resident_blocks": {k: plain["resident_blocks"][k] for k in keep},
                       "span_resident_S128_zcbatch16_packed": {k: plain["tuned_packed"][k] for k in keep},
-                      "span_resident_S128_zcbatch16_blocks": {k: plain["tuned_blocks"][k] for k in keep}},
+                      "span_resident_S128_zcbatch16_blocks": {k: plain["tuned_blocks"][k] for k in keep},
+                      "plain_rule_prefetch_blocks": {k: plain["prefetch_rule_blocks"][k] for k in keep}},
             "learned": {"blocks": {k: learned[k] for k in keep}, "packed": {k: learned_gap[k] for k in keep},
                         "unpacked": {k: learned_raw[k] for k in keep}},
             "learned_vs_plain": {
-                "speedup_vs_reference_plain": learned["steps_per_s"] / ref_plain["steps_per_s"],
-                "h2d_bytes_ratio_vs_reference_plain": learned["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"],
+                # the headline pair: the plain rules with every non-learned setting of the
+                # learned configuration (span, zero-copy threshold and batches, block copies)
+                "like_for_like": dict(vs(plain["tuned_blocks"]), plain="span_resident_S128_zcbatch16_blocks",
+                                      target={"speedup": 1.3, "h2d_bytes_ratio": 0.75}),
+                "vs_plain_rule_prefetch": dict(vs(plain["prefetch_rule_blocks"]), note=(
+                    "the same prefetching loop with the largest queue as target and the plain victim rule: "
+                    "what the network itself adds")),
+                "vs_reference_plain": vs(ref_plain),
                 "vs_plain_same_copies": {k: vs(v) for k, v in same_copy.items()},
-                "best_plain_same_copies": best_name,
                 "unpacked_learned_vs_reference_plain": {
                     "speedup": learned_raw["steps_per_s"] / ref_plain["steps_per_s"],
                     "h2d_bytes_ratio": learned_raw["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"]},
@@ -347,11 +399,13 @@ def main():
                     k: {"speedup": learned_gap["steps_per_s"] / plain[k]["steps_per_s"],
                         "h2d_bytes_ratio": learned_gap["h2d_bytes_per_step"] / plain[k]["h2d_bytes_per_step"]}
                     for k in plain if k.endswith("packed")},
+                "belady": belady,
                 "visit_counts_identical": same,
                 "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
                 "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],
                 "scorer_training_s": train_s, "training_decisions": n_next + n_res},
             "roofline": roofline(learned, "c5", "resident"),
+            "roofline_pcie": roofline_pcie(learned, h2d_peak),
             "stats_per_step": learned["per_job"],
         })
     elif args.workload == "c4":
@@ -382,6 +436,7 @@ def main():
                     "median_ms_per_step": m["median_ms_per_step"], "e2e_ms_per_step": m["e2e_ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, "c4", best["span"]),
+                    "roofline_pcie": roofline_pcie(m, h2d_peak),
                     "stats_per_step": m["per_job"],
                     "autotune": {"seconds": tune_s, "sample": "every 16th query; the 3 fastest and the fastest of each copy mode "
                                            "again on every 4th",
@@ -429,6 +484,7 @@ def main():
                     "config": {"workload": desc, "scheduler": "greedy",
                                **{k: v for k, v in ekw.items()}, "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, args.workload, span),
+                    **({"roofline_pcie": roofline_pcie(m, h2d_peak)} if wl["slots"] else {}),
                     "stats_per_step": m["per_job"]})
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, wl, w)

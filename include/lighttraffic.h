@@ -96,6 +96,11 @@ typedef struct {
                                        partition that does not get smaller is
                                        copied as is (default 0: off;
                                        device_schedule: unused)                 */
+    int32_t  prefetch_plain;        /* LEARNED: 1 = run the learned loop without
+                                       its scorer: prefetch the largest queues
+                                       that need host data and evict by the
+                                       plain rule (an ablation that shows what
+                                       the network adds; default 0)            */
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines
@@ -129,6 +134,9 @@ typedef struct {
                                        h2d_bytes_explicit)                      */
     uint64_t packed_col_bytes;      /* column bytes those images unpacked to    */
     uint64_t cache_slots;           /* partitions the HBM cache holds at once   */
+    uint64_t zerocopy_launches;     /* walk launches whose source queues were
+                                       read over PCIe (part of kernel_launches) */
+    double   walk_kernel_zc_ms;     /* their summed time (part of walk_kernel_ms) */
     uint64_t reserved[1];
 } lt_stats;
 
@@ -306,9 +314,13 @@ void lt_scorer_free(lt_scorer *s);
 uint64_t lt_scorer_num_params(void);
 /* Add the decisions of e's last traced run (record_trace=1) as training
  * samples: with use_next, the label of output 0 is the partition the traced
- * scheduler loaded or read over PCIe at its next decision; with use_keep, the
- * label of output 1 is each partition's share of the walkers the trace
- * processes after the decision (times P).  Returns the samples added. */
+ * scheduler loaded or read over PCIe at its next decision.  With use_keep =
+ * 1 the label of output 1 is each partition's share of the walkers the trace
+ * processes after the decision (times P); with use_keep = 2 it is Belady's
+ * next-use quantity: with d the number of later records of kind 0, 2 or 3
+ * up to and including the partition's next one, 1 - ln(1+d)/ln(1+len), and
+ * 0 if it is never needed in HBM again (the eviction of the lowest score is
+ * then Belady's choice; cf. lt_sched_belady).  Returns the samples added. */
 lt_status lt_scorer_add_trace(lt_scorer *s, const lt_engine *e, int32_t use_next, int32_t use_keep,
                               uint64_t *added);
 /* Adam on the accumulated samples; loss_out[2] = last epoch's mean losses. */
@@ -350,6 +362,21 @@ int32_t lt_sched_evict_min(uint32_t P, const uint64_t *counts, const uint8_t *re
  * fewest walkers, else the resident one with the most (>0) walkers. */
 int32_t lt_sched_computed_partition(uint32_t P, const uint64_t *counts, const uint8_t *resident,
                                     int32_t except, uint64_t page);
+
+/* Belady's MIN on a recorded demand sequence (no counterpart in the
+ * reference; the bound for the eviction decision of
+ * GPUScheduler::findPartitionwithleastWalker, src/Scheduler.h:149-186): seq[0..n) are
+ * the partitions that had to be resident, in order (for an engine trace, the
+ * records of kind 0 and 3).  A cache of `slots` slots, holding
+ * init_resident's partitions at first (u8[P], NULL = empty; at most `slots`
+ * of them, lowest ids first), loads a demanded partition that is absent and,
+ * when full, evicts the resident partition whose next demand lies farthest
+ * ahead.  *loads_out = its loads, the fewest any eviction rule needs on this
+ * sequence; *bytes_out (may be NULL) = the sum of part_bytes (u64[P], may be
+ * NULL) over them; victims_out (i32[n], may be NULL) = the partition evicted
+ * at each demand, or -1. */
+lt_status lt_sched_belady(const int32_t *seq, uint64_t n, uint32_t P, uint32_t slots, const uint8_t *init_resident,
+                          const uint64_t *part_bytes, uint64_t *loads_out, uint64_t *bytes_out, int32_t *victims_out);
 
 /* ---- device probe -------------------------------------------------------- */
 lt_status lt_device_info(char *name, size_t cap, int *num_cu, uint64_t *hbm_bytes);

@@ -51,7 +51,7 @@ class Config(ctypes.Structure):
                 ("seed", _u64), ("policy", _i32), ("pipeline", _i32), ("span", _i32), ("event_timers", _i32),
                 ("record_trace", _i32), ("prefetch_depth", _i32), ("prefetch_stick_pct", _i32),
                 ("device_schedule", _i32), ("slice_chunks", _i32), ("zc_batch", _i32),
-                ("rowptr_resident", _i32), ("pack_partitions", _i32)]
+                ("rowptr_resident", _i32), ("pack_partitions", _i32), ("prefetch_plain", _i32)]
 
 
 class Stats(ctypes.Structure):
@@ -67,8 +67,8 @@ class Stats(ctypes.Structure):
                 ("nominal_steps", _u64), ("kernel_launches", _u64), ("walk_kernel_ms", ctypes.c_double),
                 ("learned_decisions", _u64), ("fallback_decisions", _u64), ("prefetch_issued", _u64),
                 ("prefetch_hits", _u64), ("h2d_bytes_unpacked", _u64), ("packed_image_bytes", _u64),
-                ("packed_col_bytes", _u64), ("cache_slots", _u64),
-                ("reserved", _u64 * 1)]
+                ("packed_col_bytes", _u64), ("cache_slots", _u64), ("zerocopy_launches", _u64),
+                ("walk_kernel_zc_ms", ctypes.c_double), ("reserved", _u64 * 1)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
@@ -143,6 +143,8 @@ _sig("lt_scorer_load", ctypes.c_int, _vp, ctypes.c_char_p)
 _sig("lt_sched_greedy_next", _i32, _u32, _vp, _i32, _i32)
 _sig("lt_sched_evict_min", _i32, _u32, _vp, _vp, _i32)
 _sig("lt_sched_computed_partition", _i32, _u32, _vp, _vp, _i32, _u64)
+_sig("lt_sched_belady", ctypes.c_int, _vp, _u64, _u32, _u32, _vp, _vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64),
+     _vp)
 _sig("lt_device_info", ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
      ctypes.POINTER(_u64))
 _sig("lt_device_copy_bandwidth", ctypes.c_int, _u64, _u32, ctypes.POINTER(ctypes.c_double))
@@ -449,6 +451,18 @@ def sched_computed_partition(counts, resident, except_, page):
     c = np.ascontiguousarray(counts, dtype=np.uint64)
     r = np.ascontiguousarray(resident, dtype=np.uint8)
     return _lib.lt_sched_computed_partition(len(c), c.ctypes.data, r.ctypes.data, except_, page)
+
+
+def sched_belady(seq, P, slots, init_resident=None, part_bytes=None):
+    """Belady's MIN on demand sequence `seq`: (loads, bytes, victims[i] or -1)."""
+    q = np.ascontiguousarray(seq, dtype=np.int32)
+    r = None if init_resident is None else np.ascontiguousarray(init_resident, dtype=np.uint8)
+    b = None if part_bytes is None else np.ascontiguousarray(part_bytes, dtype=np.uint64)
+    victims = np.zeros(max(len(q), 1), np.int32)
+    loads, nbytes = _u64(), _u64()
+    _check(_lib.lt_sched_belady(q.ctypes.data, len(q), P, slots, _ptr(r), _ptr(b), ctypes.byref(loads),
+                                ctypes.byref(nbytes), victims.ctypes.data), "lt_sched_belady")
+    return loads.value, nbytes.value, victims[:len(q)]
 
 
 class Scorer:

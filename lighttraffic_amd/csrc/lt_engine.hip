@@ -153,7 +153,7 @@ struct Engine {
     hipStream_t s_compute = nullptr, s_copy = nullptr;
     int num_cu = 0;
     int max_grid = 0;           // init kernel (4 walkers per thread)
-    int walk_grid[5] = {0, 0, 0, 0, 0};  // walk kernel, by walkers per thread
+    int walk_grid = 0;          // walk kernel: workgroups resident at once
 
     // partition table
     uint32_t *d_node_off = nullptr;
@@ -257,7 +257,7 @@ struct Engine {
     std::vector<float> scores;            // scorer output of the current decision
     bool scores_valid = false;
     uint64_t w_run = 0;                   // walkers of the current wave
-    EventTimer t_load, t_compute, t_zc, t_wload, t_wevict, t_kernel;
+    EventTimer t_load, t_compute, t_zc, t_wload, t_wevict, t_kernel, t_kzc;  // t_kzc: walk launches over PCIe
     Scheduler sched;
 
     ~Engine();
@@ -307,6 +307,7 @@ struct Engine {
     AppArgs app{};
     int app_kind = APP_PAGERANK;
     size_t lds_bytes(int wpt = kWpt) const { return append_lds_bytes(P, wpt); }
+    size_t walk_lds_bytes() const { return append_lds_bytes(P, 2); }  // stage of 2 * kBlock walkers
     lt_status launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s);
     uint64_t partition_bytes(uint32_t p) const {
         return ((uint64_t)part.node_off[p + 1] - part.node_off[p] + 1) * 8 +
@@ -397,12 +398,8 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     int occ = 0;
     LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, init_kernel<APP_PAGERANK>, kBlock, lds_bytes()));
     max_grid = std::max(1, occ) * num_cu;
-    const void *wk[5] = {nullptr, (const void *)walk_kernel<APP_PAGERANK, 1>, (const void *)walk_kernel<APP_PAGERANK, 2>,
-                         nullptr, (const void *)walk_kernel<APP_PAGERANK, 4>};
-    for (int wpt : {1, 2, 4}) {
-        LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wk[wpt], kBlock, lds_bytes(wpt)));
-        walk_grid[wpt] = std::max(1, occ) * num_cu;
-    }
+    LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, walk_kernel<APP_PAGERANK>, kBlock, walk_lds_bytes()));
+    walk_grid = std::max(1, occ) * num_cu;
 
     // partition table + vertex -> partition lookup table
     LT_TRY(dmalloc(&d_node_off, P + 1));
@@ -519,7 +516,7 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
 
     bool timers = cfg.event_timers != 0;
     t_load.enabled = t_compute.enabled = t_zc.enabled = t_wload.enabled = t_wevict.enabled = timers;
-    t_kernel.enabled = timers;
+    t_kernel.enabled = t_kzc.enabled = timers;
     sched.init(P, cfg);
     return LT_OK;
 }
@@ -746,27 +743,19 @@ lt_status Engine::fetch_partition(uint32_t p, uint64_t *rowptr_out, uint32_t *co
     return LT_OK;
 }
 
-// Walk launch sized to the slice: 1 walker per thread while the slice is
-// small enough that more walkers per thread would leave SIMDs without waves
-// (a thread walks its walkers one after another), 2 or 4 for large slices,
-// where the per-workgroup append cost (histogram, scan, one atomic per
-// destination queue) is spread over more walkers.
+// Walk launch: one workgroup per kBlock walkers of the slice, at most as many
+// as are resident at once (each then streams its share through its lanes).
 template <int APP>
-static void launch_walk_app(const WalkArgs &a, int wpt, int grid, size_t lds, hipStream_t s) {
-    if (wpt == 4) walk_kernel<APP, 4><<<grid, kBlock, lds, s>>>(a);
-    else if (wpt == 2) walk_kernel<APP, 2><<<grid, kBlock, lds, s>>>(a);
-    else walk_kernel<APP, 1><<<grid, kBlock, lds, s>>>(a);
+static void launch_walk_app(const WalkArgs &a, int grid, size_t lds, hipStream_t s) {
+    walk_kernel<APP><<<grid, kBlock, lds, s>>>(a);
 }
 
 lt_status Engine::launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s) {
-    const uint64_t fill = (uint64_t)num_cu * 4 * 6 * 64;  // threads for ~6 waves per SIMD
-    const int wpt = walkers >= 4 * fill ? 4 : walkers >= 2 * fill ? 2 : 1;
-    const uint64_t bw = (uint64_t)kBlock * wpt;
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + bw - 1) / bw, (uint64_t)walk_grid[wpt]));
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + kBlock - 1) / kBlock, (uint64_t)walk_grid));
     switch (app_kind) {
-        case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, wpt, grid, lds_bytes(wpt), s); break;
-        case APP_PPR: launch_walk_app<APP_PPR>(a, wpt, grid, lds_bytes(wpt), s); break;
-        default: launch_walk_app<APP_GENERIC>(a, wpt, grid, lds_bytes(wpt), s); break;
+        case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, grid, walk_lds_bytes(), s); break;
+        case APP_PPR: launch_walk_app<APP_PPR>(a, grid, walk_lds_bytes(), s); break;
+        default: launch_walk_app<APP_GENERIC>(a, grid, walk_lds_bytes(), s); break;
     }
     LT_HIP(hipGetLastError());
     meta_dirty = true;
@@ -815,9 +804,11 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
         a.chunk_lo = (uint32_t)lo;
         a.chunk_hi = last ? 0xFFFFFFFFu : (uint32_t)hi;
         const uint64_t sw = std::min<uint64_t>(walkers, hi * kChunk) - lo * kChunk;
-        hipEvent_t tk = t_kernel.begin(s_compute);
+        EventTimer &kt = zerocopy ? t_kzc : t_kernel;
+        hipEvent_t tk = kt.begin(s_compute);
         LT_TRY(launch_walk(a, sw, s_compute));
-        t_kernel.end(tk, s_compute);
+        kt.end(tk, s_compute);
+        st.zerocopy_launches += zerocopy ? 1 : 0;
         fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, p, (uint32_t)lo, last ? 0xFFFFFFFFu : (uint32_t)hi, last ? 1 : 0,
                                                   nullptr);
         LT_HIP(hipGetLastError());
@@ -1087,9 +1078,10 @@ lt_status Engine::process_zc_batch(const std::vector<uint32_t> &parts) {
     hipEvent_t tp = t_zc.begin(s_compute);
     set_zc_tables_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, d_zc_part, b, 1);
     LT_HIP(hipGetLastError());
-    hipEvent_t tk = t_kernel.begin(s_compute);
+    hipEvent_t tk = t_kzc.begin(s_compute);
     LT_TRY(launch_walk(a, total, s_compute));
-    t_kernel.end(tk, s_compute);
+    t_kzc.end(tk, s_compute);
+    st.zerocopy_launches++;
     for (uint32_t p : parts) fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, p, 0, 0xFFFFFFFFu, 1, nullptr);
     LT_HIP(hipGetLastError());
     meta_dirty = false;
@@ -1250,6 +1242,10 @@ lt_status Engine::scheduler_loop_learned() {
     std::vector<uint8_t> prefetched(P, 0);  // resident because of a prefetch, not yet picked
     std::vector<uint8_t> pending(P, 0);     // being copied by a prefetch
     const uint32_t depth = (uint32_t)std::max(0, cfg.prefetch_depth);
+    // Plain-rule prefetch: the same loop without the scorer, prefetching the
+    // largest non-resident queues and evicting by the plain rule (an ablation
+    // that isolates what the network adds).
+    const bool plain_prefetch = cfg.prefetch_plain != 0;
     scores_valid = false;
     sched.begin_run();
     // make completed prefetches resident, in order; with `upto`, wait for it
@@ -1345,7 +1341,9 @@ lt_status Engine::scheduler_loop_learned() {
         // freed before the walk is launched (the walk then never reads them),
         // with the scores of the previous decision; the targets are chosen
         // with the scores of this decision, computed while the walk runs.
-        size_t want = scorer && scorer_trained(scorer) && inflight.size() < depth ? depth - inflight.size() : 0;
+        size_t want = (plain_prefetch || (scorer && scorer_trained(scorer))) && inflight.size() < depth
+                          ? depth - inflight.size()
+                          : 0;
         if (want) {  // no more slots than there are partitions a prefetch may take
             size_t cand = 0;
             for (uint32_t i = 0; i < P && cand < want; i++)
@@ -1353,9 +1351,9 @@ lt_status Engine::scheduler_loop_learned() {
             want = cand;
         }
         if (want) {
-            if (!scores_valid) LT_TRY(infer_scores(c));
+            if (!scores_valid && !plain_prefetch) LT_TRY(infer_scores(c));
             bool evicted_any = false;
-            while (scores_valid && free_slots.size() < want) {
+            while ((scores_valid || plain_prefetch) && free_slots.size() < want) {
                 const int32_t v = learned_victim(c, -1);
                 if (v < 0) break;
                 LT_TRY(evict((uint32_t)v));
@@ -1372,13 +1370,15 @@ lt_status Engine::scheduler_loop_learned() {
         st.computing_calls++;
         LT_TRY(zc ? process_zc((uint32_t)c, pending.data()) : process((uint32_t)c, false, counts[c]));
         if (want && !free_slots.empty()) {
-            LT_TRY(infer_scores(c));  // while c is walked
-            while (scores_valid && inflight.size() < depth && !free_slots.empty()) {
+            if (!plain_prefetch) LT_TRY(infer_scores(c));  // while c is walked
+            while ((scores_valid || plain_prefetch) && inflight.size() < depth && !free_slots.empty()) {
                 int32_t q = -1;
                 float best = 0.f;
                 for (uint32_t i = 0; i < P; i++) {
                     if (resident[i] || pending[i] || (int32_t)i == c || counts[i] < zc_threshold) continue;
-                    const float v = scores[(size_t)i * kOut];
+                    // the scorer's guess of the next partition needing host data, or
+                    // (plain rule) the largest queue among those partitions
+                    const float v = plain_prefetch ? (float)counts[i] : scores[(size_t)i * kOut];
                     if (q < 0 || v > best) {
                         best = v;
                         q = (int32_t)i;
@@ -1465,9 +1465,9 @@ lt_status Engine::scheduler_loop_device() {
         decide_kernel<<<1, 1024, 0, s_compute>>>(q, a.ss, a.pt);
         load_kernel<<<load_grid, 256, 0, s_compute>>>(a.ss, a.pt);
         switch (app_kind) {
-            case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
-            case APP_PPR: launch_walk_app<APP_PPR>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
-            default: launch_walk_app<APP_GENERIC>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
+            case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, walk_grid, walk_lds_bytes(), s_compute); break;
+            case APP_PPR: launch_walk_app<APP_PPR>(a, walk_grid, walk_lds_bytes(), s_compute); break;
+            default: launch_walk_app<APP_GENERIC>(a, walk_grid, walk_lds_bytes(), s_compute); break;
         }
         fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, d_ds);
     }
@@ -1533,6 +1533,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     t_wload.reset();
     t_wevict.reset();
     t_kernel.reset();
+    t_kzc.reset();
     const uint64_t cap = cfg.num_device_walkers ? std::min<uint64_t>(cfg.num_device_walkers, rs.total) : rs.total;
     // More walkers than the device holds: spill to host queues (the
     // reference's -b); the device-scheduled loop runs them in waves instead.
@@ -1621,7 +1622,8 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
         const auto t0 = std::chrono::steady_clock::now();
         w_run = id1 - id0;
         if (cfg.device_schedule) LT_TRY(scheduler_loop_device());
-        else LT_TRY(cfg.policy == LT_SCHED_LEARNED && scorer ? scheduler_loop_learned() : scheduler_loop());
+        else LT_TRY(cfg.policy == LT_SCHED_LEARNED && (scorer || cfg.prefetch_plain) ? scheduler_loop_learned()
+                                                                                   : scheduler_loop());
         const auto t1 = std::chrono::steady_clock::now();
         running_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
@@ -1661,6 +1663,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     t_compute.collect();
     t_zc.collect();
     t_kernel.collect();
+    t_kzc.collect();
     st.graph_loading_ms = t_load.ms;
     st.computing_ms = t_compute.ms;
     st.zerocopy_ms = t_zc.ms;
@@ -1668,7 +1671,8 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     t_wevict.collect();
     st.walker_loading_ms = t_wload.ms;
     st.walker_evicting_ms = t_wevict.ms;
-    st.walk_kernel_ms = t_kernel.ms;
+    st.walk_kernel_ms = t_kernel.ms + t_kzc.ms;
+    st.walk_kernel_zc_ms = t_kzc.ms;
     st.learned_decisions = sched.learned_decisions;
     st.fallback_decisions = sched.fallback_decisions;
     return LT_OK;
@@ -1865,8 +1869,12 @@ lt_status lt_engine_trace_state(const lt_engine *e, uint32_t *counts_out, uint8_
 // Training samples from a traced run.  Decisions are the records of kind 0,
 // 1 and 3 (kind 2 = pipelined work inside a load).  Output 0's label is the
 // next decision's partition when that decision needed host data (kind 1 or
-// 3); output 1's label is each partition's share of the walkers processed by
-// all later records, times P.
+// 3).  Output 1's label, with use_keep = 1, is each partition's share of the
+// walkers processed by all later records, times P; with use_keep = 2 it is
+// Belady's quantity, how soon the partition is next needed in HBM: with d
+// the number of later records of kind 0, 2 or 3 up to and including its next
+// one, 1 - ln(1 + d) / ln(1 + n) (0 when it is never needed again), so that
+// evicting the lowest score evicts the partition needed farthest ahead.
 lt_status lt_scorer_add_trace(lt_scorer *s_, const lt_engine *e, int32_t use_next, int32_t use_keep,
                               uint64_t *added) {
     if (!s_ || !e) return LT_E_ARG;
@@ -1882,19 +1890,38 @@ lt_status lt_scorer_add_trace(lt_scorer *s_, const lt_engine *e, int32_t use_nex
         set_error("the engine holds no trace (set record_trace)");
         return LT_E_ARG;
     }
-    // suffix traffic per partition, built from the back
+    if (use_keep < 0 || use_keep > 2) {
+        set_error("use_keep must be 0, 1 or 2 (got %d)", use_keep);
+        return LT_E_ARG;
+    }
+    // suffix traffic per partition, and the position of each partition's next
+    // HBM demand, built from the back
     std::vector<double> future(P, 0.0);
+    std::vector<uint64_t> next_demand(P, UINT64_MAX);
     std::vector<std::vector<float>> keep_at(n);
     double total = 0;
+    uint64_t demands_after = 0;  // records of kind 0, 2, 3 after position j
     int32_t next_need = -1;
     std::vector<int32_t> next_of(n, -1);
+    const double ln_n = log1p((double)n);
     for (size_t j = n; j-- > 0;) {
         const auto &r = tr[j];
         if (r.kind != 2) {
             next_of[j] = next_need;
             keep_at[j].resize(P);
-            for (uint32_t q = 0; q < P; q++) keep_at[j][q] = total > 0 ? (float)(future[q] / total * P) : 0.f;
+            for (uint32_t q = 0; q < P; q++) {
+                if (use_keep == 2) {
+                    const double d = next_demand[q] == UINT64_MAX ? -1.0 : (double)(demands_after - next_demand[q] + 1);
+                    keep_at[j][q] = d < 0 ? 0.f : (float)(1.0 - log1p(d) / ln_n);
+                } else {
+                    keep_at[j][q] = total > 0 ? (float)(future[q] / total * P) : 0.f;
+                }
+            }
             next_need = (r.kind == 1 || r.kind == 3) ? r.partition : -1;
+        }
+        if (r.kind != 1) {  // needed in HBM at this record; numbered from the back
+            demands_after++;
+            next_demand[r.partition] = demands_after;
         }
         future[r.partition] += (double)r.walkers;
         total += (double)r.walkers;

@@ -10,7 +10,10 @@
 #include <math.h>
 
 #include <algorithm>
+#include <functional>
 #include <memory>
+#include <set>
+#include <vector>
 
 #include "lt_internal.h"
 #include "lt_rng.h"
@@ -427,6 +430,71 @@ int32_t lt_sched_computed_partition(uint32_t P, const uint64_t *counts, const ui
         }
     }
     return full >= 0 ? full : partial;
+}
+
+// Belady's MIN (no counterpart in the reference): replay a demand sequence
+// through a cache of `slots` slots, evicting on a miss the resident
+// partition whose next demand is farthest ahead (never demanded again counts
+// as infinitely far; ties go to the lowest id).  For unit-cost loads no
+// eviction rule loads less on the same sequence, so the result bounds what
+// any victim choice of the scheduler could save on a recorded trace.
+lt_status lt_sched_belady(const int32_t *seq, uint64_t n, uint32_t P, uint32_t slots, const uint8_t *init_resident,
+                          const uint64_t *part_bytes, uint64_t *loads_out, uint64_t *bytes_out, int32_t *victims_out) {
+    if ((!seq && n) || P == 0 || slots == 0 || !loads_out) {
+        set_error("lt_sched_belady: null argument, no partition or no slot");
+        return LT_E_ARG;
+    }
+    for (uint64_t i = 0; i < n; i++)
+        if (seq[i] < 0 || (uint32_t)seq[i] >= P) {
+            set_error("lt_sched_belady: demand %llu names partition %d of %u", (unsigned long long)i, seq[i], P);
+            return LT_E_ARG;
+        }
+    const uint64_t never = UINT64_MAX;
+    std::vector<uint64_t> next_use(n), upcoming(P, never);  // next demand after position i; of each partition
+    for (uint64_t i = n; i-- > 0;) {
+        next_use[i] = upcoming[seq[i]];
+        upcoming[seq[i]] = i;
+    }
+    // resident partitions ordered by their next demand, farthest first
+    std::set<std::pair<uint64_t, int32_t>, std::greater<std::pair<uint64_t, int32_t>>> by_next;
+    std::vector<uint64_t> key(P, 0);
+    std::vector<uint8_t> in(P, 0);
+    uint32_t used = 0;
+    if (init_resident) {
+        for (uint32_t p = 0; p < P && used < slots; p++)
+            if (init_resident[p]) {
+                in[p] = 1;
+                key[p] = upcoming[p];
+                by_next.insert({key[p], -(int32_t)p});  // negated id: the lowest id wins ties
+                used++;
+            }
+    }
+    uint64_t loads = 0, bytes = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        const int32_t p = seq[i];
+        int32_t victim = -1;
+        if (in[p]) {
+            by_next.erase({key[p], -p});
+        } else {
+            loads++;
+            if (part_bytes) bytes += part_bytes[p];
+            if (used == slots) {
+                auto it = by_next.begin();
+                victim = -it->second;
+                by_next.erase(it);
+                in[victim] = 0;
+            } else {
+                used++;
+            }
+            in[p] = 1;
+        }
+        key[p] = next_use[i];
+        by_next.insert({key[p], -p});
+        if (victims_out) victims_out[i] = victim;
+    }
+    *loads_out = loads;
+    if (bytes_out) *bytes_out = bytes;
+    return LT_OK;
 }
 
 lt_status lt_make_ppr_queries(const lt_graph *g, uint32_t nq, uint64_t lo, uint64_t hi, uint64_t seed,

@@ -1,0 +1,104 @@
+"""GPU parity at the measured workloads' full sizes.
+
+The smaller parity cases (test_gpu_parity.py) cover every cache, span and
+policy setting on graphs the oracle walks in milliseconds.  These cases run
+the benchmark configurations themselves and compare with the oracle on the
+same graph, seed and walker ids:
+
+  C2  R-MAT s22 ef16, 32 MB partitions all in HBM, genericwalk w = 2|V|,
+      l = 10 (app/genericwalk.cu:22-32): every endpoint and move count;
+  C3  R-MAT s25 ef44, 4 GB cache (128 slots of 32 MB), 2^26 device walkers,
+      ppr from vertex 0, w = 2|V|, p = 0.15 (app/ppr.cu:22-42): every visit
+      count and endpoint, for the reference's rules and for the benchmark's
+      configuration (span resident, zero-copy batches, block slots);
+  C4  the same graph, 4,096 batched PPR queries, 8 GB cache: sampled queries
+      (each walked by the oracle over its own walker ids) and the endpoint
+      total, host- and device-scheduled;
+  C5  R-MAT s26 ef27, 8 GB cache, pagerank w = 2|V|, l = 10
+      (app/pagerank.cu:26-44): the full u32 visit counts, in
+      test_gpu_parity.py.
+"""
+import numpy as np
+import pytest
+
+import lighttraffic_amd as lt
+from oracle import ffi as oracle
+
+pytestmark = pytest.mark.gpu
+THREADS = 16
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def test_c2_genericwalk_all_resident():
+    log("C2: R-MAT s22 ef16 genericwalk, all partitions resident")
+    g = lt.Graph.rmat(22, 16)
+    part = lt.Partition.build(g, 32 << 20)
+    w, l = 2 * g.num_node, 10
+    re, rm, _ = oracle.genericwalk(g.rowptr(), g.col(), w, l, threads=THREADS)
+    for span in ("resident", "partition"):
+        eng = lt.Engine(g, part, span=span)
+        ends, moves = eng.genericwalk(w, l, endpoints=True, moves=True)
+        st = eng.stats()
+        eng.close()
+        log(f"  span {span}: {st['running_ms']:.1f} ms, {st['kernel_launches']} launches")
+        assert np.array_equal(ends, re), span
+        assert np.array_equal(moves, rm), span
+        assert int(moves.sum()) == st["moves"]
+
+
+@pytest.fixture(scope="module")
+def s25():
+    log("R-MAT s25 ef44")
+    g = lt.Graph.rmat(25, 44)
+    yield g
+    g.close()
+
+
+@pytest.mark.parametrize("label,kw", [
+    ("reference rules", {}),
+    ("bench value: span resident, zc batches, block slots", dict(span="resident", zc_batch=16, pack_partitions=2)),
+])
+def test_c3_ppr_capped_cache(s25, label, kw):
+    log(f"C3: ppr from vertex 0, 4 GB cache, 2^26 device walkers, {label}")
+    g = s25
+    part = lt.Partition.build(g, 32 << 20)
+    w = 2 * g.num_node
+    eng = lt.Engine(g, part, num_device_partitions=128, num_device_walkers=1 << 26, **kw)
+    counts, ends = eng.ppr(w, source=0, prob=0.15, endpoints=True)
+    st = eng.stats()
+    eng.close()
+    log(f"  {st['running_ms']:.1f} ms, loads {st['graph_loading_calls']}, zero-copy {st['zerocopy_calls']}")
+    assert part.count > 128 and st["graph_loading_calls"] > 0
+    assert int(counts.sum()) == w  # one endpoint per walker (app/ppr.cu:82)
+    rc, re, _ = oracle.ppr(g.rowptr(), g.col(), w, source=0, prob=0.15, endpoints=True, threads=THREADS)
+    assert np.array_equal(ends, re)
+    assert np.array_equal(counts, rc)
+
+
+@pytest.mark.parametrize("label,mb,kw", [
+    ("64 MB, span resident, zc batches, block slots", 64, dict(span="resident", zc_batch=16, pack_partitions=2)),
+    ("32 MB, device-scheduled", 32, dict(device_schedule=1, span="resident")),
+])
+def test_c4_batched_ppr(s25, label, mb, kw):
+    log(f"C4: 4,096 batched ppr queries, 8 GB cache, {label}")
+    g = s25
+    src, cnt = g.ppr_queries(4096)
+    assert len(src) == 4096 and 1000 <= int(cnt.min()) and int(cnt.max()) < 1000000
+    part = lt.Partition.build(g, mb << 20)
+    eng = lt.Engine(g, part, num_device_partitions=(8 << 30) // (mb << 20), **kw)
+    ends = eng.ppr_batch(src, cnt, prob=0.15, endpoints=True)
+    st = eng.stats()
+    eng.close()
+    log(f"  {st['running_ms']:.1f} ms, {int(cnt.sum())} walkers, loads {st['graph_loading_calls']}")
+    assert len(ends) == int(cnt.sum()) and int(ends.max()) < g.num_node
+    off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    rng = np.random.default_rng(4096)
+    sample = sorted(set([0, 4095] + [int(q) for q in rng.integers(0, 4096, 30)]))
+    for q in sample:  # query q's walkers are ids [off[q], off[q+1]), all starting at src[q]
+        _, re, _ = oracle.ppr(g.rowptr(), g.col(), int(off[-1]), source=int(src[q]), prob=0.15,
+                              id_begin=int(off[q]), id_end=int(off[q + 1]), counts=False, endpoints=True,
+                              threads=THREADS)
+        assert np.array_equal(ends[off[q]:off[q + 1]], re), q

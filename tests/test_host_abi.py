@@ -262,3 +262,67 @@ def test_ppr_query_table():
     exp = nq * deg / deg.sum()
     big = exp > 20
     assert np.all(np.abs(hits[big] - exp[big]) < 6 * np.sqrt(exp[big]))
+
+
+def _min_loads_brute(seq, slots, start=()):
+    """Fewest loads over every eviction choice (exhaustive search with memo)."""
+    from functools import lru_cache
+
+    @lru_cache(maxsize=None)
+    def go(i, cache):
+        if i == len(seq):
+            return 0
+        p = seq[i]
+        if p in cache:
+            return go(i + 1, cache)
+        if len(cache) < slots:
+            return 1 + go(i + 1, frozenset(cache | {p}))
+        return 1 + min(go(i + 1, frozenset((cache - {v}) | {p})) for v in cache)
+
+    return go(0, frozenset(start))
+
+
+def _lru_loads(seq, slots):
+    cache, loads = [], 0
+    for p in seq:
+        if p in cache:
+            cache.remove(p)
+        else:
+            loads += 1
+            if len(cache) == slots:
+                cache.pop(0)
+        cache.append(p)
+    return loads
+
+
+def test_belady_hand_computed():
+    # 3 slots: 0 1 2 load; 3 evicts 2 (next use farthest); 2 evicts 0 (0 and 1
+    # never used again, lowest id first); 3 hits
+    loads, nbytes, victims = lt.sched_belady([0, 1, 2, 0, 1, 3, 0, 1, 2, 3], 4, 3, part_bytes=[10, 20, 30, 40])
+    assert loads == 5 and nbytes == 10 + 20 + 30 + 40 + 30
+    assert list(victims) == [-1, -1, -1, -1, -1, 2, -1, -1, 0, -1]
+    # a warm start: partitions 0 and 2 resident, 2 slots
+    loads, _, victims = lt.sched_belady([2, 1, 0, 1], 3, 2, init_resident=[1, 0, 1])
+    assert loads == 1 and list(victims) == [-1, 2, -1, -1]
+    with pytest.raises(lt.LTError) as e:
+        lt.sched_belady([0, 5], 4, 2)
+    assert e.value.status == 6
+
+
+def test_belady_is_the_minimum():
+    """Belady's loads equal the exhaustive minimum on small random sequences
+    and never exceed LRU's."""
+    rng = np.random.default_rng(7)
+    for trial in range(300):
+        P = int(rng.integers(2, 6))
+        slots = int(rng.integers(1, P))
+        seq = [int(x) for x in rng.integers(0, P, int(rng.integers(1, 13)))]
+        start = tuple(sorted(set(int(x) for x in rng.integers(0, P, int(rng.integers(0, slots + 1))))))
+        init = np.zeros(P, np.uint8)
+        init[list(start)] = 1
+        loads, _, victims = lt.sched_belady(seq, P, slots, init_resident=init)
+        assert loads == _min_loads_brute(seq, slots, start), (seq, slots, start)
+        if not start:
+            assert loads <= _lru_loads(seq, slots)
+        assert loads == sum(1 for i, p in enumerate(seq) if victims[i] >= 0) + min(
+            slots - len(start), len(set(seq) - set(start)))  # every load past the free slots evicts
