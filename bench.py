@@ -176,6 +176,7 @@ def belady_bound(eng, part):
     order, through the same number of cache slots, starting from the job's
     first residency.  Its loads are the fewest any victim rule needs for this
     processing order; the H2D floor scales the job's explicit bytes by them."""
+    import lighttraffic_amd as lt
     tr = eng.trace()
     _, resident, _ = eng.trace_state()
     st = eng.stats()

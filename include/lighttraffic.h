@@ -95,7 +95,7 @@ typedef struct {
                                        cache holds more partitions.  A
                                        partition that does not get smaller is
                                        copied as is (default 0: off;
-                                       device_schedule: unused)                 */
+                                       device_schedule: 2 only, 1 is ignored)   */
     int32_t  prefetch_plain;        /* LEARNED: 1 = run the learned loop without
                                        its scorer: prefetch the largest queues
                                        that need host data and evict by the
@@ -281,6 +281,39 @@ lt_status lt_engine_fetch_partition(lt_engine *e, uint32_t part, uint64_t *rowpt
  * walk count is floor(lo * (hi/lo)^(x2/2^32)) (log-uniform in [lo, hi)). */
 lt_status lt_make_ppr_queries(const lt_graph *g, uint32_t nq, uint64_t lo, uint64_t hi, uint64_t seed,
                               uint32_t *sources_out, uint64_t *counts_out);
+
+/* Autotuner of batched PPR (no counterpart in the reference, whose knobs are
+ * compile-time, src/Type.h:5-22, and which runs one source per program,
+ * src/walk/SingleSource.h:6-20).  Times engine configurations on every
+ * sample_every-th query of the batch (a warm second job each): partition
+ * size sizes_mb[0..nsizes) (NULL: 16, 32 and 64 MB) with cache_bytes of
+ * cache, span partition or resident,
+ * host or device scheduling, 2048 or 8192 chunks per walk launch, zero-copy
+ * batches of 1 or 16 (host scheduling, span resident), raw / gap-coded /
+ * block copies (gap-coded needs host scheduling).  Then it times the three
+ * fastest and the fastest of each copy mode again on a sample four times
+ * larger, and returns the fastest of those in *best_out.  Every trial (both
+ * stages) is written to trials_out (up to trials_cap; *ntrials_out = their
+ * number).  Results of the batch do not depend on the configuration. */
+typedef struct {
+    uint32_t stage;            /* 1: timed on the sample, 2: again on the larger one */
+    uint32_t partition_mb;     /* partition size S in MB                           */
+    uint32_t slots;            /* num_device_partitions = cache_bytes / S (<= P)   */
+    uint32_t partitions;       /* P at this size                                   */
+    uint32_t cache_slots;      /* partitions the cache held (lt_stats.cache_slots) */
+    int32_t  span;             /* the lt_config fields of the configuration        */
+    int32_t  device_schedule;
+    int32_t  slice_chunks;
+    int32_t  zc_batch;
+    int32_t  pack_partitions;
+    double   running_ms;       /* the warm job's scheduler-loop time               */
+    uint64_t h2d_bytes;        /* its explicit + zero-copied H2D bytes             */
+    uint64_t iterations;       /* its scheduler iterations                         */
+} lt_tune_trial;
+lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uint32_t *sources, const uint64_t *counts,
+                            float prob, uint64_t cache_bytes, const uint32_t *sizes_mb, uint32_t nsizes,
+                            uint32_t sample_every, lt_tune_trial *best_out, lt_tune_trial *trials_out,
+                            uint32_t trials_cap, uint32_t *ntrials_out);
 
 /* Scheduler decision trace of the last run (record_trace=1): one record per
  * decision.  kind: 0 partition already resident, 1 zero-copy, 2 pipelined

@@ -36,7 +36,8 @@ int main(int argc, char **argv) {
                 "usage: %s -f <graph.bcsr> [-m [-s MB]] [-p slots] [-b walkers] [-w walkers] [-l length]\n"
                 "          [--runs n] [--source v] [--prob p] [--scheduler greedy|fifo|learned]\n"
                 "          [--no-pipeline] [--zerocopy-threshold n] [--span partition|resident]\n"
-                "          [--seed n] [--timers] [--json] [--queries file] [--pack | --pack-blocks] [--rowptr-resident]\n",
+                "          [--seed n] [--timers] [--json] [--queries file [--tune]] [--pack | --pack-blocks]\n"
+                "          [--rowptr-resident]\n",
                 app.c_str());
         return 1;
     }
@@ -106,13 +107,29 @@ int main(int argc, char **argv) {
     cfg.device_schedule = c.device_schedule ? 1 : 0;
     cfg.pack_partitions = c.pack;
     cfg.rowptr_resident = c.rowptr_resident ? 1 : 0;
+    if (c.tune && !qsrc.empty()) {
+        // the same cache bytes as -p slots of this .part's size, the rest chosen by the autotuner
+        lt_tune_trial best;
+        const uint64_t cache = std::max<uint64_t>(1, pdev) * lt_partition_bytes(part);
+        s = lt_tune_ppr_batch(g, (uint32_t)qsrc.size(), qsrc.data(), qcnt.data(), c.prob, cache, nullptr, 0, 16,
+                              &best, nullptr, 0, nullptr);
+        if (s != LT_OK) return fail(s, "autotune");
+        printf("autotuned: %u MB partitions, %u cache slots, span %s, %s scheduling, slice %d, zero-copy batch %d, "
+               "%s copies\n",
+               best.partition_mb, best.slots, best.span == LT_SPAN_RESIDENT ? "resident" : "partition",
+               best.device_schedule ? "device" : "host", best.slice_chunks, best.zc_batch,
+               best.pack_partitions == 2 ? "block" : best.pack_partitions == 1 ? "gap-coded" : "raw");
+        lt_partition_free(part);
+        if ((s = lt_partition_build(g, (uint64_t)best.partition_mb << 20, &part)) != LT_OK) return fail(s, "partition");
+        cfg.num_device_partitions = best.slots;
+        cfg.span = best.span;
+        cfg.device_schedule = best.device_schedule;
+        cfg.slice_chunks = best.slice_chunks;
+        cfg.zc_batch = best.zc_batch;
+        cfg.pack_partitions = best.pack_partitions;
+    }
     lt_engine *e = nullptr;
     s = lt_engine_create(g, part, &cfg, &e);
-    if (s == LT_E_ARG && P > 2900) {
-        // the reference's LDS budget check (Config.h:77-80)
-        printf("GPU shared memory overflow! Please consider reduce the number of partitions or batch size.\n");
-        return 1;
-    }
     if (s != LT_OK) return fail(s, "engine");
     lt_scorer *scorer = nullptr;
     if (!c.scorer.empty()) {
@@ -120,8 +137,9 @@ int main(int argc, char **argv) {
         if ((s = lt_scorer_load(scorer, c.scorer.c_str())) != LT_OK) return fail(s, "scorer weights");
         if ((s = lt_engine_set_scorer(e, scorer)) != LT_OK) return fail(s, "scorer");
     }
-    printf("number of partitons: %u, graph memory pool: %lu MB, walker memory pool: %lu MB\n", P,
-           (unsigned long)(lt_partition_bytes(part) * pdev / 1024 / 1024),
+    printf("number of partitons: %u, graph memory pool: %lu MB, walker memory pool: %lu MB\n",
+           lt_partition_count(part),
+           (unsigned long)(lt_partition_bytes(part) * cfg.num_device_partitions / 1024 / 1024),
            (unsigned long)(std::min(wdev, total) * 16 / 1024 / 1024));
 
     std::vector<uint32_t> counts(V, 0);

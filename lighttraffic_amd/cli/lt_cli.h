@@ -17,7 +17,7 @@
 //   --no-pipeline, --zerocopy-threshold <n>, --span partition|resident,
 //   --prefetch-depth <n>, --zc-batch <n>, --device-schedule, --pack,
 //   --pack-blocks, --rowptr-resident, --seed <n>, --timers, --json,
-//   --queries <file>
+//   --queries <file>, --tune (with --queries: autotune the configuration)
 #pragma once
 #include <stdint.h>
 #include <stdio.h>
@@ -45,6 +45,7 @@ struct CliConfig {
     bool device_schedule = false;
     int pack = 0;                // lt_config.pack_partitions: 1 --pack, 2 --pack-blocks
     bool rowptr_resident = false;
+    bool tune = false;           // ppr --queries: pick the configuration with lt_tune_ppr_batch
     bool pipeline = true;
     bool has_zc = false;
     uint64_t zc = 0;
@@ -109,6 +110,7 @@ static bool cli_parse(int argc, char **argv, CliConfig &c) {
         else if (a == "--timers") c.timers = true;
         else if (a == "--json") c.json = true;
         else if (a == "--queries") c.queries = next("queries");
+        else if (a == "--tune") c.tune = true;
         else {
             fprintf(stderr, "unknown option %s\n", argv[i]);
             return false;

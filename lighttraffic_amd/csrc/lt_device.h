@@ -21,7 +21,7 @@ constexpr int kBlockWalkers = kBlock * kWpt;  // 1024 walkers per workgroup pass
 constexpr int kChunkLog2 = 12;
 constexpr uint32_t kChunk = 1u << kChunkLog2;  // 4096 walkers = 64 KiB per chunk
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr uint32_t kMaxPartitions = 2900;   // append-stage LDS stays within 64 KiB
+constexpr uint32_t kMaxPartitions = 65535;  // partition ids are staged as u16 (and .part stores a u16 count)
 constexpr uint32_t kSliceChunks = 2048;     // chunks of one source queue per launch
 constexpr uint32_t kLostCap = 8192;         // chunks lost to allocation races per launch
 constexpr uint32_t kQStride = 32;           // u32 words per queue length: one 128-byte line each,
@@ -124,6 +124,17 @@ struct SchedState {
     const uint32_t *zc_col;     // device view of the pinned host graph's col
     uint64_t zc_threshold;
     uint32_t rp_host;           // 1: zc_rowptr is host memory (counted as H2D by the copies)
+};
+
+// Block slots in the device loop (decide_kernel, load_kernel): partition p's
+// block image is size[p] bytes at img + off[p] (mapped host memory), and its
+// bit streams are entered in words_tab when it is loaded; img == nullptr or
+// size[p] == 0: raw columns.
+struct BlockImages {
+    const uint32_t **words_tab;  // writable view of Parts::words_tab
+    const uint8_t *img;
+    const uint64_t *off;
+    const uint64_t *size;
 };
 
 struct WalkArgs {

@@ -153,7 +153,7 @@ struct Engine {
     hipStream_t s_compute = nullptr, s_copy = nullptr;
     int num_cu = 0;
     int max_grid = 0;           // init kernel (4 walkers per thread)
-    int walk_grid = 0;          // walk kernel: workgroups resident at once
+    int walk_grid[5] = {0, 0, 0, 0, 0};  // walk kernel, by walkers per thread
 
     // partition table
     uint32_t *d_node_off = nullptr;
@@ -249,6 +249,8 @@ struct Engine {
     uint8_t *d_resident = nullptr;
     uint8_t *d_zc_part = nullptr;  // [P] partitions of the running zero-copy batch
     bool rp_resident = false;      // rowptr of the whole graph kept in HBM (zc_rowptr points there)
+    uint64_t *d_pk_off = nullptr, *d_pk_size = nullptr;  // device loop with block slots: image table
+    const uint8_t *d_pk_img = nullptr;                     // device view of the pinned images
     uint64_t *d_rowptr_all = nullptr;
     uint64_t pending_h2d = 0;      // one-off H2D bytes not yet charged to a run
     uint32_t zc_batch = 1;         // partitions per zero-copy launch
@@ -307,7 +309,6 @@ struct Engine {
     AppArgs app{};
     int app_kind = APP_PAGERANK;
     size_t lds_bytes(int wpt = kWpt) const { return append_lds_bytes(P, wpt); }
-    size_t walk_lds_bytes() const { return append_lds_bytes(P, 2); }  // stage of 2 * kBlock walkers
     lt_status launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s);
     uint64_t partition_bytes(uint32_t p) const {
         return ((uint64_t)part.node_off[p + 1] - part.node_off[p] + 1) * 8 +
@@ -350,6 +351,8 @@ Engine::~Engine() {
     dfree(d_resident);
     dfree(d_zc_part);
     dfree(d_rowptr_all);
+    dfree(d_pk_off);
+    dfree(d_pk_size);
     dfree(d_stage[0]);
     dfree(d_stage[1]);
     if (s_compute) (void)hipStreamDestroy(s_compute);
@@ -398,8 +401,12 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     int occ = 0;
     LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, init_kernel<APP_PAGERANK>, kBlock, lds_bytes()));
     max_grid = std::max(1, occ) * num_cu;
-    LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, walk_kernel<APP_PAGERANK>, kBlock, walk_lds_bytes()));
-    walk_grid = std::max(1, occ) * num_cu;
+    const void *wk[5] = {nullptr, (const void *)walk_kernel<APP_PAGERANK, 1>, (const void *)walk_kernel<APP_PAGERANK, 2>,
+                         nullptr, (const void *)walk_kernel<APP_PAGERANK, 4>};
+    for (int wpt : {1, 2, 4}) {
+        LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wk[wpt], kBlock, lds_bytes(wpt)));
+        walk_grid[wpt] = std::max(1, occ) * num_cu;
+    }
 
     // partition table + vertex -> partition lookup table
     LT_TRY(dmalloc(&d_node_off, P + 1));
@@ -426,7 +433,9 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     // read in place, so slots are as large as the largest packed partition and
     // the same cache bytes hold more of them.
     s_dma = s_copy;
-    if (cfg.pack_partitions > 0 && !cfg.device_schedule) {
+    // The device loop copies with load_kernel and has no unpack step: it
+    // takes block slots (2), not gap-coded copies (1).
+    if (cfg.pack_partitions == 2 || (cfg.pack_partitions == 1 && !cfg.device_schedule)) {
         LT_TRY(packed_image(*g, part, cfg.pack_partitions >= 2 ? 2 : 1, &pk_hold));
         pk = pk_hold.get();
         if (!pk->max_size) {
@@ -480,6 +489,18 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
             LT_HIP(hipEventCreateWithFlags(&ev_dma, hipEventDisableTiming));
             LT_HIP(hipStreamCreateWithFlags(&s_dma, hipStreamNonBlocking));
         }
+    }
+    if (cfg.device_schedule && slot_packed) {  // the device loop's copies read the images in place
+        std::vector<uint64_t> off(P), size(P);
+        for (uint32_t i = 0; i < P; i++) {
+            off[i] = pk->off[i];
+            size[i] = pk->size[i];
+        }
+        LT_TRY(dmalloc(&d_pk_off, P));
+        LT_TRY(dmalloc(&d_pk_size, P));
+        LT_HIP(hipMemcpy(d_pk_off, off.data(), 8 * (size_t)P, hipMemcpyHostToDevice));
+        LT_HIP(hipMemcpy(d_pk_size, size.data(), 8 * (size_t)P, hipMemcpyHostToDevice));
+        LT_HIP(hipHostGetDevicePointer((void **)&d_pk_img, pk->host, 0));
     }
     if (rp_resident) {
         LT_TRY(dmalloc(&d_rowptr_all, (size_t)V + 1));
@@ -743,19 +764,27 @@ lt_status Engine::fetch_partition(uint32_t p, uint64_t *rowptr_out, uint32_t *co
     return LT_OK;
 }
 
-// Walk launch: one workgroup per kBlock walkers of the slice, at most as many
-// as are resident at once (each then streams its share through its lanes).
+// Walk launch sized to the slice: 1 walker per thread while the slice is
+// small enough that more walkers per thread would leave SIMDs without waves
+// (a thread walks its walkers one after another), 2 or 4 for large slices,
+// where the per-workgroup append cost (histogram, scan, one atomic per
+// destination queue) is spread over more walkers.
 template <int APP>
-static void launch_walk_app(const WalkArgs &a, int grid, size_t lds, hipStream_t s) {
-    walk_kernel<APP><<<grid, kBlock, lds, s>>>(a);
+static void launch_walk_app(const WalkArgs &a, int wpt, int grid, size_t lds, hipStream_t s) {
+    if (wpt == 4) walk_kernel<APP, 4><<<grid, kBlock, lds, s>>>(a);
+    else if (wpt == 2) walk_kernel<APP, 2><<<grid, kBlock, lds, s>>>(a);
+    else walk_kernel<APP, 1><<<grid, kBlock, lds, s>>>(a);
 }
 
 lt_status Engine::launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s) {
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + kBlock - 1) / kBlock, (uint64_t)walk_grid));
+    const uint64_t fill = (uint64_t)num_cu * 4 * 6 * 64;  // threads for ~6 waves per SIMD
+    const int wpt = walkers >= 4 * fill ? 4 : walkers >= 2 * fill ? 2 : 1;
+    const uint64_t bw = (uint64_t)kBlock * wpt;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + bw - 1) / bw, (uint64_t)walk_grid[wpt]));
     switch (app_kind) {
-        case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, grid, walk_lds_bytes(), s); break;
-        case APP_PPR: launch_walk_app<APP_PPR>(a, grid, walk_lds_bytes(), s); break;
-        default: launch_walk_app<APP_GENERIC>(a, grid, walk_lds_bytes(), s); break;
+        case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, wpt, grid, lds_bytes(wpt), s); break;
+        case APP_PPR: launch_walk_app<APP_PPR>(a, wpt, grid, lds_bytes(wpt), s); break;
+        default: launch_walk_app<APP_GENERIC>(a, wpt, grid, lds_bytes(wpt), s); break;
     }
     LT_HIP(hipGetLastError());
     meta_dirty = true;
@@ -1458,16 +1487,21 @@ lt_status Engine::scheduler_loop_device() {
     a.stats = d_stats;
     a.ss = sched_state(true);
     const int load_grid = 4 * num_cu;
+    BlockImages bi{};
+    bi.words_tab = d_words_tab;
+    bi.img = d_pk_img;
+    bi.off = d_pk_off;
+    bi.size = d_pk_size;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     LT_HIP(hipStreamBeginCapture(s_compute, hipStreamCaptureModeThreadLocal));
     for (int it = 0; it < kGraphIters; it++) {
-        decide_kernel<<<1, 1024, 0, s_compute>>>(q, a.ss, a.pt);
-        load_kernel<<<load_grid, 256, 0, s_compute>>>(a.ss, a.pt);
+        decide_kernel<<<1, 1024, 0, s_compute>>>(q, a.ss, a.pt, bi);
+        load_kernel<<<load_grid, 256, 0, s_compute>>>(a.ss, a.pt, bi);
         switch (app_kind) {
-            case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, walk_grid, walk_lds_bytes(), s_compute); break;
-            case APP_PPR: launch_walk_app<APP_PPR>(a, walk_grid, walk_lds_bytes(), s_compute); break;
-            default: launch_walk_app<APP_GENERIC>(a, walk_grid, walk_lds_bytes(), s_compute); break;
+            case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
+            case APP_PPR: launch_walk_app<APP_PPR>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
+            default: launch_walk_app<APP_GENERIC>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
         }
         fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, d_ds);
     }

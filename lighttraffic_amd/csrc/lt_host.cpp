@@ -308,6 +308,10 @@ static std::string part_path(const char *graph_path) { return std::string(graph_
 // (PartitionStrategy.h:325-337 writes it, :76-97 reads it).
 lt_status lt_partition_save(const lt_partition *p, const char *graph_path) {
     if (!p || !graph_path) return LT_E_ARG;
+    if (p->P() > 65535) {  // the count field is a u16
+        set_error("%u partitions do not fit a .part file (at most 65535); use a larger partition size", p->P());
+        return LT_E_ARG;
+    }
     const std::string path = part_path(graph_path);
     FILE *f = fopen(path.c_str(), "wb");
     if (!f) {

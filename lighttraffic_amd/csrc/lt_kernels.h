@@ -72,32 +72,52 @@ __device__ __forceinline__ uint32_t get_chunk(const Queues &q, uint32_t part, ui
     return c != kEmpty ? c : alloc_chunk(q, part, k);
 }
 
-// LDS layout of the append stage for P partitions.
+// LDS layout of the append stage.  Every destination of a pass gets a slot:
+// in direct mode the slot is the partition id (P slots); in hashed mode, for
+// partition tables too large for that, the distinct destinations of the pass
+// are numbered through an LDS hash table, so the layout depends only on the
+// walkers per pass (at most one slot each).
+constexpr uint32_t kDirectMaxP = 1024;  // larger partition tables use hashed slots
 struct AppendLds {
-    uint32_t *hist;   // [P] bucket sizes, then the first reserved queue slot
-    uint32_t *scan;   // [P+1] exclusive prefix of the bucket sizes
-    uint32_t *ch0;    // [P] chunk holding the first reserved slot
-    uint32_t *ch1;    // [P] chunk holding the last reserved slot
-    Walker *stage;    // [kBlockWalkers] walkers grouped by destination
-    uint16_t *dst;    // [kBlockWalkers] destination of each staged walker
+    uint32_t *hist;   // [nslot] walkers per slot, then the slot's first reserved queue slot
+    uint32_t *scan;   // [nslot+1] exclusive prefix of the walkers per slot
+    uint32_t *ch0;    // [nslot] chunk holding the first reserved queue slot
+    uint32_t *ch1;    // [nslot] chunk holding the last reserved queue slot
+    uint32_t *part;   // hashed: [nslot] partition of each slot in use
+    uint32_t *keys;   // hashed: [2*nslot] hash table of destination partitions (kEmpty = free)
+    uint32_t *tslot;  // hashed: [2*nslot] slot given to each table entry
+    uint32_t *nused;  // hashed: slots in use in the pass
+    Walker *stage;    // [kBlock * wpt] walkers grouped by slot
+    uint16_t *dst;    // [kBlock * wpt] slot of each staged walker
+    uint32_t nslot;
+    bool hashed;
 };
 
 // walkers staged per workgroup pass = kBlock * wpt
 __host__ __device__ inline size_t append_lds_bytes(uint32_t P, int wpt) {
-    size_t b = (4 * (size_t)P + 1) * 4;
+    const size_t nw = (size_t)kBlock * wpt;
+    size_t b = P <= kDirectMaxP ? (4 * (size_t)P + 1) * 4 : (9 * nw + 2) * 4;
     b = (b + 15) & ~(size_t)15;
-    return b + (size_t)kBlock * wpt * (16 + 2);
+    return b + nw * (16 + 2);
 }
 
 __device__ __forceinline__ AppendLds carve_lds(uint8_t *smem, uint32_t P, int wpt) {
     AppendLds L;
+    const uint32_t nw = kBlock * (uint32_t)wpt;
+    L.hashed = P > kDirectMaxP;
+    L.nslot = L.hashed ? nw : P;
     L.hist = (uint32_t *)smem;
-    L.scan = L.hist + P;
-    L.ch0 = L.scan + P + 1;
-    L.ch1 = L.ch0 + P;
-    size_t off = ((4 * (size_t)P + 1) * 4 + 15) & ~(size_t)15;
+    L.scan = L.hist + L.nslot;
+    L.ch0 = L.scan + L.nslot + 1;
+    L.ch1 = L.ch0 + L.nslot;
+    L.part = L.ch1 + L.nslot;      // hashed mode only from here on
+    L.keys = L.part + L.nslot;
+    L.tslot = L.keys + 2 * L.nslot;
+    L.nused = L.tslot + 2 * L.nslot;
+    const size_t words = L.hashed ? 9 * (size_t)nw + 2 : 4 * (size_t)P + 1;
+    const size_t off = (words * 4 + 15) & ~(size_t)15;
     L.stage = (Walker *)(smem + off);
-    L.dst = (uint16_t *)(L.stage + kBlock * wpt);
+    L.dst = (uint16_t *)(L.stage + nw);
     return L;
 }
 
@@ -131,16 +151,111 @@ __device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint
     return total;
 }
 
+// block_append for partition tables above kDirectMaxP: the pass's distinct
+// destinations are numbered through an open-addressed LDS table of
+// 2 * nslot entries (at most nslot keys, so every probe sequence finds its
+// key or a free entry), and the histogram, scan, reservations and staging of
+// block_append then work on those slots instead of on partition ids.
+template <int WPT>
+__device__ void block_append_hashed(const Queues &q, const AppendLds &L, const Walker (&w)[WPT],
+                                    const uint32_t (&dst)[WPT], uint32_t *s_wsum) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t T = 2 * L.nslot;  // a power of two
+    for (uint32_t i = tid; i < T; i += kBlock) L.keys[i] = kEmpty;
+    if (tid == 0) *L.nused = 0;
+    __syncthreads();
+    uint32_t ent[WPT] = {};
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        if (dst[k] == kEmpty) continue;
+        uint32_t h = dst[k] * 2654435761u;
+        h = (h ^ (h >> 15)) & (T - 1);
+        for (uint32_t probe = 0; probe < T; probe++) {
+            const uint32_t old = atomicCAS(&L.keys[h], kEmpty, dst[k]);
+            if (old == kEmpty || old == dst[k]) break;
+            h = (h + 1) & (T - 1);
+        }
+        ent[k] = h;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < T; i += kBlock) {
+        const uint32_t key = L.keys[i];
+        if (key != kEmpty) {
+            const uint32_t sidx = atomicAdd(L.nused, 1u);
+            L.tslot[i] = sidx;
+            L.part[sidx] = key;
+            L.hist[sidx] = 0;
+        }
+    }
+    __syncthreads();
+    const uint32_t n = *L.nused;
+    uint32_t sl[WPT], rank[WPT];
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        sl[k] = dst[k] != kEmpty ? L.tslot[ent[k]] : kEmpty;
+        if (dst[k] != kEmpty) rank[k] = atomicAdd(&L.hist[sl[k]], 1u);
+    }
+    __syncthreads();
+    const uint32_t total = block_exclusive_scan(L.hist, L.scan, n, s_wsum);
+    if (tid == 0) L.scan[n] = total;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kBlock) {  // pass 1: the chunks this workgroup owns
+        const uint32_t c = L.hist[i];
+        if (c) {
+            const uint32_t p = L.part[i];
+            const uint32_t b = atomicAdd(&qlen_of(q, p), c);
+            const uint32_t k0 = b >> kChunkLog2, k1 = (b + c - 1) >> kChunkLog2;
+            const uint32_t c0 = (b & (kChunk - 1)) == 0 ? alloc_chunk(q, p, k0) : kEmpty;
+            L.ch0[i] = c0;
+            L.ch1[i] = k1 != k0 ? alloc_chunk(q, p, k1) : c0;
+            L.hist[i] = b;
+        }
+    }
+    for (uint32_t i = tid; i < n; i += kBlock) {  // pass 2: chunks owned by earlier reservations
+        const uint32_t b = L.hist[i];
+        if (L.scan[i + 1] != L.scan[i] && (b & (kChunk - 1)) != 0) {
+            const uint32_t c0 = get_chunk(q, L.part[i], b >> kChunkLog2);
+            const uint32_t k1 = (b + (L.scan[i + 1] - L.scan[i]) - 1) >> kChunkLog2;
+            L.ch0[i] = c0;
+            if (k1 == (b >> kChunkLog2)) L.ch1[i] = c0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        if (dst[k] != kEmpty) {
+            const uint32_t pos = L.scan[sl[k]] + rank[k];
+            L.stage[pos] = w[k];
+            L.dst[pos] = (uint16_t)sl[k];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < total; i += kBlock) {
+        const uint32_t slot = L.dst[i];
+        const uint32_t b = L.hist[slot];
+        const uint32_t s = b + (i - L.scan[slot]);
+        const uint32_t c = (s >> kChunkLog2) == (b >> kChunkLog2) ? L.ch0[slot] : L.ch1[slot];
+        if (c < q.nchunks) q.slab[(size_t)c * kChunk + (s & (kChunk - 1))] = L.stage[i];
+        else atomicOr(&q.ctl->overflow, 8u);
+    }
+    __syncthreads();
+}
+
 // Append the walkers with dst[k] != kEmpty to their partitions' queues.
 // Collective: every thread of the workgroup must call it.  Chunk ids are
 // resolved once per (workgroup, destination): a reservation of at most
 // kBlockWalkers <= kChunk slots touches at most two chunks; the range that
 // contains a chunk's first slot links it, and a later range that arrives
 // before that link is published links it itself (the CAS loser's chunk goes
-// to the lost list).
+// to the lost list).  Above kDirectMaxP partitions the slots are numbered
+// through a hash table (block_append_hashed).
 template <int WPT>
-__device__ __forceinline__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, const Walker (&w)[WPT],
+__device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, const Walker (&w)[WPT],
                              const uint32_t (&dst)[WPT], uint32_t *s_wsum) {
+    if (L.hashed) {  // uniform: more partitions than LDS slots
+        block_append_hashed<WPT>(q, L, w, dst, s_wsum);
+        return;
+    }
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < P; i += kBlock) L.hist[i] = 0;
     __syncthreads();
@@ -234,12 +349,25 @@ struct Span {
     int zc;
 };
 
+// Column of partition-local edge e.  A packed slot holds, per block of 16
+// edges, {base, bit offset} (the next block's offset gives the width) and
+// the bit streams: column = base + bits(offset + (e % 16) * width, width).
+__device__ __forceinline__ uint32_t span_col(const Span &sp, uint64_t e) {
+    if (!sp.words) return sp.col[e];
+    const uint64_t k = e >> 4;
+    const uint2 h = ((const uint2 *)sp.col)[k];
+    const uint32_t w = (sp.col[2 * k + 3] - h.y) >> 4;
+    const uint32_t pos = h.y + (uint32_t)(e & 15) * w;
+    const uint32_t lo = sp.words[pos >> 5], hi = sp.words[(pos >> 5) + 1];
+    const uint32_t bits = __builtin_amdgcn_alignbit(hi, lo, pos & 31u);
+    return h.x + (w >= 32 ? bits : bits & ((1u << w) - 1u));
+}
+
 __device__ __forceinline__ uint64_t walker_id(const Walker &w) { return ((uint64_t)w.id_hi << 32) | w.id_lo; }
 
 // Visits go to a workgroup table in LDS (open addressing, 4 probes) and reach
-// the global counters once per vertex and flush (flush_visits, at every
-// append of staged walkers and at the end); a vertex that finds no slot is
-// counted in global memory directly.
+// the global counters once per vertex and pass (flush_visits); a vertex that
+// finds no slot is counted in global memory directly.
 constexpr uint32_t kVisitSlots = 2048;
 constexpr uint32_t kVisitEmpty = 0xFFFFFFFFu;
 
@@ -277,166 +405,113 @@ __device__ __forceinline__ void flush_visits(const AppArgs &a, const Counters &c
     __syncthreads();
 }
 
+// One update of app/pagerank.cu:26-44: count the visit, then teleport with
+// probability 0.15 or move to a uniform neighbour.  A vertex without
+// neighbours teleports (the reference divides by zero there).
+__device__ __forceinline__ void step_pagerank(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
+    count_visit(a, w.cur, ct);
+    ct.visits++;
+    w.step -= 1;
+    if (w.step == 0) return;
+    const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_PAGERANK);
+    bool tele = x.x < a.thresh;
+    if (!tele) {
+        const uint64_t *r = sp.rowptr + (w.cur - sp.nb);
+        const uint64_t b = r[0], d = r[1] - b;
+        if (d == 0) {
+            tele = true;
+            if (sp.zc) ct.zc_reads += a.rp_host;
+        } else {
+            w.cur = span_col(sp, b - sp.eb + bounded(x.y, x.z, d));
+            ct.moves++;
+            if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
+        }
+    }
+    if (tele) {
+        w.cur = (uint32_t)bounded(x.y, x.z, a.V);
+        ct.tele++;
+    }
+}
+
+// One update of app/ppr.cu:22-42: the endpoint is counted when one step is
+// left; a vertex without neighbours is counted and ends the walk.
+__device__ __forceinline__ void step_ppr(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
+    if (w.step == 1) {
+        if (a.counts) count_visit(a, w.cur, ct);
+        ct.visits++;
+    }
+    w.step -= 1;
+    if (w.step == 0) return;
+    const uint64_t *r = sp.rowptr + (w.cur - sp.nb);
+    const uint64_t b = r[0], d = r[1] - b;
+    if (d > 0) {
+        const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_PPR);
+        w.cur = span_col(sp, b - sp.eb + bounded(x.y, x.z, d));
+        ct.moves++;
+        if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
+    } else {
+        if (a.counts) count_visit(a, w.cur, ct);
+        ct.visits++;
+        w.step = 0;
+        if (sp.zc) ct.zc_reads += a.rp_host;
+    }
+}
+
+// One update of app/genericwalk.cu:22-32: a uniform neighbour move, or stop
+// at a vertex without neighbours.  Records the number of moves made.
+__device__ __forceinline__ void step_generic(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
+    const uint64_t *r = sp.rowptr + (w.cur - sp.nb);
+    const uint64_t b = r[0], d = r[1] - b;
+    uint32_t *path = a.paths ? a.paths + walker_id(w) * ((uint64_t)a.length + 1) : nullptr;
+    if (path) path[a.length - w.step] = w.cur;  // the vertex after (length - step) moves
+    if (d > 0) {
+        const u32x4 x = draw(a.seed, walker_id(w), w.step, TAG_GENERIC);
+        w.cur = span_col(sp, b - sp.eb + bounded(x.y, x.z, d));
+        ct.moves++;
+        if (sp.zc) { ct.zc_moves++; ct.zc_reads += 1 + a.rp_host; }
+        w.step -= 1;
+        if (w.step == 0) {
+            if (a.moves_out) a.moves_out[walker_id(w)] = a.length;
+            if (path) path[a.length] = w.cur;
+        }
+    } else {
+        if (a.moves_out) a.moves_out[walker_id(w)] = a.length - w.step;
+        w.step = 0;
+        if (sp.zc) ct.zc_reads += a.rp_host;
+    }
+}
+
+template <int APP>
+__device__ __forceinline__ void app_step(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
+    if (APP == APP_PAGERANK) step_pagerank(w, sp, a, ct);
+    else if (APP == APP_PPR) step_ppr(w, sp, a, ct);
+    else step_generic(w, sp, a, ct);
+}
+
 __device__ __forceinline__ void finish_walker(const Walker &w, const AppArgs &a) {
     if (a.endpoints) a.endpoints[walker_id(w)] = w.cur;
 }
 
-// Walker steps are split into stages so that the K walkers of a thread step
-// in lockstep and their dependent reads overlap: stage_a draws the step's
-// Philox words (and counts the visit), stage_b reads the rowptr pair of
-// every walker, stage_c the chosen column (or, in a packed slot, its block
-// header), stage_d the bit-stream words of packed slots, and stage_e moves
-// the walkers.  Per walker the draws, reads and updates are exactly those of
-// one App::update; only their order across walkers changes.
-struct StepTmp {
-    u32x4 x;        // Philox words of the step
-    uint64_t b, d;  // first edge (global) and degree of the current vertex
-    uint64_t e;     // partition-local edge taken; packed slot: bit position of its value
-    uint2 h;        // packed slot: block header {base, bit offset}
-    uint32_t nx;    // packed slot: next block's bit offset, then the value width
-    uint32_t c, c2; // column (raw); packed slot: the two stream words holding the value
-    bool need;      // reads the rowptr pair this step
-    bool mv;        // moves to a neighbour this step
-};
-
-// Stage a of app/pagerank.cu:26-44 (count the visit; after the last step
-// nothing moves), app/ppr.cu:22-42 (count the endpoint when one step is
-// left) and app/genericwalk.cu:22-32 (record the path).  A walker whose walk
-// just ended keeps need = false and step = 0.
+// Walk one residency; returns the destination partition or kEmpty when the
+// walk is over.  With span_resident the walker continues into any partition
+// whose slot is resident (and into the source slice itself).
 template <int APP>
-__device__ __forceinline__ void stage_a(Walker &w, const AppArgs &a, Counters &ct, StepTmp &t) {
-    t.mv = false;
-    t.need = false;
-    if (APP == APP_PAGERANK) {
-        count_visit(a, w.cur, ct);
-        ct.visits++;
-        w.step -= 1;
-        if (w.step == 0) return;
-        t.x = draw(a.seed, walker_id(w), w.step, TAG_PAGERANK);
-        t.need = !(t.x.x < a.thresh);  // teleport without reading the graph
-    } else if (APP == APP_PPR) {
-        if (w.step == 1) {
-            if (a.counts) count_visit(a, w.cur, ct);
-            ct.visits++;
-        }
-        w.step -= 1;
-        if (w.step == 0) return;
-        t.x = draw(a.seed, walker_id(w), w.step, TAG_PPR);
-        t.need = true;
-    } else {
-        if (a.paths) a.paths[walker_id(w) * ((uint64_t)a.length + 1) + (a.length - w.step)] = w.cur;
-        t.x = draw(a.seed, walker_id(w), w.step, TAG_GENERIC);
-        t.need = true;
-    }
-}
-
-__device__ __forceinline__ void stage_b(const Walker &w, const Span &sp, StepTmp &t) {
-    if (!t.need) return;
-    const uint64_t *r = sp.rowptr + (w.cur - sp.nb);
-    t.b = r[0];
-    t.d = r[1];
-}
-
-// Column of partition-local edge e.  A packed slot holds, per block of 16
-// edges, {base, bit offset} (the next block's offset gives the width) and
-// the bit streams: column = base + bits(offset + (e % 16) * width, width).
-__device__ __forceinline__ void stage_c(const Span &sp, StepTmp &t) {
-    if (!t.need) return;
-    t.d -= t.b;
-    if (t.d == 0) return;
-    t.mv = true;
-    t.e = t.b - sp.eb + bounded(t.x.y, t.x.z, t.d);
-    if (!sp.words) {
-        t.c = sp.col[t.e];
-    } else {
-        const uint64_t k = t.e >> 4;
-        t.h = ((const uint2 *)sp.col)[k];
-        t.nx = sp.col[2 * k + 3];
-    }
-}
-
-__device__ __forceinline__ void stage_d(const Span &sp, StepTmp &t) {
-    if (!t.mv || !sp.words) return;
-    const uint32_t w = (t.nx - t.h.y) >> 4;
-    const uint32_t pos = t.h.y + (uint32_t)(t.e & 15) * w;
-    t.c = sp.words[pos >> 5];
-    t.c2 = sp.words[(pos >> 5) + 1];
-    t.nx = w;
-    t.e = pos;
-}
-
-__device__ __forceinline__ uint32_t step_col(const Span &sp, const StepTmp &t) {
-    if (!sp.words) return t.c;
-    const uint32_t bits = __builtin_amdgcn_alignbit(t.c2, t.c, (uint32_t)t.e & 31u);
-    return t.h.x + (t.nx >= 32 ? bits : bits & ((1u << t.nx) - 1u));
-}
-
-// Stage e: pagerank teleports when it drew a jump or stands on a vertex
-// without neighbours (the reference divides by zero there); ppr counts a
-// vertex without neighbours and stops there; genericwalk stops there and
-// records its number of moves.
-template <int APP>
-__device__ __forceinline__ void stage_e(Walker &w, const Span &sp, const AppArgs &a, Counters &ct, const StepTmp &t) {
-    if (t.mv) {
-        w.cur = step_col(sp, t);
-        ct.moves++;
-        if (sp.zc) {
-            ct.zc_moves++;
-            ct.zc_reads += 1 + a.rp_host;
-        }
-        if (APP == APP_GENERIC) {
-            w.step -= 1;
-            if (w.step == 0) {
-                if (a.moves_out) a.moves_out[walker_id(w)] = a.length;
-                if (a.paths) a.paths[walker_id(w) * ((uint64_t)a.length + 1) + a.length] = w.cur;
-            }
-        }
-        return;
-    }
-    if (t.need && sp.zc) ct.zc_reads += a.rp_host;  // read a rowptr pair of degree 0
-    if (APP == APP_PAGERANK) {
-        if (w.step == 0) return;
-        w.cur = (uint32_t)bounded(t.x.y, t.x.z, a.V);
-        ct.tele++;
-    } else if (APP == APP_PPR) {
-        if (w.step == 0) return;
-        if (a.counts) count_visit(a, w.cur, ct);
-        ct.visits++;
-        w.step = 0;
-    } else {
-        if (a.moves_out) a.moves_out[walker_id(w)] = a.length - w.step;
-        w.step = 0;
-    }
-}
-
-// Before each lockstep step: end the walk, or find the span the walker's
-// vertex lies in.  Returns true while the walker takes steps in this launch;
-// otherwise dst is its destination partition, or kEmpty when its walk is
-// over.  With span_resident the walker continues into any partition whose
-// slot is resident (and into the source slice itself).
-__device__ __forceinline__ bool settle(Walker &w, const WalkArgs &a, uint32_t src, const Span &src_sp, Span &sp,
-                                       uint32_t &dst) {
+__device__ uint32_t walk_residency(Walker &w, const WalkArgs &a, uint32_t src, const Span &src_sp, Counters &ct) {
+    Span sp = src_sp;
     while (true) {
+        while (w.step > 0 && (w.cur - sp.nb) < (sp.ne - sp.nb)) app_step<APP>(w, sp, a.app, ct);
         if (w.step == 0) {
             finish_walker(w, a.app);
-            dst = kEmpty;
-            return false;
+            return kEmpty;
         }
-        if ((w.cur - sp.nb) < (sp.ne - sp.nb)) return true;
         const uint32_t p = find_part(a.pt, w.cur);
-        if (!a.span_resident) {
-            dst = p;
-            return false;
-        }
+        if (!a.span_resident) return p;
         if (p == src) {
             sp = src_sp;
             continue;
         }
         const uint64_t *rp = a.pt.rowptr_tab[p];
-        if (!rp) {
-            dst = p;
-            return false;
-        }
+        if (!rp) return p;
         sp.rowptr = rp;
         sp.col = a.pt.col_tab[p];
         sp.words = a.pt.words_tab[p];
@@ -447,137 +522,55 @@ __device__ __forceinline__ bool settle(Walker &w, const WalkArgs &a, uint32_t sr
     }
 }
 
-// One update of the walker: the stages in order.
-template <int APP>
-__device__ __forceinline__ void app_step(Walker &w, const Span &sp, const AppArgs &a, Counters &ct) {
-    StepTmp t;
-    stage_a<APP>(w, a, ct, t);
-    stage_b(w, sp, t);
-    stage_c(sp, t);
-    stage_d(sp, t);
-    stage_e<APP>(w, sp, a, ct, t);
-}
-
-// Where the walkers of one launch come from, kept in LDS: the launch's
-// walkers form one dense range, and source k holds [pref[k], pref[k+1]).  A
-// source is a queue (walkers from queue slot first[k] of partition part[k]
-// on) or a piece of spilled walkers in host memory (host[k]).
-constexpr uint32_t kMaxSources = kMaxHostPieces > kMaxZcBatch ? kMaxHostPieces : kMaxZcBatch;
-struct SrcTable {
-    uint64_t pref[kMaxSources + 1];
-    uint64_t first[kMaxSources];
-    const Walker *host[kMaxSources];
-    uint32_t part[kMaxSources];
-    uint32_t n;
-    uint32_t cursor;  // walkers of this workgroup's share taken so far
-    uint32_t staged;  // walkers waiting in the LDS stage for their append
-};
-
-// Thread 0 fills the table; the caller synchronises.  src_out is the
-// partition whose queue the launch consumes (kEmpty for a zero-copy batch,
-// whose partitions are entered in the partition table for the launch).
-__device__ void build_sources(const WalkArgs &a, SrcTable &s, uint32_t &src_out) {
-    s.n = 0;
-    s.pref[0] = 0;
-    s.cursor = 0;
-    s.staged = 0;
-    src_out = a.src;
-    if (a.nsrc) {  // zero-copy batch: whole queues of srcs[0..nsrc)
-        src_out = kEmpty;
-#pragma unroll  // constant indices keep the argument arrays out of scratch memory
-        for (uint32_t k = 0; k < kMaxZcBatch; k++) {
-            if (k < a.nsrc) {
-                s.part[k] = a.srcs[k];
-                s.first[k] = 0;
-                s.host[k] = nullptr;
-                s.pref[k + 1] = s.pref[k] + qlen_of(a.q, a.srcs[k]);
-            }
-        }
-        s.n = a.nsrc;
-        return;
-    }
-    if (a.nhost) {  // walkers of src spilled to host memory
+// One workgroup pass over nvalid <= kBlock * WPT walkers stored
+// contiguously at cw: walk each, then append the survivors to their queues.
+template <int APP, int WPT>
+__device__ __forceinline__ void walk_walkers(const WalkArgs &a, const AppendLds &L, uint32_t *s_wsum, uint32_t src,
+                                            const Span &src_sp, const Walker *cw, uint32_t nvalid, Counters &ct) {
+    Walker w[WPT];
+    uint32_t dst[WPT];
 #pragma unroll
-        for (uint32_t k = 0; k < kMaxHostPieces; k++) {
-            if (k < a.nhost) {
-                s.part[k] = a.src;
-                s.first[k] = 0;
-                s.host[k] = a.hptr[k];
-                s.pref[k + 1] = s.pref[k] + a.hn[k];
-            }
+    for (int k = 0; k < WPT; k++) {
+        const uint32_t i = (uint32_t)k * kBlock + threadIdx.x;
+        dst[k] = kEmpty;
+        if (i < nvalid) {
+            w[k] = cw[i];
+            ct.resid++;
+        } else {
+            w[k].step = 0;
         }
-        s.n = a.nhost;
-        return;
     }
-    uint32_t src = a.src, chunk_lo = a.chunk_lo, chunk_hi = a.chunk_hi;
-    if (a.ss.ds) {  // the decision of this iteration was taken on the device
-        if (a.ss.ds->choice < 0) return;
-        src = (uint32_t)a.ss.ds->choice;
-        chunk_lo = 0;
-        chunk_hi = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        if ((uint32_t)k * kBlock + threadIdx.x < nvalid) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
     }
-    src_out = src;
-    const uint64_t lo = (uint64_t)chunk_lo << kChunkLog2;
-    const uint64_t hi = min((uint64_t)qlen_of(a.q, src), (uint64_t)chunk_hi << kChunkLog2);
-    s.part[0] = src;
-    s.first[0] = lo;
-    s.host[0] = nullptr;
-    s.pref[1] = hi > lo ? hi - lo : 0;
-    s.n = 1;
+    block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum);
+    flush_visits(a.app, ct);
 }
 
-// Walker i of the launch's dense range, and the span it starts in.  Returns
-// false (and flags the pool) if its queue chunk is not a valid chunk id.
-__device__ __forceinline__ bool fetch_walker(const WalkArgs &a, const SrcTable &s, const Span &src_sp, uint64_t i,
-                                             Walker &w, Span &sp) {
-    uint32_t k = 0;
-    while (k + 1 < s.n && i >= s.pref[k + 1]) k++;
-    const uint64_t local = i - s.pref[k];
-    if (s.host[k]) {
-        w = s.host[k][local];
-        sp = src_sp;
-        return true;
-    }
-    const uint32_t p = s.part[k];
-    const uint64_t pos = s.first[k] + local;
-    const uint32_t c = a.q.frame[(size_t)p * a.q.maxc + (uint32_t)(pos >> kChunkLog2)];
-    if (c >= a.q.nchunks) {
-        atomicOr(&a.q.ctl->overflow, 8u);
-        return false;
-    }
-    w = a.q.slab[(size_t)c * kChunk + (uint32_t)(pos & (kChunk - 1))];
-    if (a.nsrc) {  // a zero-copy batch partition, read from the pinned host graph
-        sp.rowptr = a.ss.zc_rowptr + a.pt.node_off[p];
-        sp.col = a.ss.zc_col + a.pt.edge_off[p];
-        sp.words = nullptr;
-        sp.nb = a.pt.node_off[p];
-        sp.ne = a.pt.node_off[p + 1];
-        sp.eb = a.pt.edge_off[p];
-        sp.zc = 1;
-    } else {
-        sp = src_sp;
-    }
-    return true;
+// One workgroup pass: walkers [base, base + kBlock * WPT) of src's queue,
+// clipped to hi64; base is a multiple of kBlock * WPT, so the pass reads one
+// chunk.
+template <int APP, int WPT>
+__device__ __forceinline__ void walk_pass(const WalkArgs &a, const AppendLds &L, uint32_t *s_wsum, uint32_t src,
+                                          const Span &src_sp, uint64_t base, uint64_t hi64, Counters &ct) {
+    const uint32_t chunk = a.q.frame[(size_t)src * a.q.maxc + (uint32_t)(base >> kChunkLog2)];
+    const bool chunk_ok = chunk < a.q.nchunks;
+    if (!chunk_ok && threadIdx.x == 0) atomicOr(&a.q.ctl->overflow, 8u);
+    const Walker *cw = a.q.slab + (size_t)(chunk_ok ? chunk : 0) * kChunk + (uint32_t)(base & (kChunk - 1));
+    const uint64_t left = hi64 > base ? hi64 - base : 0;
+    const uint32_t nvalid = chunk_ok ? (uint32_t)min(left, (uint64_t)(kBlock * WPT)) : 0u;
+    walk_walkers<APP, WPT>(a, L, s_wsum, src, src_sp, cw, nvalid, ct);
 }
 
-// Walk kernel: consumes the walkers of one launch (a slice of a partition's
-// queue, a zero-copy batch of whole queues, or spilled pieces in host
-// memory).  Each workgroup takes an equal share of the launch's walkers and
-// streams it through its lanes: every iteration each lane takes one step of
-// its walker; a lane whose walker ended or left the span stages it in LDS
-// and takes the next walker of the share at once.  So a lane idles only at
-// the end of the share, not while the longest walker of a fixed batch
-// finishes.  Once a workgroup's stage holds a workgroup's worth of walkers
-// they are appended to their destination queues together (block_append:
-// LDS histogram, scan, one atomic per destination).
-template <int APP>
+// WPT walkers per thread per pass (1, 2 or 4): small queues use 1 so that
+// enough workgroups exist to fill the chip, large ones 4.
+template <int APP, int WPT>
 __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
+    constexpr int BW = kBlock * WPT;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ uint32_t s_wsum[kBlock / 64];
-    __shared__ SrcTable s_src;
-    __shared__ uint32_t s_srcpart;
-    constexpr int kStageWpt = 2;  // stage holds kBlock * 2 walkers
-    const AppendLds L = carve_lds(smem, a.pt.P, kStageWpt);
+    const AppendLds L = carve_lds(smem, a.pt.P, WPT);
     Counters ct;
     if constexpr (APP != APP_GENERIC) {
         __shared__ uint32_t s_vkeys[kVisitSlots], s_vcnt[kVisitSlots];
@@ -586,91 +579,77 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
                 s_vkeys[i] = kVisitEmpty;
                 s_vcnt[i] = 0;
             }
+            __syncthreads();
             ct.vkeys = s_vkeys;
             ct.vcnt = s_vcnt;
         }
     }
-    if (threadIdx.x == 0) build_sources(a, s_src, s_srcpart);
-    __syncthreads();
-    const uint64_t total = s_src.pref[s_src.n];
-    if (total == 0) return;  // uniform: nothing counted, nothing staged
-    const uint32_t src = s_srcpart;
+    if (a.nsrc) {  // zero-copy batch: every source is read over PCIe
+        const uint64_t vend = a.voff[a.nsrc];
+        for (uint64_t base = (uint64_t)blockIdx.x * BW; base < vend; base += (uint64_t)gridDim.x * BW) {
+            uint32_t k = 0;
+            while (k + 1 < a.nsrc && base >= a.voff[k + 1]) k++;
+            const uint32_t src = a.srcs[k];
+            Span sp;
+            sp.rowptr = a.ss.zc_rowptr + a.pt.node_off[src];
+            sp.col = a.ss.zc_col + a.pt.edge_off[src];
+            sp.words = nullptr;
+            sp.nb = a.pt.node_off[src];
+            sp.ne = a.pt.node_off[src + 1];
+            sp.eb = a.pt.edge_off[src];
+            sp.zc = 1;
+            const uint64_t local = base - a.voff[k], n = qlen_of(a.q, src);
+            if (local < n) walk_pass<APP, WPT>(a, L, s_wsum, src, sp, local, n, ct);  // uniform per workgroup
+        }
+        flush_counters(ct, a.stats);
+        return;
+    }
+    uint32_t src = a.src, chunk_lo = a.chunk_lo, chunk_hi = a.chunk_hi;
     Span src_sp;
     src_sp.rowptr = a.rowptr;
     src_sp.col = a.col;
     src_sp.words = a.words;
     src_sp.zc = a.zerocopy;
-    if (a.ss.ds && src != kEmpty) {  // device-scheduled iteration: raw columns in a slot or in host memory
+    if (a.nhost) {  // walkers of src spilled to host memory, read over PCIe
+        src_sp.nb = a.pt.node_off[src];
+        src_sp.ne = a.pt.node_off[src + 1];
+        src_sp.eb = a.pt.edge_off[src];
+        const uint64_t vend = (uint64_t)a.nhost << kChunkLog2;
+        for (uint64_t base = (uint64_t)blockIdx.x * BW; base < vend; base += (uint64_t)gridDim.x * BW) {
+            const uint32_t k = (uint32_t)(base >> kChunkLog2), local = (uint32_t)(base & (kChunk - 1));
+            if (local < a.hn[k])  // uniform per workgroup
+                walk_walkers<APP, WPT>(a, L, s_wsum, src, src_sp, a.hptr[k] + local,
+                                       min(a.hn[k] - local, (uint32_t)BW), ct);
+        }
+        flush_counters(ct, a.stats);
+        return;
+    }
+    if (a.ss.ds) {  // the decision of this iteration was taken on the device
+        const int32_t c = a.ss.ds->choice;
+        if (c < 0) return;
+        src = (uint32_t)c;
+        chunk_lo = 0;
+        chunk_hi = 0xFFFFFFFFu;
         src_sp.zc = a.ss.ds->zerocopy;
         src_sp.words = nullptr;
         if (src_sp.zc) {
             src_sp.rowptr = a.ss.zc_rowptr + a.pt.node_off[src];
             src_sp.col = a.ss.zc_col + a.pt.edge_off[src];
-        } else {
+        } else {  // a raw slot, or a block slot whose bit streams decide_kernel entered in the table
             const uint8_t *base = a.ss.slots + (size_t)a.ss.slot_of[src] * a.ss.slot_bytes;
             src_sp.rowptr = (const uint64_t *)base;
             src_sp.col = (const uint32_t *)(base + ((uint64_t)a.pt.node_off[src + 1] - a.pt.node_off[src] + 1) * 8);
+            src_sp.words = a.pt.words_tab[src];
         }
     }
-    if (src != kEmpty) {
-        src_sp.nb = a.pt.node_off[src];
-        src_sp.ne = a.pt.node_off[src + 1];
-        src_sp.eb = a.pt.edge_off[src];
-    }
-    // this workgroup's share [lo, hi) of the dense range, 64-aligned
-    const uint64_t per = ((total + gridDim.x - 1) / gridDim.x + 63) & ~(uint64_t)63;
-    const uint64_t lo = min(total, (uint64_t)blockIdx.x * per), hi = min(total, lo + per);
-    const uint32_t lane = threadIdx.x & 63u;
-    Walker w;
-    Span sp = src_sp;
-    bool live = false;
-    while (true) {
-        // lanes without a walker take the next ones of the share, one LDS
-        // atomic per wave, consecutive lanes getting consecutive walkers
-        const uint64_t want = __ballot(!live);
-        if (want) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&s_src.cursor, (uint32_t)__popcll(want));
-            base = __shfl(base, leader, 64);
-            if (!live) {
-                const uint64_t i = lo + base + (uint32_t)__popcll(want & ((1ull << lane) - 1));
-                if (i < hi && fetch_walker(a, s_src, src_sp, i, w, sp)) {
-                    live = true;
-                    ct.resid++;
-                }
-            }
-        }
-        if (live) {
-            uint32_t dst;
-            live = settle(w, a, src, src_sp, sp, dst);
-            if (!live && dst != kEmpty) {  // at most one per lane and iteration
-                const uint32_t slot = atomicAdd(&s_src.staged, 1u);
-                L.stage[slot] = w;
-                L.dst[slot] = (uint16_t)dst;
-            }
-        }
-        if (live) app_step<APP>(w, sp, a.app, ct);
-        const bool any_live = __syncthreads_or(live) != 0;
-        const uint32_t staged = s_src.staged;
-        const bool done = !any_live && lo + s_src.cursor >= hi;
-        if (staged >= kBlock || (done && staged > 0)) {  // staged < 2 * kBlock: at most kBlock added per iteration
-            Walker sw[kStageWpt];
-            uint32_t sd[kStageWpt];
-#pragma unroll
-            for (int j = 0; j < kStageWpt; j++) {
-                const uint32_t i = (uint32_t)j * kBlock + threadIdx.x;  // < 2 * kBlock, inside the stage
-                sw[j] = L.stage[i];
-                sd[j] = i < staged ? (uint32_t)L.dst[i] : kEmpty;
-            }
-            __syncthreads();  // the stage is read; block_append reuses it
-            if (threadIdx.x == 0) s_src.staged = 0;
-            block_append<kStageWpt>(a.q, a.pt.P, L, sw, sd, s_wsum);
-            flush_visits(a.app, ct);  // keep the visit table for the walkers to come
-        }
-        if (done) break;
-    }
-    flush_visits(a.app, ct);
+    const uint32_t n = qlen_of(a.q, src);
+    const uint64_t lo64 = (uint64_t)chunk_lo << kChunkLog2;
+    const uint64_t hi64 = min((uint64_t)n, (uint64_t)chunk_hi << kChunkLog2);
+    src_sp.nb = a.pt.node_off[src];
+    src_sp.ne = a.pt.node_off[src + 1];
+    src_sp.eb = a.pt.edge_off[src];
+    for (uint64_t base = lo64 + (uint64_t)blockIdx.x * BW; base < hi64; base += (uint64_t)gridDim.x * BW)
+        walk_pass<APP, WPT>(a, L, s_wsum, src, src_sp, base, hi64, ct);
     flush_counters(ct, a.stats);
 }
 
@@ -926,7 +905,7 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
 // read it over PCIe if it holds fewer than zc_threshold walkers, else copy it
 // into a free slot or into the slot of the resident partition with the fewest
 // walkers.  Tables are updated here; the copy is done by load_kernel next.
-__global__ __launch_bounds__(1024) void decide_kernel(Queues q, SchedState ss, Parts pt) {
+__global__ __launch_bounds__(1024) void decide_kernel(Queues q, SchedState ss, Parts pt, BlockImages bi) {
     __shared__ unsigned long long s_best[1024];
     __shared__ unsigned long long s_victim[1024];
     __shared__ uint32_t s_free[1024];
@@ -1002,6 +981,7 @@ __global__ __launch_bounds__(1024) void decide_kernel(Queues q, SchedState ss, P
         ss.resident[v] = 0;
         ss.rowptr_tab[v] = nullptr;
         ss.col_tab[v] = nullptr;
+        bi.words_tab[v] = nullptr;
         ds->victim = (int32_t)v;
     }
     uint8_t *base = ss.slots + (size_t)slot * ss.slot_bytes;
@@ -1012,23 +992,27 @@ __global__ __launch_bounds__(1024) void decide_kernel(Queues q, SchedState ss, P
     ss.resident[c] = 1;
     ss.rowptr_tab[c] = (const uint64_t *)base;
     ss.col_tab[c] = (const uint32_t *)(base + n1 * 8);
+    const uint64_t img = bi.img ? bi.size[c] : 0;  // block image bytes, 0 = raw columns
+    bi.words_tab[c] = img ? ss.col_tab[c] + 2 * ((m + kBlockEdges - 1) / kBlockEdges + 1) : nullptr;
     ds->slot = (int32_t)slot;
     ds->load = 1;
     ds->loads++;
-    ds->h2d_bytes += n1 * 8 * ss.rp_host + m * 4;
+    ds->h2d_bytes += n1 * 8 * ss.rp_host + (img ? img : m * 4);
 }
 
 // Copy the decided partition from pinned host memory into its slot: rowptr
-// slice, then col slice, as 4-byte words (grid-stride reads over PCIe).
-__global__ __launch_bounds__(256) void load_kernel(SchedState ss, Parts pt) {
+// slice, then the col slice or (block slots) the partition's block image, as
+// 4-byte words (grid-stride reads over PCIe).
+__global__ __launch_bounds__(256) void load_kernel(SchedState ss, Parts pt, BlockImages bi) {
     const DevSched *ds = ss.ds;
     if (!ds->load) return;
     const uint32_t c = (uint32_t)ds->choice;
     const uint64_t n1 = (uint64_t)pt.node_off[c + 1] - pt.node_off[c] + 1;
-    const uint64_t m = pt.edge_off[c + 1] - pt.edge_off[c];
+    const uint64_t img = bi.img ? bi.size[c] : 0;
+    const uint64_t m = img ? img / 4 : pt.edge_off[c + 1] - pt.edge_off[c];  // words after the rowptr slice
     uint32_t *dst = (uint32_t *)(ss.slots + (size_t)ds->slot * ss.slot_bytes);
     const uint32_t *src_r = (const uint32_t *)(ss.zc_rowptr + pt.node_off[c]);
-    const uint32_t *src_c = ss.zc_col + pt.edge_off[c];
+    const uint32_t *src_c = img ? (const uint32_t *)(bi.img + bi.off[c]) : ss.zc_col + pt.edge_off[c];
     const uint64_t nr = 2 * n1, total = nr + m;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride)

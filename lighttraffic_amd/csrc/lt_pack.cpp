@@ -206,7 +206,9 @@ lt_status packed_image(const Graph &g, const Partition &p, int format, std::shar
     }
     if (pk->total) {
         if (g.pinned) {
-            if (hipHostMalloc((void **)&pk->host, pk->total, hipHostMallocPortable) != hipSuccess) {
+            // mapped: the device-scheduled loop's copy kernel reads block images in place
+            if (hipHostMalloc((void **)&pk->host, pk->total, hipHostMallocMapped | hipHostMallocPortable) !=
+                hipSuccess) {
                 (void)hipGetLastError();
                 pk->host = nullptr;
                 set_error("pinned host allocation of %llu bytes for packed partitions failed",

@@ -1,0 +1,168 @@
+// lt_tune.cpp — autotuner of batched PPR (lt_tune_ppr_batch), host code over
+// the engine's own C ABI.  No counterpart in the reference, which runs one
+// source per program (src/walk/SingleSource.h:6-20) with compile-time knobs.
+//
+// Every configuration gives the same endpoints (walker ids and Philox draws
+// do not depend on it), so the tuner only trades speed against PCIe bytes.
+// It times each configuration on every sample_every-th query (a warm second
+// job), then the three fastest and the fastest of each copy mode again on a
+// sample four times larger, since a small sample walks few steps per
+// partition load and so over-weights the copies; the pick is the fastest of
+// that second stage.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "lt_internal.h"
+
+namespace {
+
+struct Sample {
+    std::vector<uint32_t> src;
+    std::vector<uint64_t> cnt;
+};
+
+Sample take_sample(uint32_t nq, const uint32_t *sources, const uint64_t *counts, uint32_t every) {
+    Sample s;
+    for (uint32_t q = 0; q < nq; q += std::max(1u, every)) {
+        s.src.push_back(sources[q]);
+        s.cnt.push_back(counts[q]);
+    }
+    return s;
+}
+
+// Running time of a warm second job of `t`'s configuration on sample `s`.
+lt_status time_trial(const lt_graph *g, const lt_partition *p, const Sample &s, float prob, lt_tune_trial &t) {
+    lt_config cfg;
+    lt_config_default(&cfg);
+    cfg.num_device_partitions = t.slots;
+    cfg.span = t.span;
+    cfg.device_schedule = t.device_schedule;
+    cfg.slice_chunks = t.slice_chunks;
+    cfg.zc_batch = t.zc_batch;
+    cfg.pack_partitions = t.pack_partitions;
+    lt_engine *e = nullptr;
+    lt_status st = lt_engine_create(g, p, &cfg, &e);
+    if (st != LT_OK) return st;
+    const uint32_t n = (uint32_t)s.src.size();
+    st = lt_run_ppr_batch(e, n, s.src.data(), s.cnt.data(), prob, nullptr);  // fills the cache
+    if (st == LT_OK) st = lt_run_ppr_batch(e, n, s.src.data(), s.cnt.data(), prob, nullptr);
+    lt_stats stats;
+    if (st == LT_OK) st = lt_engine_stats(e, &stats);
+    lt_engine_destroy(e);
+    if (st != LT_OK) return st;
+    t.cache_slots = (uint32_t)stats.cache_slots;
+    t.running_ms = stats.running_ms;
+    t.h2d_bytes = stats.h2d_bytes_explicit + stats.h2d_bytes_zerocopy;
+    t.iterations = stats.iterations;
+    return LT_OK;
+}
+
+}  // namespace
+
+extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uint32_t *sources, const uint64_t *counts,
+                                       float prob, uint64_t cache_bytes, const uint32_t *sizes_mb, uint32_t nsizes,
+                                       uint32_t sample_every, lt_tune_trial *best_out, lt_tune_trial *trials_out,
+                                       uint32_t trials_cap, uint32_t *ntrials_out) {
+    if (!g || nq == 0 || !sources || !counts || !best_out || !(prob > 0.0f) || cache_bytes == 0 ||
+        (sizes_mb && nsizes == 0)) {
+        lt::set_error("lt_tune_ppr_batch: null argument, no query, no cache, no partition size or prob <= 0");
+        return LT_E_ARG;
+    }
+    static const uint32_t kDefaultSizesMb[] = {16, 32, 64};
+    const std::vector<uint32_t> kSizesMb = sizes_mb ? std::vector<uint32_t>(sizes_mb, sizes_mb + nsizes)
+                                                    : std::vector<uint32_t>(kDefaultSizesMb, kDefaultSizesMb + 3);
+    static const int32_t kSpans[] = {LT_SPAN_PARTITION, LT_SPAN_RESIDENT};
+    static const int32_t kSlices[] = {2048, 8192};
+    static const int32_t kZcBatches[] = {1, 16};
+    static const int32_t kPacks[] = {0, 1, 2};
+    const uint32_t every = std::max(1u, sample_every);
+    const Sample s1 = take_sample(nq, sources, counts, every);
+    std::vector<lt_tune_trial> trials;
+    lt_graph *gw = const_cast<lt_graph *>(g);  // only its cache of packed images changes
+    for (uint32_t mb : kSizesMb) {
+        lt_partition *p = nullptr;
+        lt_status st = lt_partition_build(g, (uint64_t)mb << 20, &p);
+        if (st != LT_OK) return st;
+        const uint32_t P = lt_partition_count(p);
+        const uint32_t slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(P, cache_bytes / ((uint64_t)mb << 20)));
+        for (int32_t span : kSpans)
+            for (int32_t dev : {0, 1})
+                for (int32_t sl : kSlices)
+                    for (int32_t zb : kZcBatches)
+                        for (int32_t pk : kPacks) {
+                            if (dev && sl != kSlices[0]) continue;  // the device loop consumes whole queues
+                            if (zb != kZcBatches[0] && (dev || span != LT_SPAN_RESIDENT))
+                                continue;  // zero-copy batches need host scheduling and the resident span
+                            if (dev && pk == 1) continue;  // the device loop has no unpack step
+                            lt_tune_trial t{};
+                            t.stage = 1;
+                            t.partition_mb = mb;
+                            t.slots = slots;
+                            t.partitions = P;
+                            t.span = span;
+                            t.device_schedule = dev;
+                            t.slice_chunks = sl;
+                            t.zc_batch = zb;
+                            t.pack_partitions = pk;
+                            if ((st = time_trial(g, p, s1, prob, t)) != LT_OK) {
+                                lt_partition_free(p);
+                                return st;
+                            }
+                            trials.push_back(t);
+                        }
+        lt_graph_drop_packed(gw);  // the next partition size needs images of its own
+        lt_partition_free(p);
+    }
+    // stage 2: the three fastest and the fastest of each copy mode, on a sample four times larger
+    std::vector<lt_tune_trial> top(trials);
+    std::sort(top.begin(), top.end(), [](const lt_tune_trial &a, const lt_tune_trial &b) {
+        return a.running_ms < b.running_ms;
+    });
+    std::vector<lt_tune_trial> pick(top.begin(), top.begin() + std::min<size_t>(3, top.size()));
+    for (int32_t pk : kPacks) {
+        for (const lt_tune_trial &t : top) {  // sorted, so the first of a mode is its fastest
+            if (t.pack_partitions != pk) continue;
+            const bool have = std::any_of(pick.begin(), pick.end(), [&](const lt_tune_trial &x) {
+                return memcmp(&x, &t, sizeof(t)) == 0;
+            });
+            if (!have) pick.push_back(t);
+            break;
+        }
+    }
+    std::vector<lt_tune_trial> last;
+    if (every > 1) {
+        const Sample s2 = take_sample(nq, sources, counts, std::max(1u, every / 4));
+        for (uint32_t mb : kSizesMb) {
+            if (!std::any_of(pick.begin(), pick.end(), [&](const lt_tune_trial &t) { return t.partition_mb == mb; }))
+                continue;
+            lt_partition *p = nullptr;
+            lt_status st = lt_partition_build(g, (uint64_t)mb << 20, &p);
+            if (st != LT_OK) return st;
+            for (lt_tune_trial t : pick) {
+                if (t.partition_mb != mb) continue;
+                t.stage = 2;
+                if ((st = time_trial(g, p, s2, prob, t)) != LT_OK) {
+                    lt_partition_free(p);
+                    return st;
+                }
+                last.push_back(t);
+            }
+            lt_graph_drop_packed(gw);
+            lt_partition_free(p);
+        }
+    } else {
+        last = trials;
+    }
+    const lt_tune_trial *best = &last[0];
+    for (const lt_tune_trial &t : last)
+        if (t.running_ms < best->running_ms) best = &t;
+    *best_out = *best;
+    trials.insert(trials.end(), last.begin(), last.end());
+    if (every <= 1) trials.resize(trials.size() - last.size());  // stage 1 was the last stage
+    if (trials_out) memcpy(trials_out, trials.data(), sizeof(lt_tune_trial) * std::min<size_t>(trials_cap, trials.size()));
+    if (ntrials_out) *ntrials_out = (uint32_t)trials.size();
+    return LT_OK;
+}

@@ -81,6 +81,22 @@ def test_ppr_and_genericwalk_cli(graph_file):
     assert re.search(r"Running Time: [0-9.]+ ms", out)
 
 
+def test_ppr_queries_autotuned(graph_file, tmp_path):
+    """ppr --queries with --tune: lt_tune_ppr_batch picks the partition size
+    and engine configuration, and every walker's endpoint is counted."""
+    g = lt.Graph.open(graph_file)
+    src, cnt = g.ppr_queries(64, lo=100, hi=5000, seed=3)
+    qfile = tmp_path / "queries.txt"
+    qfile.write_text("".join(f"{s} {c}\n" for s, c in zip(src, cnt)))
+    run("pagerank", "-f", graph_file, "-m", "-s", 1)
+    out = run("ppr", "-f", graph_file, "-p", 4, "--queries", qfile, "--tune")
+    m = re.search(r"autotuned: (\d+) MB partitions, (\d+) cache slots, span (\w+), (\w+) scheduling", out)
+    assert m, out
+    assert int(m.group(1)) in (16, 32, 64) and m.group(3) in ("partition", "resident")
+    assert "result correct" in out
+    assert grep_num(r"sum: \d+, (\d+)", out) == int(cnt.sum())
+
+
 def test_learned_scheduler_from_saved_scorer(graph_file, tmp_path):
     g = lt.Graph.open(graph_file)
     part = lt.Partition.load(graph_file)

@@ -99,6 +99,10 @@ CONFIGS = [
     ("blocks-rp-resident", 16 << 10, {"num_device_partitions": 6, "zerocopy_threshold": 4000, "rowptr_resident": 1,
                                       "pack_partitions": 2}),
     ("blocks-spill", 16 << 10, {"num_device_partitions": 4, "num_device_walkers": 5000, "pack_partitions": 2}),
+    # the device-scheduled loop copying block images into block slots
+    ("blocks-device-sched", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "pack_partitions": 2}),
+    ("blocks-device-sched-zc", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "pack_partitions": 2,
+                                          "zerocopy_threshold": 2000, "span": "resident"}),
 ]
 
 
@@ -238,10 +242,12 @@ def test_autotuner_picks_a_trial():
     best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=48 << 20, sample_every=4, sizes_mb=(16,))
     # host scheduling: (span partition x 2 slices + span resident x 2 slices x 2 batch
     # sizes) x raw, gap-coded or block copies; device scheduling: 2 spans (whole
-    # queues, raw copies)
+    # queues) x raw or block copies
     first = [t for t in trials if t["stage"] == 1]
     refined = [t for t in trials if t["stage"] == 2]
-    assert len(first) == 3 * (2 + 4) + 2
+    assert len(first) == 3 * (2 + 4) + 2 * 2
+    assert {(t["span"], t["pack_partitions"]) for t in first if t["device_schedule"]} == {
+        ("partition", 0), ("partition", 2), ("resident", 0), ("resident", 2)}
     keys = ("partition_mb", "span", "device_schedule", "slice_chunks", "zc_batch", "pack_partitions")
     want = {tuple(t[k] for k in keys) for t in sorted(first, key=lambda t: t["running_ms"])[:3]}
     for pk in (0, 1, 2):
@@ -250,6 +256,7 @@ def test_autotuner_picks_a_trial():
     assert sorted(tuple(t[k] for k in keys) for t in refined) == sorted(want)
     assert {t["pack_partitions"] for t in refined} == {0, 1, 2}
     assert best in refined and best["running_ms"] == min(t["running_ms"] for t in refined)
+    assert all(t["cache_slots"] >= t["slots"] and t["iterations"] > 0 for t in trials)
     assert {t["zc_batch"] for t in first} == {1, 16}
     assert {t["pack_partitions"] for t in first} == {0, 1, 2}
 
@@ -310,6 +317,7 @@ def test_generator_dedupe_seams(monkeypatch):
 
 
 def _many_partition_graph():
+    """60,000 vertices of degree 0-8: small partition sizes give thousands of partitions."""
     rng = np.random.default_rng(21)
     V = 60000
     deg = rng.integers(0, 9, V)
@@ -318,17 +326,20 @@ def _many_partition_graph():
     return lt.Graph.from_csr(rowptr, col)
 
 
+@pytest.mark.parametrize("S", [600, 150])
 @pytest.mark.parametrize("kw", [{"num_device_partitions": 600},
                                 {"num_device_partitions": 600, "span": "resident", "zerocopy_threshold": 40,
                                  "zc_batch": 16},
                                 {"num_device_partitions": 600, "device_schedule": 1}])
-def test_near_the_partition_limit(kw):
-    """About 2,400 partitions (the append stage's LDS histogram is sized by P;
-    the engine limit is 2,900) with a capped cache."""
+def test_many_partitions(S, kw):
+    """About 2,400 and 12,000 partitions (above 1,024 the append stage numbers
+    each pass's destinations through an LDS hash table; the reference's
+    constant table holds 12,287, src/partition/PartitionStrategy.h:13) with
+    a capped cache."""
     g = _many_partition_graph()
-    part = lt.Partition.build(g, 600)
+    part = lt.Partition.build(g, S)
     log(f"P={part.count} {kw}")
-    assert 2000 < part.count <= 2900
+    assert part.count > 2000 if S == 600 else part.count > 11000
     eng = lt.Engine(g, part, **kw)
     w = 100000
     counts, ends = eng.pagerank(w, 8, endpoints=True)
@@ -338,15 +349,6 @@ def test_near_the_partition_limit(kw):
     assert np.array_equal(counts, oracle.ppr(g.rowptr(), g.col(), w, source=5)[0])
     ends, _ = eng.genericwalk(w, 12)
     assert np.array_equal(ends, oracle.genericwalk(g.rowptr(), g.col(), w, 12)[0])
-
-
-def test_more_partitions_than_the_limit_is_an_error():
-    g = _many_partition_graph()
-    part = lt.Partition.build(g, 400)
-    assert part.count > 2900
-    with pytest.raises(lt.LTError) as e:
-        lt.Engine(g, part)
-    assert e.value.status == 6  # LT_E_ARG
 
 
 def test_c5_full_size():

@@ -152,6 +152,20 @@ def test_partition_file_layout_and_roundtrip(tmp_path):
     assert np.array_equal(q.edge_offsets(), p.edge_offsets())
 
 
+def test_more_than_65535_partitions_is_an_error():
+    """Partition ids are u16 in .part files (src/partition/PartitionStrategy.h:76-97)
+    and in the engine's append stage: a split into more is refused."""
+    rng = np.random.default_rng(22)
+    V = 200000
+    deg = rng.integers(0, 5, V)
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    g = lt.Graph.from_csr(rowptr, rng.integers(0, V, int(rowptr[-1])).astype(np.uint32))
+    assert lt.Partition.build(g, 1000).count <= 65535
+    with pytest.raises(lt.LTError) as e:
+        lt.Partition.build(g, 40)
+    assert e.value.status == 6
+
+
 def test_partition_errors(tmp_path):
     rowptr = np.array([0, 3, 1003, 1005], np.uint64)
     col = np.zeros(1005, np.uint32)
