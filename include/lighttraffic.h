@@ -375,6 +375,15 @@ lt_status lt_scorer_loss_grad(lt_scorer *s, uint64_t n, const uint32_t *counts, 
 /* Scores (f32[P*2]) of one decision. */
 lt_status lt_scorer_infer(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const int32_t *recent,
                           int32_t cur, float w_total, float *out);
+/* lt_scorer_infer for callers that cannot allocate (code under HIP graph
+ * capture, a scheduler's tight decision loop): no counterpart in the
+ * reference.  Same scores as lt_scorer_infer, computed through a static
+ * per-GPU scratch (about 1.2 MB, held until the process ends) instead of the
+ * scorer's staging buffers; calls no allocating routine.  Scorers of at most
+ * 65,535 partitions (LT_E_ARG beyond).  Calls on one GPU share its scratch:
+ * calls that overlap on one GPU (from several threads) race. */
+lt_status lt_scorer_infer_noalloc(lt_scorer *s, const uint32_t *counts, const uint8_t *resident,
+                                  const int32_t *recent, int32_t cur, float w_total, float *out);
 /* FP32 master parameters: W1[16][256] b1[256] W2[256][256] b2[256] W3[256][2] b3[2]. */
 lt_status lt_scorer_get_params(const lt_scorer *s, float *out);
 lt_status lt_scorer_set_params(lt_scorer *s, const float *in);

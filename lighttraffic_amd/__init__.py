@@ -136,6 +136,7 @@ _sig("lt_scorer_train", ctypes.c_int, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _
      ctypes.c_float, _u64, _vp)
 _sig("lt_scorer_loss_grad", ctypes.c_int, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("lt_scorer_infer", ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, _vp)
+_sig("lt_scorer_infer_noalloc", ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, _vp)
 _sig("lt_scorer_get_params", ctypes.c_int, _vp, _vp)
 _sig("lt_scorer_set_params", ctypes.c_int, _vp, _vp)
 _sig("lt_scorer_save", ctypes.c_int, _vp, ctypes.c_char_p)
@@ -524,6 +525,18 @@ class Scorer:
         out = np.zeros((self.P, 2), np.float32)
         _check(_lib.lt_scorer_infer(self._h, counts.ctypes.data, resident.ctypes.data, recent.ctypes.data, cur,
                                     w_total, out.ctypes.data), "lt_scorer_infer")
+        return out
+
+    def infer_noalloc(self, counts, resident, recent, cur, w_total, out=None):
+        """infer() through the GPU's static scratch (lt_scorer_infer_noalloc);
+        `out` (float32 [P, 2]) may be given so that nothing is allocated."""
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        resident = np.ascontiguousarray(resident, dtype=np.uint8)
+        recent = np.ascontiguousarray(recent, dtype=np.int32)
+        if out is None:
+            out = np.zeros((self.P, 2), np.float32)
+        _check(_lib.lt_scorer_infer_noalloc(self._h, counts.ctypes.data, resident.ctypes.data, recent.ctypes.data,
+                                            cur, w_total, out.ctypes.data), "lt_scorer_infer_noalloc")
         return out
 
     def get_params(self):

@@ -15,12 +15,20 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <vector>
 
 #include "lt_internal.h"
 #include "lt_mlp.h"
+
+#define LT_NA_TRY(expr)             \
+    do {                            \
+        lt_status _s = (expr);      \
+        if (_s != LT_OK) return _s; \
+    } while (0)
 
 #define MLP_HIP(call)                                                                                  \
     do {                                                                                               \
@@ -456,6 +464,7 @@ struct Scorer {
     uint16_t *Wb = nullptr;
     uint16_t *Wi = nullptr;     // inference copy of W1, W2 (output-major)
     uint64_t wver = 0, wi_ver = ~0ull;  // bumped on every weight update; Wi is valid for wi_ver
+    uint64_t uid = 0;                    // unique per scorer created in this process
     uint64_t adam_t = 0;
     bool trained = false;
     // training buffers (capacity in decisions)
@@ -491,8 +500,11 @@ Scorer::~Scorer() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
+static std::atomic<uint64_t> g_scorer_uid{0};
+
 lt_status Scorer::init(uint32_t P_, const uint32_t *node_off, const uint64_t *edge_off, uint64_t seed) {
     P = P_;
+    uid = ++g_scorer_uid;
     MLP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     const double V = std::max<double>(1.0, node_off[P]), E = std::max<double>(1.0, (double)edge_off[P]);
     h_vsh.resize(P);
@@ -628,6 +640,52 @@ lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float 
     }
     MLP_HIP(e);
     memcpy(out, s->h_Y, sizeof(float) * P * kOut);
+    return LT_OK;
+}
+
+// ---- inference without allocation ----------------------------------------------
+// Scratch of lt_scorer_infer_noalloc, one copy per GPU, held until the
+// process ends: one decision's packed state (the layout infer_kernel reads),
+// its scores, and the output-major copy of the first two layers' weights.
+// Sized for kNoAllocMaxP partitions (about 1.2 MB).  In host code the name
+// `scratch` is only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr uint32_t kNoAllocMaxP = 65535;  // the largest partition count a .part file holds
+constexpr size_t kNaStateBytes = ((((size_t)kNoAllocMaxP * 5 + 7) & ~(size_t)7) + 4 * (kRecent + 2) + 255) & ~(size_t)255;
+constexpr size_t kNaYBytes = ((sizeof(float) * kNoAllocMaxP * kOut) + 255) & ~(size_t)255;
+constexpr size_t kNaWiBytes = ((sizeof(uint16_t) * kWiSize) + 255) & ~(size_t)255;
+static __device__ __align__(256) uint8_t scratch[kNaStateBytes + kNaYBytes + kNaWiBytes];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+struct NoAllocDevice {
+    uint8_t *base = nullptr;          // device address of this GPU's scratch
+    uint64_t wi_uid = 0;  // scorer (Scorer::uid) whose weights the scratch's inference copy holds
+    uint64_t wi_ver = 0;  // and their version
+};
+NoAllocDevice g_noalloc[kNaMaxDevices];
+std::mutex g_noalloc_mu;
+}  // namespace
+
+// This GPU's scratch address, looked up on the first call made on it.
+static lt_status noalloc_scratch(int dev, NoAllocDevice **out) {
+    if (dev < 0 || dev >= kNaMaxDevices) {
+        set_error("lt_scorer_infer_noalloc: device %d beyond the %d supported", dev, kNaMaxDevices);
+        return LT_E_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocDevice &d = g_noalloc[dev];
+    if (!d.base) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error("lt_scorer_infer_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return LT_E_HIP;
+        }
+        d.base = (uint8_t *)ptr;
+    }
+    *out = &d;
     return LT_OK;
 }
 
@@ -769,6 +827,50 @@ lt_status lt_scorer_infer(lt_scorer *s, const uint32_t *counts, const uint8_t *r
     d.cur = cur;
     d.w_total = w_total;
     return scorer_infer(s, d, s->stream, out);
+}
+
+// The scores of lt_scorer_infer, computed through the GPU's static scratch
+// instead of the scorer's staging buffers: state copied in, infer_kernel,
+// scores copied out, all on the scorer's stream; no allocation of any kind.
+lt_status lt_scorer_infer_noalloc(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const int32_t *recent,
+                                  int32_t cur, float w_total, float *out) {
+    if (!s || !counts || !resident || !recent || !out) return LT_E_ARG;
+    const uint32_t P = s->P;
+    if (P > kNoAllocMaxP) {
+        set_error("lt_scorer_infer_noalloc: %u partitions exceed the scratch's %u", P, kNoAllocMaxP);
+        return LT_E_ARG;
+    }
+    int dev = 0;
+    MLP_HIP(hipGetDevice(&dev));
+    NoAllocDevice *nd = nullptr;
+    LT_NA_TRY(noalloc_scratch(dev, &nd));
+    uint8_t *state = nd->base, *Y = nd->base + kNaStateBytes, *Wi = nd->base + kNaStateBytes + kNaYBytes;
+    const size_t moff = ((size_t)P * 5 + 7) & ~(size_t)7;
+    int32_t meta[kRecent + 2];
+    for (int k = 0; k < kRecent; k++) meta[k] = recent[k];
+    meta[kRecent] = cur;
+    memcpy(&meta[kRecent + 1], &w_total, 4);
+    hipStream_t st = s->stream;
+    MLP_HIP(hipMemcpyAsync(state, counts, 4 * (size_t)P, hipMemcpyHostToDevice, st));
+    MLP_HIP(hipMemcpyAsync(state + 4 * (size_t)P, resident, P, hipMemcpyHostToDevice, st));
+    MLP_HIP(hipMemcpyAsync(state + moff, meta, sizeof(meta), hipMemcpyHostToDevice, st));
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (nd->wi_uid != s->uid || nd->wi_ver != s->wver) {  // the scratch holds other weights
+            transpose_infer_weights_kernel<<<(unsigned)((kWiSize + 255) / 256), 256, 0, st>>>(s->Wb,
+                                                                                              (uint16_t *)Wi);
+            MLP_HIP(hipGetLastError());
+            nd->wi_uid = s->uid;
+            nd->wi_ver = s->wver;
+        }
+    }
+    const size_t bytes = moff + 4 * (kRecent + 2);
+    infer_kernel<<<(P + 7) / 8, 256, (bytes + 15) & ~(size_t)15, st>>>(P, state, s->d_vsh, s->d_esh, s->Wb,
+                                                                         (const uint16_t *)Wi, (float *)Y);
+    MLP_HIP(hipGetLastError());
+    MLP_HIP(hipMemcpyAsync(out, Y, sizeof(float) * P * kOut, hipMemcpyDeviceToHost, st));
+    MLP_HIP(hipStreamSynchronize(st));
+    return LT_OK;
 }
 
 uint64_t lt_scorer_num_params(void) { return kParams; }
