@@ -46,6 +46,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "walk steps/s & H2D bytes/step: learned vs plain scheduler, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+# Philox key of the walks the scorer is trained on (the timed walks use the
+# engine's default key, 0x4C54524B): training and timing see different jobs
+TRAIN_SEED = 0x4C54524C
 
 WORKLOADS = {
     "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10, zc=0),
@@ -178,7 +181,7 @@ def belady_bound(eng, part):
     processing order; the H2D floor scales the job's explicit bytes by them."""
     import lighttraffic_amd as lt
     tr = eng.trace()
-    _, resident, _ = eng.trace_state()
+    _, resident, _, _ = eng.trace_state()
     st = eng.stats()
     seq = [r[0] for r in tr if r[1] in (0, 2, 3)]
     loads = st["graph_loading_calls"]
@@ -295,19 +298,13 @@ def main():
         # copy and compute streams share one loses its copy/compute overlap.
         sc = lt.Scorer(part, seed=1)
         plain = {}
-        prep_s = 0.0
         keep = ("steps_per_s", "ms_per_step", "median_ms_per_step", "e2e_ms_per_step", "h2d_bytes_per_step",
                 "h2d_GB_per_job", "pcie_GBps")
         for span in ("partition", "resident"):
-            eng = lt.Engine(g, part, span=span, record_trace=1, **base)
+            eng = lt.Engine(g, part, span=span, **base)
             plain[span] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}")
-            t1 = time.perf_counter()
-            if span == "partition":
-                n_next = sc.add_trace(eng, use_next=True, use_keep=True)
-            else:
-                n_res = sc.add_trace(eng, use_next=True, use_keep=False)
+            if span == "resident":
                 ref = eng.pagerank(w, wl["l"])
-            prep_s += time.perf_counter() - t1
             eng.close()
             # the same rules with packed partition copies: gap-coded and unpacked
             # in HBM (1), or blocks kept packed in the slots (2)
@@ -331,26 +328,53 @@ def main():
         eng = lt.Engine(g, part, span="resident", policy="learned", prefetch_plain=1, pack_partitions=2, **base)
         plain["prefetch_rule_blocks"] = measure(eng, wl, w, args.warmup, args.steps, "plain-rule prefetch blocks")
         eng.close()
+        # Training traces: the plain rules with the learned configuration's other
+        # settings, on walks of another Philox key (held out from the timed
+        # jobs), warm jobs only (the first fills an empty cache and evicts
+        # nothing).  Labels: the next partition needing host data (output 0)
+        # and Belady's next-use quantity (output 1, which picks victims;
+        # prefetch targets follow the plain rule).
+        t1 = time.perf_counter()
+        eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
+                        pack_partitions=2, record_trace=1, seed=TRAIN_SEED, **base)
+        n_train = 0
+        for i in range(4):
+            run_job(eng, wl, w)
+            if i > 0:
+                n_train += sc.add_trace(eng, use_next=True, use_keep=2)
+        eng.close()
+        prep_s = time.perf_counter() - t1
         t1 = time.perf_counter()
         loss = sc.fit(epochs=args.epochs, batch=32, lr=1e-3)
-        train_s = time.perf_counter() - t1 + prep_s
-        log(f"scorer trained on {n_next + n_res} decisions in {train_s:.1f} s, losses {loss.tolist()}")
-        eng = lt.Engine(g, part, span="resident", policy="learned", **base)
+        train_s = time.perf_counter() - t1
+        log(f"scorer trained on {n_train} decisions in {train_s:.1f} s (traces {prep_s:.1f} s), "
+            f"losses {loss.tolist()}")
+        lcfg = dict(span="resident", policy="learned", prefetch_plain=2)
+        eng = lt.Engine(g, part, **lcfg, **base)
         eng.set_scorer(sc)
         learned_raw = measure(eng, wl, w, args.warmup, args.steps, "learned/resident")
         eng.close()
-        eng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=1, **base)
+        eng = lt.Engine(g, part, pack_partitions=1, **lcfg, **base)
         eng.set_scorer(sc)
         learned_gap = measure(eng, wl, w, args.warmup, args.steps, "learned/resident packed")
         eng.close()
-        eng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=2, record_trace=1, **base)
+        # what the network's victim choice adds: the same loop evicting the
+        # least recently walked candidate (shortlist 1), and the network
+        # choosing among the 4 least recently walked (shortlist 4)
+        ablate = {}
+        for k, tag in ((1, "lru_only"), (4, "shortlist4")):
+            eng = lt.Engine(g, part, pack_partitions=2, evict_shortlist=k, **lcfg, **base)
+            eng.set_scorer(sc)
+            ablate[tag] = measure(eng, wl, w, args.warmup, args.steps, f"learned blocks, shortlist {k}")
+            eng.close()
+        eng = lt.Engine(g, part, pack_partitions=2, record_trace=1, **lcfg, **base)
         eng.set_scorer(sc)
         learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident blocks")
         belady["learned"] = belady_bound(eng, part)
         same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
         slots = eng.stats()["cache_slots"]
         eng.close()
-        teng = lt.Engine(g, part, span="resident", policy="learned", pack_partitions=2, event_timers=1, **base)
+        teng = lt.Engine(g, part, pack_partitions=2, event_timers=1, **lcfg, **base)
         teng.set_scorer(sc)
         learned.update(timed_walk(teng, wl, w))
         teng.close()
@@ -366,9 +390,12 @@ def main():
             "ms_per_step": learned["ms_per_step"],
             "median_ms_per_step": learned["median_ms_per_step"],
             "e2e_ms_per_step": learned["e2e_ms_per_step"],
-            "config": {"workload": desc, "scheduler": "learned (prefetch depth 2, span resident, zero-copy "
-                                                      "threshold S/128, zero-copy batches of 16), partitions "
-                                                      "kept packed in HBM (16-edge frame-of-reference blocks, "
+            "config": {"workload": desc, "scheduler": "learned (victims: fewest queued walkers, then the BF16 "
+                                                      "MLP's lowest keep score; "
+                                                      "prefetch depth 2 of the largest queues needing host data; "
+                                                      "span resident, zero-copy threshold S/128, zero-copy "
+                                                      "batches of 16), partitions kept packed in HBM (16-edge "
+                                                      "frame-of-reference blocks, "
                                                       f"{slots} cache slots in the 8 GB)",
                        "device": info["name"]},
             "h2d_bytes_per_step": learned["h2d_bytes_per_step"],
@@ -382,15 +409,21 @@ def main():
                       "span_resident_S128_zcbatch16_blocks": {k: plain["tuned_blocks"][k] for k in keep},
                       "plain_rule_prefetch_blocks": {k: plain["prefetch_rule_blocks"][k] for k in keep}},
             "learned": {"blocks": {k: learned[k] for k in keep}, "packed": {k: learned_gap[k] for k in keep},
-                        "unpacked": {k: learned_raw[k] for k in keep}},
+                        "unpacked": {k: learned_raw[k] for k in keep},
+                        "blocks_lru_only": {k: ablate["lru_only"][k] for k in keep},
+                        "blocks_shortlist4": {k: ablate["shortlist4"][k] for k in keep}},
             "learned_vs_plain": {
                 # the headline pair: the plain rules with every non-learned setting of the
                 # learned configuration (span, zero-copy threshold and batches, block copies)
                 "like_for_like": dict(vs(plain["tuned_blocks"]), plain="span_resident_S128_zcbatch16_blocks",
                                       target={"speedup": 1.3, "h2d_bytes_ratio": 0.75}),
                 "vs_plain_rule_prefetch": dict(vs(plain["prefetch_rule_blocks"]), note=(
-                    "the same prefetching loop with the largest queue as target and the plain victim rule: "
-                    "what the network itself adds")),
+                    "the same prefetching loop with the same targets and the plain victim rule")),
+                "vs_lru_only": dict(vs(ablate["lru_only"]), note=(
+                    "the same loop evicting the least recently walked fewest-walker candidate: what the "
+                    "network's victim choice adds")),
+                "vs_shortlist4": dict(vs(ablate["shortlist4"]), note=(
+                    "the network choosing among the 4 least recently walked fewest-walker candidates")),
                 "vs_reference_plain": vs(ref_plain),
                 "vs_plain_same_copies": {k: vs(v) for k, v in same_copy.items()},
                 "unpacked_learned_vs_reference_plain": {
@@ -404,7 +437,10 @@ def main():
                 "visit_counts_identical": same,
                 "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
                 "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],
-                "scorer_training_s": train_s, "training_decisions": n_next + n_res},
+                "scorer_training_s": train_s, "training_decisions": n_train,
+                "training_walks": f"3 warm pagerank jobs of the plain tuned_blocks configuration, Philox key "
+                                  f"{TRAIN_SEED:#x} (timed jobs: 0x4c54524b)",
+                "labels": "next partition needing host data (output 0), Belady next-use (output 1, use_keep=2)"},
             "roofline": roofline(learned, "c5", "resident"),
             "roofline_pcie": roofline_pcie(learned, h2d_peak),
             "stats_per_step": learned["per_job"],
@@ -420,7 +456,7 @@ def main():
         part = lt.Partition.build(g, best["partition_mb"] << 20)
         cfg = dict(num_device_partitions=best["slots"], span=best["span"], device_schedule=best["device_schedule"],
                    slice_chunks=best["slice_chunks"], zc_batch=best["zc_batch"],
-                   pack_partitions=best["pack_partitions"])
+                   pack_partitions=best["pack_partitions"], walkers_per_thread=best["walkers_per_thread"])
         eng = lt.Engine(g, part, **cfg)
         # walk-kernel time for the roofline: the same partitioning, host-scheduled, event timers on
         teng = lt.Engine(g, part, **dict(cfg, device_schedule=0), event_timers=1)
@@ -432,7 +468,8 @@ def main():
                 f"{best['cache_slots']} cache slots, span {best['span']}, "
                 f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}, "
                 f"zero-copy batch {best['zc_batch']}, "
-                f"{('raw', 'gap-coded', 'block')[best['pack_partitions']]} copies")
+                f"{('raw', 'gap-coded', 'block')[best['pack_partitions']]} copies, walkers per lane "
+                f"{best['walkers_per_thread'] or 'by launch size'}")
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "median_ms_per_step": m["median_ms_per_step"], "e2e_ms_per_step": m["e2e_ms_per_step"],
                     "config": {"workload": desc, "scheduler": "greedy", "device": info["name"]},

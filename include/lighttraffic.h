@@ -96,11 +96,25 @@ typedef struct {
                                        partition that does not get smaller is
                                        copied as is (default 0: off;
                                        device_schedule: 2 only, 1 is ignored)   */
-    int32_t  prefetch_plain;        /* LEARNED: 1 = run the learned loop without
-                                       its scorer: prefetch the largest queues
-                                       that need host data and evict by the
-                                       plain rule (an ablation that shows what
-                                       the network adds; default 0)            */
+    int32_t  prefetch_plain;        /* LEARNED: prefetch targets and victims.
+                                       0: the scorer's "next" output picks the
+                                       targets, its "keep" output the victims;
+                                       1: no scorer: the largest queues that
+                                       need host data are the targets and the
+                                       plain rule picks victims (an ablation
+                                       that shows what the network adds); 2:
+                                       largest-queue targets, the scorer's
+                                       victims (default 0)                     */
+    int32_t  evict_shortlist;       /* LEARNED victims: the scorer chooses among
+                                       this many least recently walked of the
+                                       resident partitions with the fewest
+                                       queued walkers (0 = among all of them;
+                                       1 = least recently walked, no scorer
+                                       choice; default 0)                      */
+    int32_t  walkers_per_thread;    /* walkers one walk-kernel lane holds: 0 =
+                                       by launch size (4, 2 or 1, host
+                                       scheduling; 1 in the device loop), or
+                                       exactly 1, 2 or 4                        */
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines
@@ -292,11 +306,15 @@ lt_status lt_make_ppr_queries(const lt_graph *g, uint32_t nq, uint64_t lo, uint6
  * batches of 1 or 16 (host scheduling, span resident), raw / gap-coded /
  * block copies (gap-coded needs host scheduling).  Then it times the three
  * fastest and the fastest of each copy mode again on a sample four times
- * larger, and returns the fastest of those in *best_out.  Every trial (both
- * stages) is written to trials_out (up to trials_cap; *ntrials_out = their
- * number).  Results of the batch do not depend on the configuration. */
+ * larger, then the fastest of those with each fixed number of walkers per
+ * walk-kernel lane (1, 2, 4) on that sample, and returns the fastest of
+ * stages 2 and 3 in *best_out.  Every trial is written to trials_out (up to
+ * trials_cap; *ntrials_out = their number).  The queue chunk (4096 walkers)
+ * is fixed at compile time; slice_chunks is the launch size searched.
+ * Results of the batch do not depend on the configuration. */
 typedef struct {
-    uint32_t stage;            /* 1: timed on the sample, 2: again on the larger one */
+    uint32_t stage;            /* 1: timed on the sample, 2: again on the larger
+                                  one, 3: stage 2's pick with a fixed WPT        */
     uint32_t partition_mb;     /* partition size S in MB                           */
     uint32_t slots;            /* num_device_partitions = cache_bytes / S (<= P)   */
     uint32_t partitions;       /* P at this size                                   */
@@ -306,6 +324,7 @@ typedef struct {
     int32_t  slice_chunks;
     int32_t  zc_batch;
     int32_t  pack_partitions;
+    int32_t  walkers_per_thread;
     double   running_ms;       /* the warm job's scheduler-loop time               */
     uint64_t h2d_bytes;        /* its explicit + zero-copied H2D bytes             */
     uint64_t iterations;       /* its scheduler iterations                         */
@@ -329,19 +348,25 @@ typedef struct {
 uint64_t lt_engine_trace_len(const lt_engine *e);
 lt_status lt_engine_trace(const lt_engine *e, lt_trace_rec *out, uint64_t cap);
 /* State seen by each traced decision: queue lengths (u32[len*P]), residency
- * (u8[len*P]) and the last 8 loaded partitions (i32[len*8], most recent
- * first, -1 = none); any output may be NULL.  cap counts records. */
+ * (u8[len*P]), partition ages (u32[len*P]: walks of resident partitions since
+ * the partition's own last one, counted from the engine's creation) and the
+ * last 8 loaded partitions (i32[len*8], most recent first, -1 = none); any
+ * output may be NULL.  cap counts records. */
 lt_status lt_engine_trace_state(const lt_engine *e, uint32_t *counts_out, uint8_t *resident_out,
-                                int32_t *recent_out, uint64_t cap);
+                                uint32_t *age_out, int32_t *recent_out, uint64_t cap);
 
 /* ---- learned scheduler (no counterpart in the reference) ---------------- */
 /* Per-partition MLP, 16 features -> 256 -> 256 -> 2, BF16 weights, FP32
- * accumulation, trained and evaluated by HIP kernels.  Output 0 scores the
+ * accumulation, trained and evaluated by HIP kernels.  Features: queue length
+ * (three scalings), residency, chosen now, loaded 0..6 decisions ago, age
+ * a/(a+P), vertex and edge shares, queued walkers / W.  Output 0 scores the
  * partition the scheduler will need from the host next (used to prefetch it
- * while the current partition is walked); output 1 predicts the partition's
- * share of future walker traffic (the resident partition with the lowest
- * value is evicted).  Engines use it when cfg.policy == LT_SCHED_LEARNED and
- * a trained scorer is attached; otherwise they fall back to the plain rules. */
+ * while the current partition is walked); output 1 predicts how soon the
+ * partition is needed in HBM again (use_keep = 2) or its share of future
+ * walker traffic (use_keep = 1); among the resident partitions with the
+ * fewest queued walkers the one with the lowest value is evicted.  Engines
+ * use it when cfg.policy == LT_SCHED_LEARNED and a trained scorer is
+ * attached; otherwise they fall back to the plain rules. */
 lt_status lt_scorer_create(const lt_partition *p, uint64_t seed, lt_scorer **out);
 void lt_scorer_free(lt_scorer *s);
 uint64_t lt_scorer_num_params(void);
@@ -362,32 +387,34 @@ lt_status lt_scorer_fit(lt_scorer *s, uint32_t epochs, uint32_t batch, float lr,
  * lt_engine_trace_state); keep_label f32[n*P] may be NULL; wa/wb are the
  * per-decision loss weights of the two outputs (NULL = 1 and 0). */
 lt_status lt_scorer_train(lt_scorer *s, uint64_t n, const uint32_t *counts, const uint8_t *resident,
-                          const int32_t *recent, const int32_t *cur, const float *w_total,
+                          const uint32_t *age, const int32_t *recent, const int32_t *cur, const float *w_total,
                           const int32_t *next_label, const float *keep_label, const float *wa,
                           const float *wb, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
                           float *loss_out);
 /* One forward + backward without an update: losses (f32[2]) and the
  * gradient w.r.t. the parameters (f32[lt_scorer_num_params()]). */
 lt_status lt_scorer_loss_grad(lt_scorer *s, uint64_t n, const uint32_t *counts, const uint8_t *resident,
-                              const int32_t *recent, const int32_t *cur, const float *w_total,
+                              const uint32_t *age, const int32_t *recent, const int32_t *cur, const float *w_total,
                               const int32_t *next_label, const float *keep_label, const float *wa,
                               const float *wb, float *loss_out, float *grad_out);
 /* Scores (f32[P*2]) of one decision. */
-lt_status lt_scorer_infer(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const int32_t *recent,
-                          int32_t cur, float w_total, float *out);
+lt_status lt_scorer_infer(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const uint32_t *age,
+                          const int32_t *recent, int32_t cur, float w_total, float *out);
 /* lt_scorer_infer for callers that cannot allocate (code under HIP graph
  * capture, a scheduler's tight decision loop): no counterpart in the
  * reference.  Same scores as lt_scorer_infer, computed through a static
- * per-GPU scratch (about 1.2 MB, held until the process ends) instead of the
+ * per-GPU scratch (about 1.4 MB, held until the process ends) instead of the
  * scorer's staging buffers; calls no allocating routine.  Scorers of at most
  * 65,535 partitions (LT_E_ARG beyond).  Calls on one GPU share its scratch:
  * calls that overlap on one GPU (from several threads) race. */
 lt_status lt_scorer_infer_noalloc(lt_scorer *s, const uint32_t *counts, const uint8_t *resident,
-                                  const int32_t *recent, int32_t cur, float w_total, float *out);
+                                  const uint32_t *age, const int32_t *recent, int32_t cur, float w_total,
+                                  float *out);
 /* FP32 master parameters: W1[16][256] b1[256] W2[256][256] b2[256] W3[256][2] b3[2]. */
 lt_status lt_scorer_get_params(const lt_scorer *s, float *out);
 lt_status lt_scorer_set_params(lt_scorer *s, const float *in);
-/* Weights file: "LTMLP1\0\0", u32 16, 256, 2, 0, then the BF16 parameters. */
+/* Weights file: "LTMLP2\0\0", u32 16, 256, 2, 0, then the BF16 parameters
+ * (files of the earlier feature set, "LTMLP1", are refused). */
 lt_status lt_scorer_save(const lt_scorer *s, const char *path);
 lt_status lt_scorer_load(lt_scorer *s, const char *path);
 lt_status lt_engine_set_scorer(lt_engine *e, lt_scorer *s);

@@ -51,7 +51,8 @@ class Config(ctypes.Structure):
                 ("seed", _u64), ("policy", _i32), ("pipeline", _i32), ("span", _i32), ("event_timers", _i32),
                 ("record_trace", _i32), ("prefetch_depth", _i32), ("prefetch_stick_pct", _i32),
                 ("device_schedule", _i32), ("slice_chunks", _i32), ("zc_batch", _i32),
-                ("rowptr_resident", _i32), ("pack_partitions", _i32), ("prefetch_plain", _i32)]
+                ("rowptr_resident", _i32), ("pack_partitions", _i32), ("prefetch_plain", _i32),
+                ("evict_shortlist", _i32), ("walkers_per_thread", _i32)]
 
 
 class Stats(ctypes.Structure):
@@ -125,18 +126,18 @@ _sig("lt_engine_stats", ctypes.c_int, _vp, ctypes.POINTER(Stats))
 _sig("lt_make_ppr_queries", ctypes.c_int, _vp, _u32, _u64, _u64, _u64, _vp, _vp)
 _sig("lt_engine_trace_len", _u64, _vp)
 _sig("lt_engine_trace", ctypes.c_int, _vp, ctypes.POINTER(TraceRec), _u64)
-_sig("lt_engine_trace_state", ctypes.c_int, _vp, _vp, _vp, _vp, _u64)
+_sig("lt_engine_trace_state", ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _u64)
 _sig("lt_engine_set_scorer", ctypes.c_int, _vp, _vp)
 _sig("lt_scorer_create", ctypes.c_int, _vp, _u64, ctypes.POINTER(_vp))
 _sig("lt_scorer_free", None, _vp)
 _sig("lt_scorer_num_params", _u64)
 _sig("lt_scorer_add_trace", ctypes.c_int, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_u64))
 _sig("lt_scorer_fit", ctypes.c_int, _vp, _u32, _u32, ctypes.c_float, _u64, _vp)
-_sig("lt_scorer_train", ctypes.c_int, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32,
+_sig("lt_scorer_train", ctypes.c_int, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32,
      ctypes.c_float, _u64, _vp)
-_sig("lt_scorer_loss_grad", ctypes.c_int, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
-_sig("lt_scorer_infer", ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, _vp)
-_sig("lt_scorer_infer_noalloc", ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, _vp)
+_sig("lt_scorer_loss_grad", ctypes.c_int, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("lt_scorer_infer", ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, _vp)
+_sig("lt_scorer_infer_noalloc", ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, _vp)
 _sig("lt_scorer_get_params", ctypes.c_int, _vp, _vp)
 _sig("lt_scorer_set_params", ctypes.c_int, _vp, _vp)
 _sig("lt_scorer_save", ctypes.c_int, _vp, ctypes.c_char_p)
@@ -414,15 +415,17 @@ class Engine:
         return [(r.partition, r.kind, r.loaded, r.evicted, r.walkers) for r in buf[:n]]
 
     def trace_state(self):
-        """(counts (len,P) u32, resident (len,P) u8, recent (len,8) i32) seen by each traced decision."""
+        """(counts (len,P) u32, resident (len,P) u8, age (len,P) u32, recent (len,8) i32) seen by each
+        traced decision."""
         n = _lib.lt_engine_trace_len(self._h)
         P = self.partition.count
         c = np.zeros((n, P), np.uint32)
         r = np.zeros((n, P), np.uint8)
+        a = np.zeros((n, P), np.uint32)
         rc = np.zeros((n, 8), np.int32)
-        _check(_lib.lt_engine_trace_state(self._h, c.ctypes.data, r.ctypes.data, rc.ctypes.data, n),
+        _check(_lib.lt_engine_trace_state(self._h, c.ctypes.data, r.ctypes.data, a.ctypes.data, rc.ctypes.data, n),
                "lt_engine_trace_state")
-        return c, r, rc
+        return c, r, a, rc
 
     def set_scorer(self, scorer):
         _check(_lib.lt_engine_set_scorer(self._h, None if scorer is None else scorer._h), "lt_engine_set_scorer")
@@ -493,6 +496,7 @@ class Scorer:
         n = len(samples)
         counts = np.ascontiguousarray(np.stack([s["counts"] for s in samples]), dtype=np.uint32)
         resident = np.ascontiguousarray(np.stack([s["resident"] for s in samples]), dtype=np.uint8)
+        age = np.ascontiguousarray(np.stack([s["age"] for s in samples]), dtype=np.uint32)
         recent = np.ascontiguousarray(np.stack([s["recent"] for s in samples]), dtype=np.int32)
         cur = np.array([s["cur"] for s in samples], np.int32)
         wt = np.array([s["w_total"] for s in samples], np.float32)
@@ -501,7 +505,7 @@ class Scorer:
         wa = np.array([s["wa"] for s in samples], np.float32)
         wb = np.array([s["wb"] for s in samples], np.float32)
         assert counts.shape == (n, P)
-        return counts, resident, recent, cur, wt, nxt, keep, wa, wb
+        return counts, resident, age, recent, cur, wt, nxt, keep, wa, wb
 
     def loss_grad(self, samples):
         arrs = self._arrays(samples, self.P)
@@ -518,24 +522,27 @@ class Scorer:
                                     loss.ctypes.data), "lt_scorer_train")
         return loss
 
-    def infer(self, counts, resident, recent, cur, w_total):
-        counts = np.ascontiguousarray(counts, dtype=np.uint32)
-        resident = np.ascontiguousarray(resident, dtype=np.uint8)
-        recent = np.ascontiguousarray(recent, dtype=np.int32)
+    def _state(self, counts, resident, age, recent):
+        out = (np.ascontiguousarray(counts, dtype=np.uint32), np.ascontiguousarray(resident, dtype=np.uint8),
+               np.ascontiguousarray(age, dtype=np.uint32), np.ascontiguousarray(recent, dtype=np.int32))
+        if not all(a.shape == (self.P,) for a in out[:3]) or out[3].shape != (8,):
+            raise ValueError(f"a decision state is counts, resident and age of {self.P} partitions and 8 recent loads")
+        return out
+
+    def infer(self, counts, resident, age, recent, cur, w_total):
+        c, r, a, rc = self._state(counts, resident, age, recent)
         out = np.zeros((self.P, 2), np.float32)
-        _check(_lib.lt_scorer_infer(self._h, counts.ctypes.data, resident.ctypes.data, recent.ctypes.data, cur,
+        _check(_lib.lt_scorer_infer(self._h, c.ctypes.data, r.ctypes.data, a.ctypes.data, rc.ctypes.data, cur,
                                     w_total, out.ctypes.data), "lt_scorer_infer")
         return out
 
-    def infer_noalloc(self, counts, resident, recent, cur, w_total, out=None):
+    def infer_noalloc(self, counts, resident, age, recent, cur, w_total, out=None):
         """infer() through the GPU's static scratch (lt_scorer_infer_noalloc);
         `out` (float32 [P, 2]) may be given so that nothing is allocated."""
-        counts = np.ascontiguousarray(counts, dtype=np.uint32)
-        resident = np.ascontiguousarray(resident, dtype=np.uint8)
-        recent = np.ascontiguousarray(recent, dtype=np.int32)
+        c, r, a, rc = self._state(counts, resident, age, recent)
         if out is None:
             out = np.zeros((self.P, 2), np.float32)
-        _check(_lib.lt_scorer_infer_noalloc(self._h, counts.ctypes.data, resident.ctypes.data, recent.ctypes.data,
+        _check(_lib.lt_scorer_infer_noalloc(self._h, c.ctypes.data, r.ctypes.data, a.ctypes.data, rc.ctypes.data,
                                             cur, w_total, out.ctypes.data), "lt_scorer_infer_noalloc")
         return out
 

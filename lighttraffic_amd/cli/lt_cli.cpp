@@ -115,10 +115,11 @@ int main(int argc, char **argv) {
                               &best, nullptr, 0, nullptr);
         if (s != LT_OK) return fail(s, "autotune");
         printf("autotuned: %u MB partitions, %u cache slots, span %s, %s scheduling, slice %d, zero-copy batch %d, "
-               "%s copies\n",
+               "%s copies, walkers per lane %d\n",
                best.partition_mb, best.slots, best.span == LT_SPAN_RESIDENT ? "resident" : "partition",
                best.device_schedule ? "device" : "host", best.slice_chunks, best.zc_batch,
-               best.pack_partitions == 2 ? "block" : best.pack_partitions == 1 ? "gap-coded" : "raw");
+               best.pack_partitions == 2 ? "block" : best.pack_partitions == 1 ? "gap-coded" : "raw",
+               best.walkers_per_thread);
         lt_partition_free(part);
         if ((s = lt_partition_build(g, (uint64_t)best.partition_mb << 20, &part)) != LT_OK) return fail(s, "partition");
         cfg.num_device_partitions = best.slots;
@@ -127,6 +128,7 @@ int main(int argc, char **argv) {
         cfg.slice_chunks = best.slice_chunks;
         cfg.zc_batch = best.zc_batch;
         cfg.pack_partitions = best.pack_partitions;
+        cfg.walkers_per_thread = best.walkers_per_thread;
     }
     lt_engine *e = nullptr;
     s = lt_engine_create(g, part, &cfg, &e);

@@ -242,6 +242,12 @@ struct Engine {
     std::vector<uint32_t> trace_counts;  // queue lengths seen by each traced decision
     std::vector<uint8_t> trace_resident; // residency seen by each traced decision
     std::vector<int32_t> trace_recent;   // last kRecent loads at each traced decision
+    std::vector<uint32_t> trace_age;     // partition ages seen by each traced decision
+    // Recency of HBM use, a scorer feature: use_clock counts the walks of
+    // resident partitions since the engine was created, last_use[p] is the
+    // clock after p's last one (0 = never), and p's age is their difference.
+    std::vector<uint64_t> last_use;
+    uint64_t use_clock = 0;
     Scorer *scorer = nullptr;             // learned policy (not owned)
     // device-side scheduling state (mirrors of slot_of / part_in_slot / resident)
     DevSched *d_ds = nullptr;
@@ -258,6 +264,7 @@ struct Engine {
     std::vector<hipEvent_t> pool_events;  // reusable prefetch-completion events
     std::vector<float> scores;            // scorer output of the current decision
     bool scores_valid = false;
+    uint64_t scores_clock = 0;            // use_clock when the scores were computed
     uint64_t w_run = 0;                   // walkers of the current wave
     EventTimer t_load, t_compute, t_zc, t_wload, t_wevict, t_kernel, t_kzc;  // t_kzc: walk launches over PCIe
     Scheduler sched;
@@ -304,6 +311,11 @@ struct Engine {
     lt_status infer_scores(int32_t cur);
     int32_t learned_victim(int32_t a, int32_t b);
     void recent_loads(int32_t *out) const;
+    uint32_t age_of(uint32_t p) const;
+    // scores at most kStaleWalks walks of resident partitions old (the ages
+    // and residency they saw have moved by at most that much)
+    static constexpr uint64_t kStaleWalks = 16;
+    bool scores_recent() const { return scores_valid && use_clock - scores_clock <= kStaleWalks; }
     lt_status run_partition(uint32_t p, int32_t &prev_loaded);
 
     AppArgs app{};
@@ -376,6 +388,19 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     }
     if (V == 0) {
         set_error("empty graph");
+        return LT_E_ARG;
+    }
+    if (cfg.evict_shortlist < 0) {
+        set_error("evict_shortlist must be >= 0 (got %d)", cfg.evict_shortlist);
+        return LT_E_ARG;
+    }
+    if (cfg.prefetch_plain < 0 || cfg.prefetch_plain > 2) {
+        set_error("prefetch_plain must be 0, 1 or 2 (got %d)", cfg.prefetch_plain);
+        return LT_E_ARG;
+    }
+    if (cfg.walkers_per_thread != 0 && cfg.walkers_per_thread != 1 && cfg.walkers_per_thread != 2 &&
+        cfg.walkers_per_thread != 4) {
+        set_error("walkers_per_thread must be 0, 1, 2 or 4 (got %d)", cfg.walkers_per_thread);
         return LT_E_ARG;
     }
     if (cfg.pack_partitions < 0 || cfg.pack_partitions > 2) {
@@ -464,6 +489,7 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     slot_of.assign(P, -1);
     part_in_slot.assign(nslots, -1);
     resident.assign(P, 0);
+    last_use.assign(P, 0);
     free_slots.clear();
     for (uint32_t s = 0; s < nslots; s++) free_slots.push_back(nslots - 1 - s);
     // Default threshold: the reference's partitionSize / cachelineSize / 2
@@ -778,7 +804,10 @@ static void launch_walk_app(const WalkArgs &a, int wpt, int grid, size_t lds, hi
 
 lt_status Engine::launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s) {
     const uint64_t fill = (uint64_t)num_cu * 4 * 6 * 64;  // threads for ~6 waves per SIMD
-    const int wpt = walkers >= 4 * fill ? 4 : walkers >= 2 * fill ? 2 : 1;
+    const int wpt = cfg.walkers_per_thread > 0 ? cfg.walkers_per_thread
+                    : walkers >= 4 * fill         ? 4
+                    : walkers >= 2 * fill         ? 2
+                                                  : 1;
     const uint64_t bw = (uint64_t)kBlock * wpt;
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + bw - 1) / bw, (uint64_t)walk_grid[wpt]));
     switch (app_kind) {
@@ -1121,15 +1150,23 @@ lt_status Engine::process_zc_batch(const std::vector<uint32_t> &parts) {
     return LT_OK;
 }
 
-// One traced decision: the choice and the queue lengths it was taken on.
+// One decision: with tracing, the choice and the state it was taken on; then
+// a walk from HBM (every kind but 1, zero-copy) advances p's recency.
 void Engine::record(uint32_t p, int kind, int32_t loaded, int32_t evicted) {
-    if (!cfg.record_trace) return;
-    trace.push_back(lt_trace_rec{(int32_t)p, kind, loaded, evicted, counts[p]});
-    for (uint32_t i = 0; i < P; i++) trace_counts.push_back((uint32_t)counts[i]);
-    trace_resident.insert(trace_resident.end(), resident.begin(), resident.end());
-    int32_t rec[kRecent];
-    recent_loads(rec);
-    trace_recent.insert(trace_recent.end(), rec, rec + kRecent);
+    if (cfg.record_trace) {
+        trace.push_back(lt_trace_rec{(int32_t)p, kind, loaded, evicted, counts[p]});
+        for (uint32_t i = 0; i < P; i++) trace_counts.push_back((uint32_t)counts[i]);
+        trace_resident.insert(trace_resident.end(), resident.begin(), resident.end());
+        for (uint32_t i = 0; i < P; i++) trace_age.push_back(age_of(i));
+        int32_t rec[kRecent];
+        recent_loads(rec);
+        trace_recent.insert(trace_recent.end(), rec, rec + kRecent);
+    }
+    if (kind != 1) last_use[p] = ++use_clock;
+}
+
+uint32_t Engine::age_of(uint32_t p) const {
+    return (uint32_t)std::min<uint64_t>(use_clock - last_use[p], 0xFFFFFFFFu);
 }
 
 void Engine::recent_loads(int32_t *out) const {
@@ -1141,39 +1178,62 @@ void Engine::recent_loads(int32_t *out) const {
 lt_status Engine::infer_scores(int32_t cur) {
     scores_valid = false;
     if (!scorer || !scorer_trained(scorer)) return LT_OK;
-    std::vector<uint32_t> c32(P);
-    for (uint32_t i = 0; i < P; i++) c32[i] = (uint32_t)std::min<uint64_t>(counts[i], 0xFFFFFFFFu);
+    std::vector<uint32_t> c32(P), a32(P);
+    for (uint32_t i = 0; i < P; i++) {
+        c32[i] = (uint32_t)std::min<uint64_t>(counts[i], 0xFFFFFFFFu);
+        a32[i] = age_of(i);
+    }
     DecisionState d;
     d.counts = c32.data();
     d.resident = resident.data();
+    d.age = a32.data();
     recent_loads(d.recent);
     d.cur = cur;
     d.w_total = (float)w_run;
     scores.resize((size_t)P * kOut);
     LT_TRY(scorer_infer(scorer, d, nullptr, scores.data()));
+    scores_clock = use_clock;
     bool ok = true;
     for (float v : scores) ok = ok && std::isfinite(v);
     scores_valid = ok;
     return LT_OK;
 }
 
-// Resident partition (other than a and b) with the fewest queued walkers, ties
-// broken by the lowest keep score; without valid scores, the plain rule.
+// Victim among the resident partitions other than a and b: those with the
+// fewest queued walkers; of them the evict_shortlist least recently walked
+// (all of them with 0, the default); of those the one with the lowest keep
+// score (ties: lowest id).  A shortlist bounds the network's mistakes on
+// states unlike its training traces (a network trained on Belady's choices
+// for the plain scheduler's states can rank a partition walked moments ago
+// lowest on the states its own choices lead to).  Without valid scores, the
+// plain rule.
 int32_t Engine::learned_victim(int32_t a, int32_t b) {
     if (scores_valid) {
-        int32_t v = -1;
-        uint64_t fewest = 0;
-        float keep = 0.f;
-        for (uint32_t i = 0; i < P; i++) {
-            if (!resident[i] || (int32_t)i == a || (int32_t)i == b) continue;
-            const float k = scores[(size_t)i * kOut + 1];
-            if (v < 0 || counts[i] < fewest || (counts[i] == fewest && k < keep)) {
-                fewest = counts[i];
-                keep = k;
-                v = (int32_t)i;
+        uint64_t fewest = UINT64_MAX;
+        for (uint32_t i = 0; i < P; i++)
+            if (resident[i] && (int32_t)i != a && (int32_t)i != b) fewest = std::min(fewest, counts[i]);
+        if (fewest != UINT64_MAX) {
+            std::vector<std::pair<uint32_t, int32_t>> cand;  // (age, id)
+            for (uint32_t i = 0; i < P; i++)
+                if (resident[i] && (int32_t)i != a && (int32_t)i != b && counts[i] == fewest)
+                    cand.push_back({age_of(i), (int32_t)i});
+            const size_t k = cfg.evict_shortlist > 0 ? (size_t)cfg.evict_shortlist : cand.size();
+            if (k < cand.size()) {  // oldest first, ties by id
+                std::partial_sort(cand.begin(), cand.begin() + k, cand.end(),
+                                  [](const std::pair<uint32_t, int32_t> &x, const std::pair<uint32_t, int32_t> &y) {
+                                      return x.first > y.first || (x.first == y.first && x.second < y.second);
+                                  });
+                cand.resize(k);
             }
-        }
-        if (v >= 0) {
+            int32_t v = -1;
+            float keep = 0.f;
+            for (const auto &e : cand) {
+                const float kv = scores[(size_t)e.second * kOut + 1];
+                if (v < 0 || kv < keep || (kv == keep && e.second < v)) {
+                    keep = kv;
+                    v = e.second;
+                }
+            }
             sched.learned_decisions++;
             return v;
         }
@@ -1257,7 +1317,8 @@ lt_status Engine::scheduler_loop() {
 // partition is walked, the scorer's best guess for the next partition that
 // will need host data is copied into a slot ahead of time (prefetch), and a
 // victim is the resident partition with the fewest queued walkers, ties
-// broken by the lowest predicted future traffic (the plain rule breaks them by
+// broken by the lowest keep score: trained on Belady's next-use labels, it
+// names the partition needed farthest ahead (the plain rule breaks ties by
 // partition id).  Evictions happen only while the compute stream is idle, and
 // a prefetched slot joins the resident table only after its copy completed.
 lt_status Engine::scheduler_loop_learned() {
@@ -1271,10 +1332,12 @@ lt_status Engine::scheduler_loop_learned() {
     std::vector<uint8_t> prefetched(P, 0);  // resident because of a prefetch, not yet picked
     std::vector<uint8_t> pending(P, 0);     // being copied by a prefetch
     const uint32_t depth = (uint32_t)std::max(0, cfg.prefetch_depth);
-    // Plain-rule prefetch: the same loop without the scorer, prefetching the
-    // largest non-resident queues and evicting by the plain rule (an ablation
-    // that isolates what the network adds).
-    const bool plain_prefetch = cfg.prefetch_plain != 0;
+    // Prefetch targets by the plain rule (prefetch_plain 1 and 2): the
+    // largest non-resident queues.  With prefetch_plain 1 the victims follow
+    // the plain rule too and the scorer is not used (an ablation that
+    // isolates what the network adds); with 2 the scorer picks the victims.
+    const bool plain_victims = cfg.prefetch_plain == 1;
+    const bool plain_targets = cfg.prefetch_plain != 0;
     scores_valid = false;
     sched.begin_run();
     // make completed prefetches resident, in order; with `upto`, wait for it
@@ -1346,7 +1409,7 @@ lt_status Engine::scheduler_loop_learned() {
                 LT_TRY(retire(inflight.empty() ? -1 : inflight.back().part));  // the copy engine is in-order
                 int32_t evicted = -1;
                 if (free_slots.empty()) {
-                    if (!scores_valid) LT_TRY(infer_scores(c));
+                    if (!scores_recent()) LT_TRY(infer_scores(c));
                     evicted = learned_victim(c, prev_loaded);
                     if (evicted < 0) {
                         set_error("no partition to evict");
@@ -1370,7 +1433,7 @@ lt_status Engine::scheduler_loop_learned() {
         // freed before the walk is launched (the walk then never reads them),
         // with the scores of the previous decision; the targets are chosen
         // with the scores of this decision, computed while the walk runs.
-        size_t want = (plain_prefetch || (scorer && scorer_trained(scorer))) && inflight.size() < depth
+        size_t want = (plain_victims || (scorer && scorer_trained(scorer))) && inflight.size() < depth
                           ? depth - inflight.size()
                           : 0;
         if (want) {  // no more slots than there are partitions a prefetch may take
@@ -1380,9 +1443,12 @@ lt_status Engine::scheduler_loop_learned() {
             want = cand;
         }
         if (want) {
-            if (!scores_valid && !plain_prefetch) LT_TRY(infer_scores(c));
+            // the scores of the last inference (normally the previous
+            // decision's, computed while its walk ran): no inference here on
+            // the critical path unless they are far out of date
+            if (!scores_recent() && !plain_victims) LT_TRY(infer_scores(c));
             bool evicted_any = false;
-            while ((scores_valid || plain_prefetch) && free_slots.size() < want) {
+            while ((scores_valid || plain_victims) && free_slots.size() < want) {
                 const int32_t v = learned_victim(c, -1);
                 if (v < 0) break;
                 LT_TRY(evict((uint32_t)v));
@@ -1399,15 +1465,15 @@ lt_status Engine::scheduler_loop_learned() {
         st.computing_calls++;
         LT_TRY(zc ? process_zc((uint32_t)c, pending.data()) : process((uint32_t)c, false, counts[c]));
         if (want && !free_slots.empty()) {
-            if (!plain_prefetch) LT_TRY(infer_scores(c));  // while c is walked
-            while ((scores_valid || plain_prefetch) && inflight.size() < depth && !free_slots.empty()) {
+            if (!plain_victims) LT_TRY(infer_scores(c));  // while c is walked
+            while ((scores_valid || plain_victims) && inflight.size() < depth && !free_slots.empty()) {
                 int32_t q = -1;
                 float best = 0.f;
                 for (uint32_t i = 0; i < P; i++) {
                     if (resident[i] || pending[i] || (int32_t)i == c || counts[i] < zc_threshold) continue;
                     // the scorer's guess of the next partition needing host data, or
                     // (plain rule) the largest queue among those partitions
-                    const float v = plain_prefetch ? (float)counts[i] : scores[(size_t)i * kOut];
+                    const float v = plain_targets ? (float)counts[i] : scores[(size_t)i * kOut];
                     if (q < 0 || v > best) {
                         best = v;
                         q = (int32_t)i;
@@ -1487,6 +1553,9 @@ lt_status Engine::scheduler_loop_device() {
     a.stats = d_stats;
     a.ss = sched_state(true);
     const int load_grid = 4 * num_cu;
+    // walkers per lane of the captured walk launches: the queue sizes are only
+    // known on the device, so the grid is the full occupancy grid of that WPT
+    const int dwpt = cfg.walkers_per_thread > 0 ? cfg.walkers_per_thread : 1;
     BlockImages bi{};
     bi.words_tab = d_words_tab;
     bi.img = d_pk_img;
@@ -1499,9 +1568,9 @@ lt_status Engine::scheduler_loop_device() {
         decide_kernel<<<1, 1024, 0, s_compute>>>(q, a.ss, a.pt, bi);
         load_kernel<<<load_grid, 256, 0, s_compute>>>(a.ss, a.pt, bi);
         switch (app_kind) {
-            case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
-            case APP_PPR: launch_walk_app<APP_PPR>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
-            default: launch_walk_app<APP_GENERIC>(a, 1, walk_grid[1], lds_bytes(1), s_compute); break;
+            case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, dwpt, walk_grid[dwpt], lds_bytes(dwpt), s_compute); break;
+            case APP_PPR: launch_walk_app<APP_PPR>(a, dwpt, walk_grid[dwpt], lds_bytes(dwpt), s_compute); break;
+            default: launch_walk_app<APP_GENERIC>(a, dwpt, walk_grid[dwpt], lds_bytes(dwpt), s_compute); break;
         }
         fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, d_ds);
     }
@@ -1561,6 +1630,7 @@ lt_status Engine::run(const RunSpec &rs, uint32_t *counts_out, uint32_t *endpoin
     trace_counts.clear();
     trace_resident.clear();
     trace_recent.clear();
+    trace_age.clear();
     t_load.reset();
     t_compute.reset();
     t_zc.reset();
@@ -1889,13 +1959,14 @@ lt_status lt_engine_fetch_partition(lt_engine *e, uint32_t part, uint64_t *rowpt
 
 uint64_t lt_engine_trace_len(const lt_engine *e) { return e ? e->trace.size() : 0; }
 
-lt_status lt_engine_trace_state(const lt_engine *e, uint32_t *counts_out, uint8_t *resident_out, int32_t *recent_out,
-                                uint64_t cap) {
+lt_status lt_engine_trace_state(const lt_engine *e, uint32_t *counts_out, uint8_t *resident_out, uint32_t *age_out,
+                                int32_t *recent_out, uint64_t cap) {
     if (!e) return LT_E_ARG;
     const uint64_t n = std::min<uint64_t>(cap, e->trace.size());
     const size_t P = e->P;
     if (counts_out && n) memcpy(counts_out, e->trace_counts.data(), n * P * sizeof(uint32_t));
     if (resident_out && n) memcpy(resident_out, e->trace_resident.data(), n * P);
+    if (age_out && n) memcpy(age_out, e->trace_age.data(), n * P * sizeof(uint32_t));
     if (recent_out && n) memcpy(recent_out, e->trace_recent.data(), n * kRecent * sizeof(int32_t));
     return LT_OK;
 }
@@ -1970,6 +2041,7 @@ lt_status lt_scorer_add_trace(lt_scorer *s_, const lt_engine *e, int32_t use_nex
         if (!has_next && !has_keep) continue;
         d.counts.insert(d.counts.end(), e->trace_counts.begin() + j * P, e->trace_counts.begin() + (j + 1) * P);
         d.resident.insert(d.resident.end(), e->trace_resident.begin() + j * P, e->trace_resident.begin() + (j + 1) * P);
+        d.age.insert(d.age.end(), e->trace_age.begin() + j * P, e->trace_age.begin() + (j + 1) * P);
         d.recent.insert(d.recent.end(), e->trace_recent.begin() + j * kRecent,
                         e->trace_recent.begin() + (j + 1) * kRecent);
         d.cur.push_back(r.partition);

@@ -160,6 +160,7 @@ void lt_config_default(lt_config *c) {
     c->record_trace = 0;
     c->prefetch_depth = 2;
     c->prefetch_stick_pct = 50;
+    c->evict_shortlist = 0;
 }
 
 // ---- bcsr (src/GraphLoader.h:30-48; tools/convert_from_txt.cc:211-218) ----

@@ -3,10 +3,11 @@
 // A per-partition MLP (16 features -> 256 -> 256 -> 2, BF16 weights, FP32
 // accumulation) evaluated on all P partitions of one scheduler decision.
 // Output 0 ("next") scores how likely a partition is to be the one the
-// scheduler picks at its next decision; output 1 ("keep") predicts the share
-// of future walker traffic that will pass through the partition.  The engine
-// prefetches the best "next" partition while the current one is walked and
-// evicts the resident partition with the lowest "keep".
+// scheduler picks at its next decision; output 1 ("keep") predicts how soon
+// the partition is next walked from HBM (or, with the older labels, its share
+// of future walker traffic).  The engine prefetches the best "next" partition
+// while the current one is walked and, among the resident partitions with the
+// fewest queued walkers, evicts the one with the lowest "keep".
 #pragma once
 #include <stdint.h>
 
@@ -25,6 +26,7 @@ constexpr int kRecent = 8;
 struct DecisionState {
     const uint32_t *counts;    // [P] walkers queued per partition
     const uint8_t *resident;   // [P] 1 if the partition is in the HBM cache
+    const uint32_t *age;       // [P] HBM walks of any partition since this one's last
     int32_t recent[kRecent];   // last loaded partitions, most recent first, -1 = none
     int32_t cur;               // partition chosen at this decision, -1 = none
     float w_total;             // walkers of the run
@@ -35,6 +37,7 @@ struct Samples {
     uint64_t n = 0;
     std::vector<uint32_t> counts;   // [n*P]
     std::vector<uint8_t> resident;  // [n*P]
+    std::vector<uint32_t> age;      // [n*P]
     std::vector<int32_t> recent;    // [n*kRecent]
     std::vector<int32_t> cur, next; // [n]
     std::vector<float> w_total, wa, wb;  // [n]

@@ -65,14 +65,23 @@ __device__ __forceinline__ float to_f32(uint16_t h) { return bf16_to_f32(h); }
 // ---- features ----------------------------------------------------------------
 // One workgroup per decision.  Feature j of partition q:
 //  0 log1p(c)/log1p(W)   1 c/max c   2 log1p(c/mean c)   3 resident
-//  4 chosen now          5..12 loaded k decisions ago (k = 0..7)
+//  4 chosen now          5..11 loaded k decisions ago (k = 0..6)
+//  12 a/(a+P), a = HBM walks since q's last one (recency: Belady's victim is
+//     the partition needed farthest ahead, and on walk schedules that is
+//     mostly the one walked longest ago)
 //  13 vertex share * P   14 edge share * P   15 queued walkers / W
+constexpr int kRecentFeat = 7;  // recent loads that become features (the state holds kRecent)
+
+__device__ __forceinline__ float age_feature(uint32_t a, uint32_t P) { return (float)a / ((float)a + (float)P); }
+
 __global__ __launch_bounds__(256) void feat_kernel(uint32_t P, const uint32_t *counts, const uint8_t *resident,
-                                                   const int32_t *recent, const int32_t *cur, const float *w_total,
-                                                   const float *vsh, const float *esh, float *X) {
+                                                   const uint32_t *age, const int32_t *recent, const int32_t *cur,
+                                                   const float *w_total, const float *vsh, const float *esh,
+                                                   float *X) {
     const uint32_t b = blockIdx.x;
     const uint32_t *c = counts + (size_t)b * P;
     const uint8_t *r = resident + (size_t)b * P;
+    const uint32_t *ag = age + (size_t)b * P;
     __shared__ float s_max[256], s_sum[256];
     float mx = 0.f, sm = 0.f;
     for (uint32_t q = threadIdx.x; q < P; q += blockDim.x) {
@@ -103,7 +112,8 @@ __global__ __launch_bounds__(256) void feat_kernel(uint32_t P, const uint32_t *c
         x[3] = r[q] ? 1.f : 0.f;
         x[4] = cur[b] == (int32_t)q ? 1.f : 0.f;
 #pragma unroll
-        for (int k = 0; k < kRecent; k++) x[5 + k] = recent[b * kRecent + k] == (int32_t)q ? 1.f : 0.f;
+        for (int k = 0; k < kRecentFeat; k++) x[5 + k] = recent[b * kRecent + k] == (int32_t)q ? 1.f : 0.f;
+        x[12] = age_feature(ag[q], P);
         x[13] = vsh[q];
         x[14] = esh[q];
         x[15] = csum / W;
@@ -325,68 +335,65 @@ __global__ void transpose_infer_weights_kernel(const uint16_t *Wb, uint16_t *Wi)
     }
 }
 
-// The decision state arrives packed: u32 counts[P] | u8 resident[P] | pad to 8 |
-// i32 recent[8] | i32 cur | f32 w_total, in host memory mapped for the device;
-// the scores go back the same way.  Every workgroup copies the state into LDS
-// with one round of 16-byte loads, recomputes the max and sum of the counts
-// (P is small), builds the features of its 8 rows and runs the three layers.
-// Each output neuron accumulates its inputs in increasing order in FP32.
+// The decision state infer_kernel reads, packed in one buffer (host memory
+// mapped for the device, or device memory):
+//   i32 recent[8] | i32 cur | f32 w_total | f32 max count | f32 count sum |
+//   pad to kStateHead | u32 counts[P] | u32 age[P] | u8 resident[P]
+// The host supplies the max and the sum of the counts, so a workgroup reads
+// only the header and its own rows: the launch reads O(P) bytes of the state,
+// and LDS use does not grow with P.
+constexpr size_t kStateHead = 64;
+constexpr int kHeadCur = kRecent, kHeadW = kRecent + 1, kHeadMax = kRecent + 2, kHeadSum = kRecent + 3;
+static_assert(4 * (kHeadSum + 1) <= (int)kStateHead, "state header");
+__host__ __device__ constexpr size_t state_bytes(uint32_t P) { return kStateHead + 9 * (size_t)P; }
+
+// Header of one decision's packed state.
+static void pack_state_head(int32_t *head, const uint32_t *counts, uint32_t P, const int32_t *recent, int32_t cur,
+                            float w_total) {
+    uint32_t mx = 0;
+    double sum = 0.0;
+    for (uint32_t q = 0; q < P; q++) {
+        mx = std::max(mx, counts[q]);
+        sum += counts[q];
+    }
+    memset(head, 0, kStateHead);
+    for (int k = 0; k < kRecent; k++) head[k] = recent[k];
+    head[kHeadCur] = cur;
+    const float f[3] = {w_total, (float)mx, (float)sum};
+    memcpy(&head[kHeadW], f, sizeof(f));
+}
+
+// Each workgroup builds the features of 8 partitions (rows) and runs the
+// three layers on them.  Each output neuron accumulates its inputs in
+// increasing order in FP32.
 __global__ __launch_bounds__(256) void infer_kernel(uint32_t P, const uint8_t *state, const float *vsh,
                                                     const float *esh, const uint16_t *Wb, const uint16_t *Wi,
                                                     float *Y) {
     constexpr int R = 8;
-    extern __shared__ __attribute__((aligned(16))) uint8_t s_state[];
     __shared__ float s_x[R][kFeat];
     __shared__ float s_h1[R][kHidden];
     __shared__ float s_h2[R][kHidden];
-    __shared__ float s_max[256], s_sum[256];
-    const size_t moff = ((size_t)P * 5 + 7) & ~(size_t)7;
-    const size_t bytes = moff + 4 * (kRecent + 2);
     const int j = threadIdx.x;
-    for (size_t o = 16 * (size_t)j; o < bytes; o += 16 * 256) {
-        if (o + 16 <= bytes) {
-            *(uint4 *)(s_state + o) = *(const uint4 *)(state + o);
-        } else {
-            for (size_t b = o; b < bytes; b++) s_state[b] = state[b];
-        }
-    }
-    __syncthreads();
-    const uint32_t *counts = (const uint32_t *)s_state;
-    const uint8_t *res = s_state + 4 * (size_t)P;
-    const int32_t *meta = (const int32_t *)(s_state + moff);
+    const int32_t *head = (const int32_t *)state;
+    const uint32_t *counts = (const uint32_t *)(state + kStateHead);
+    const uint32_t *age = counts + P;
+    const uint8_t *res = state + kStateHead + 8 * (size_t)P;
     const uint32_t r0 = blockIdx.x * R;
-    float mx = 0.f, sm = 0.f;
-    for (uint32_t q = j; q < P; q += 256) {
-        const float v = (float)counts[q];
-        mx = fmaxf(mx, v);
-        sm += v;
-    }
-    s_max[j] = mx;
-    s_sum[j] = sm;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (j < off) {
-            s_max[j] = fmaxf(s_max[j], s_max[j + off]);
-            s_sum[j] += s_sum[j + off];
-        }
-        __syncthreads();
-    }
-    const float cmax = fmaxf(s_max[0], 1.f), csum = s_sum[0];
-    const float W = fmaxf(__int_as_float(meta[kRecent + 1]), 1.f);
-    const float mean = fmaxf(csum / (float)P, 1.f);
-    const float lw = log1pf(W);
-    const int32_t cur = meta[kRecent];
     if (j < R) {
         const uint32_t q = r0 + j;
         float *x = s_x[j];
         if (q < P) {
+            const float W = fmaxf(__int_as_float(head[kHeadW]), 1.f);
+            const float cmax = fmaxf(__int_as_float(head[kHeadMax]), 1.f), csum = __int_as_float(head[kHeadSum]);
+            const float mean = fmaxf(csum / (float)P, 1.f);
             const float v = (float)counts[q];
-            x[0] = log1pf(v) / lw;
+            x[0] = log1pf(v) / log1pf(W);
             x[1] = v / cmax;
             x[2] = log1pf(v / mean);
             x[3] = res[q] ? 1.f : 0.f;
-            x[4] = cur == (int32_t)q ? 1.f : 0.f;
-            for (int k = 0; k < kRecent; k++) x[5 + k] = meta[k] == (int32_t)q ? 1.f : 0.f;
+            x[4] = head[kHeadCur] == (int32_t)q ? 1.f : 0.f;
+            for (int k = 0; k < kRecentFeat; k++) x[5 + k] = head[k] == (int32_t)q ? 1.f : 0.f;
+            x[12] = age_feature(age[q], P);
             x[13] = vsh[q];
             x[14] = esh[q];
             x[15] = csum / W;
@@ -469,7 +476,7 @@ struct Scorer {
     bool trained = false;
     // training buffers (capacity in decisions)
     uint32_t cap = 0;
-    uint32_t *d_counts = nullptr;
+    uint32_t *d_counts = nullptr, *d_age = nullptr;
     uint8_t *d_res = nullptr;
     int32_t *d_recent = nullptr, *d_cur = nullptr, *d_next = nullptr;
     float *d_wtot = nullptr, *d_keep = nullptr, *d_wa = nullptr, *d_wb = nullptr;
@@ -491,7 +498,7 @@ struct Scorer {
 
 Scorer::~Scorer() {
     if (stream) (void)hipStreamSynchronize(stream);
-    void *ptrs[] = {d_vsh, d_esh, W, G, Mo, Vo, Wb, Wi, d_counts, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa,
+    void *ptrs[] = {d_vsh, d_esh, W, G, Mo, Vo, Wb, Wi, d_counts, d_age, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa,
                     d_wb, X, A1, A2, Y, dY, dA1, dA2, partial, d_loss};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -538,7 +545,7 @@ lt_status Scorer::init(uint32_t P_, const uint32_t *node_off, const uint64_t *ed
     round_kernel<<<(kParams + 255) / 256, 256, 0, stream>>>(kParams, W, Wb);
     MLP_HIP(hipGetLastError());
     wver++;
-    MLP_HIP(hipHostMalloc((void **)&h_stage, (size_t)P * 5 + 64, hipHostMallocMapped | hipHostMallocCoherent));
+    MLP_HIP(hipHostMalloc((void **)&h_stage, state_bytes(P), hipHostMallocMapped | hipHostMallocCoherent));
     MLP_HIP(hipHostMalloc((void **)&h_Y, sizeof(float) * P * kOut, hipHostMallocMapped | hipHostMallocCoherent));
     MLP_HIP(hipHostGetDevicePointer((void **)&d_stage, h_stage, 0));
     MLP_HIP(hipHostGetDevicePointer((void **)&d_Y, h_Y, 0));
@@ -548,13 +555,13 @@ lt_status Scorer::init(uint32_t P_, const uint32_t *node_off, const uint64_t *ed
 
 lt_status Scorer::ensure_cap(uint32_t n) {
     if (n <= cap) return LT_OK;
-    void *ptrs[] = {d_counts, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa, d_wb, X, A1, A2, Y, dY, dA1, dA2,
+    void *ptrs[] = {d_counts, d_age, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa, d_wb, X, A1, A2, Y, dY, dA1, dA2,
                     partial};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     const size_t rows = (size_t)n * P;
     lt_status st;
-    if ((st = dalloc(&d_counts, rows)) || (st = dalloc(&d_res, rows)) || (st = dalloc(&d_recent, (size_t)n * kRecent)) ||
+    if ((st = dalloc(&d_counts, rows)) || (st = dalloc(&d_age, rows)) || (st = dalloc(&d_res, rows)) || (st = dalloc(&d_recent, (size_t)n * kRecent)) ||
         (st = dalloc(&d_cur, n)) || (st = dalloc(&d_next, n)) || (st = dalloc(&d_wtot, n)) ||
         (st = dalloc(&d_keep, rows)) || (st = dalloc(&d_wa, n)) || (st = dalloc(&d_wb, n)) ||
         (st = dalloc(&X, rows * kFeat)) || (st = dalloc(&A1, rows * kHidden)) || (st = dalloc(&A2, rows * kHidden)) ||
@@ -574,7 +581,7 @@ lt_status Scorer::forward_backward(uint32_t nb, bool update, float lr) {
     const int H = kHidden;
     hipStream_t s = stream;
     MLP_HIP(hipMemsetAsync(d_loss, 0, 8, s));
-    feat_kernel<<<nb, 256, 0, s>>>(P, d_counts, d_res, d_recent, d_cur, d_wtot, d_vsh, d_esh, X);
+    feat_kernel<<<nb, 256, 0, s>>>(P, d_counts, d_res, d_age, d_recent, d_cur, d_wtot, d_vsh, d_esh, X);
     // forward with the BF16 weights
     gemm<false, false, uint16_t, EPI_BIAS_RELU>(s, M, H, kFeat, X, kFeat, Wb + kOffW1, H, A1, H, Wb + kOffB1);
     gemm<false, false, uint16_t, EPI_BIAS_RELU>(s, M, H, H, A1, H, Wb + kOffW2, H, A2, H, Wb + kOffB2);
@@ -612,28 +619,22 @@ Samples &scorer_samples(Scorer *s) { return s->data; }
 uint32_t scorer_partitions(const Scorer *s) { return s->P; }
 bool scorer_trained(const Scorer *s) { return s && s->trained; }
 
-// Stage one decision's state into pinned memory in the packed layout of
-// infer_kernel, copy it over in one transfer, run the fused MLP, read back.
+// Stage one decision's state in pinned mapped memory in the packed layout of
+// infer_kernel, which reads it over the bus; the scores come back the same way.
 lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float *out) {
     hipStream_t st = stream_v ? (hipStream_t)stream_v : s->stream;
     const uint32_t P = s->P;
     uint8_t *h = s->h_stage;
-    memcpy(h, d.counts, 4 * (size_t)P);
-    memcpy(h + 4 * (size_t)P, d.resident, P);
-    const size_t moff = ((size_t)P * 5 + 7) & ~(size_t)7;
-    int32_t *meta = (int32_t *)(h + moff);
-    for (int k = 0; k < kRecent; k++) meta[k] = d.recent[k];
-    meta[kRecent] = d.cur;
-    float wt = d.w_total;
-    memcpy(&meta[kRecent + 1], &wt, 4);
-    const size_t bytes = moff + 4 * (kRecent + 2);
+    pack_state_head((int32_t *)h, d.counts, P, d.recent, d.cur, d.w_total);
+    memcpy(h + kStateHead, d.counts, 4 * (size_t)P);
+    memcpy(h + kStateHead + 4 * (size_t)P, d.age, 4 * (size_t)P);
+    memcpy(h + kStateHead + 8 * (size_t)P, d.resident, P);
     if (s->wi_ver != s->wver) {
         transpose_infer_weights_kernel<<<(unsigned)((kWiSize + 255) / 256), 256, 0, st>>>(s->Wb, s->Wi);
         MLP_HIP(hipGetLastError());
         s->wi_ver = s->wver;
     }
-    infer_kernel<<<(P + 7) / 8, 256, (bytes + 15) & ~(size_t)15, st>>>(P, s->d_stage, s->d_vsh, s->d_esh, s->Wb,
-                                                                         s->Wi, s->d_Y);
+    infer_kernel<<<(P + 7) / 8, 256, 0, st>>>(P, s->d_stage, s->d_vsh, s->d_esh, s->Wb, s->Wi, s->d_Y);
     MLP_HIP(hipGetLastError());
     hipError_t e;  // poll: this wait is on the scheduler's critical path
     while ((e = hipStreamQuery(st)) == hipErrorNotReady) {
@@ -647,11 +648,11 @@ lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float 
 // Scratch of lt_scorer_infer_noalloc, one copy per GPU, held until the
 // process ends: one decision's packed state (the layout infer_kernel reads),
 // its scores, and the output-major copy of the first two layers' weights.
-// Sized for kNoAllocMaxP partitions (about 1.2 MB).  In host code the name
+// Sized for kNoAllocMaxP partitions (about 1.4 MB).  In host code the name
 // `scratch` is only a stand-in for the symbol: its device address comes from
 // hipGetSymbolAddress, once per device, and only that address is used.
 constexpr uint32_t kNoAllocMaxP = 65535;  // the largest partition count a .part file holds
-constexpr size_t kNaStateBytes = ((((size_t)kNoAllocMaxP * 5 + 7) & ~(size_t)7) + 4 * (kRecent + 2) + 255) & ~(size_t)255;
+constexpr size_t kNaStateBytes = (state_bytes(kNoAllocMaxP) + 255) & ~(size_t)255;
 constexpr size_t kNaYBytes = ((sizeof(float) * kNoAllocMaxP * kOut) + 255) & ~(size_t)255;
 constexpr size_t kNaWiBytes = ((sizeof(uint16_t) * kWiSize) + 255) & ~(size_t)255;
 static __device__ __align__(256) uint8_t scratch[kNaStateBytes + kNaYBytes + kNaWiBytes];
@@ -696,11 +697,11 @@ struct lt_scorer : lt::Scorer {};
 using namespace lt;
 
 static lt_status upload_samples(Scorer *s, uint64_t off, uint32_t nb, const uint32_t *counts, const uint8_t *resident,
-                                const int32_t *recent, const int32_t *cur, const float *w_total,
+                                const uint32_t *age, const int32_t *recent, const int32_t *cur, const float *w_total,
                                 const int32_t *next_label, const float *keep_label, const float *wa, const float *wb,
                                 const std::vector<uint64_t> *order) {
     const uint32_t P = s->P;
-    std::vector<uint32_t> c((size_t)nb * P);
+    std::vector<uint32_t> c((size_t)nb * P), ag((size_t)nb * P);
     std::vector<uint8_t> r((size_t)nb * P);
     std::vector<int32_t> rc((size_t)nb * kRecent), cu(nb), nx(nb);
     std::vector<float> wt(nb), kp((size_t)nb * P), a(nb), b(nb);
@@ -708,6 +709,7 @@ static lt_status upload_samples(Scorer *s, uint64_t off, uint32_t nb, const uint
         const uint64_t j = order ? (*order)[off + i] : off + i;
         memcpy(&c[(size_t)i * P], counts + j * P, 4 * (size_t)P);
         memcpy(&r[(size_t)i * P], resident + j * P, P);
+        memcpy(&ag[(size_t)i * P], age + j * P, 4 * (size_t)P);
         memcpy(&rc[(size_t)i * kRecent], recent + j * kRecent, 4 * kRecent);
         cu[i] = cur[j];
         nx[i] = next_label[j];
@@ -719,6 +721,7 @@ static lt_status upload_samples(Scorer *s, uint64_t off, uint32_t nb, const uint
     hipStream_t st = s->stream;
     MLP_HIP(hipMemcpyAsync(s->d_counts, c.data(), 4 * c.size(), hipMemcpyHostToDevice, st));
     MLP_HIP(hipMemcpyAsync(s->d_res, r.data(), r.size(), hipMemcpyHostToDevice, st));
+    MLP_HIP(hipMemcpyAsync(s->d_age, ag.data(), 4 * ag.size(), hipMemcpyHostToDevice, st));
     MLP_HIP(hipMemcpyAsync(s->d_recent, rc.data(), 4 * rc.size(), hipMemcpyHostToDevice, st));
     MLP_HIP(hipMemcpyAsync(s->d_cur, cu.data(), 4 * cu.size(), hipMemcpyHostToDevice, st));
     MLP_HIP(hipMemcpyAsync(s->d_next, nx.data(), 4 * nx.size(), hipMemcpyHostToDevice, st));
@@ -750,18 +753,18 @@ lt_status lt_scorer_create(const lt_partition *p, uint64_t seed, lt_scorer **out
 void lt_scorer_free(lt_scorer *s) { delete s; }
 
 lt_status lt_scorer_loss_grad(lt_scorer *s, uint64_t n, const uint32_t *counts, const uint8_t *resident,
-                              const int32_t *recent, const int32_t *cur, const float *w_total,
+                              const uint32_t *age, const int32_t *recent, const int32_t *cur, const float *w_total,
                               const int32_t *next_label, const float *keep_label, const float *wa, const float *wb,
                               float *loss_out, float *grad_out) {
-    if (!s || !n || !counts || !resident || !recent || !cur || !w_total || !next_label) return LT_E_ARG;
+    if (!s || !n || !counts || !resident || !age || !recent || !cur || !w_total || !next_label) return LT_E_ARG;
     if (n > 512) {
         set_error("at most 512 decisions per call");
         return LT_E_ARG;
     }
     lt_status st = s->ensure_cap((uint32_t)n);
     if (st != LT_OK) return st;
-    if ((st = upload_samples(s, 0, (uint32_t)n, counts, resident, recent, cur, w_total, next_label, keep_label, wa, wb,
-                             nullptr)) != LT_OK)
+    if ((st = upload_samples(s, 0, (uint32_t)n, counts, resident, age, recent, cur, w_total, next_label, keep_label,
+                             wa, wb, nullptr)) != LT_OK)
         return st;
     if ((st = s->forward_backward((uint32_t)n, false, 0.f)) != LT_OK) return st;
     if (loss_out) MLP_HIP(hipMemcpyAsync(loss_out, s->d_loss, 8, hipMemcpyDeviceToHost, s->stream));
@@ -771,10 +774,10 @@ lt_status lt_scorer_loss_grad(lt_scorer *s, uint64_t n, const uint32_t *counts, 
 }
 
 lt_status lt_scorer_train(lt_scorer *s, uint64_t n, const uint32_t *counts, const uint8_t *resident,
-                          const int32_t *recent, const int32_t *cur, const float *w_total, const int32_t *next_label,
-                          const float *keep_label, const float *wa, const float *wb, uint32_t epochs, uint32_t batch,
-                          float lr, uint64_t seed, float *loss_out) {
-    if (!s || !n || !counts || !resident || !recent || !cur || !w_total || !next_label || !batch || !epochs)
+                          const uint32_t *age, const int32_t *recent, const int32_t *cur, const float *w_total,
+                          const int32_t *next_label, const float *keep_label, const float *wa, const float *wb,
+                          uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *loss_out) {
+    if (!s || !n || !counts || !resident || !age || !recent || !cur || !w_total || !next_label || !batch || !epochs)
         return LT_E_ARG;
     batch = std::min<uint32_t>(batch, (uint32_t)std::min<uint64_t>(n, 512));
     lt_status st = s->ensure_cap(batch);
@@ -788,8 +791,8 @@ lt_status lt_scorer_train(lt_scorer *s, uint64_t n, const uint32_t *counts, cons
         double acc[2] = {0, 0};
         uint32_t steps = 0;
         for (uint64_t off = 0; off + batch <= n; off += batch) {
-            if ((st = upload_samples(s, off, batch, counts, resident, recent, cur, w_total, next_label, keep_label, wa,
-                                     wb, &order)) != LT_OK)
+            if ((st = upload_samples(s, off, batch, counts, resident, age, recent, cur, w_total, next_label,
+                                     keep_label, wa, wb, &order)) != LT_OK)
                 return st;
             if ((st = s->forward_backward(batch, true, lr)) != LT_OK) return st;
             float l[2];
@@ -817,12 +820,13 @@ lt_status lt_scorer_train(lt_scorer *s, uint64_t n, const uint32_t *counts, cons
     return LT_OK;
 }
 
-lt_status lt_scorer_infer(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const int32_t *recent,
-                          int32_t cur, float w_total, float *out) {
-    if (!s || !counts || !resident || !recent || !out) return LT_E_ARG;
+lt_status lt_scorer_infer(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const uint32_t *age,
+                          const int32_t *recent, int32_t cur, float w_total, float *out) {
+    if (!s || !counts || !resident || !age || !recent || !out) return LT_E_ARG;
     DecisionState d;
     d.counts = counts;
     d.resident = resident;
+    d.age = age;
     for (int k = 0; k < kRecent; k++) d.recent[k] = recent[k];
     d.cur = cur;
     d.w_total = w_total;
@@ -832,9 +836,9 @@ lt_status lt_scorer_infer(lt_scorer *s, const uint32_t *counts, const uint8_t *r
 // The scores of lt_scorer_infer, computed through the GPU's static scratch
 // instead of the scorer's staging buffers: state copied in, infer_kernel,
 // scores copied out, all on the scorer's stream; no allocation of any kind.
-lt_status lt_scorer_infer_noalloc(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const int32_t *recent,
-                                  int32_t cur, float w_total, float *out) {
-    if (!s || !counts || !resident || !recent || !out) return LT_E_ARG;
+lt_status lt_scorer_infer_noalloc(lt_scorer *s, const uint32_t *counts, const uint8_t *resident, const uint32_t *age,
+                                  const int32_t *recent, int32_t cur, float w_total, float *out) {
+    if (!s || !counts || !resident || !age || !recent || !out) return LT_E_ARG;
     const uint32_t P = s->P;
     if (P > kNoAllocMaxP) {
         set_error("lt_scorer_infer_noalloc: %u partitions exceed the scratch's %u", P, kNoAllocMaxP);
@@ -845,15 +849,13 @@ lt_status lt_scorer_infer_noalloc(lt_scorer *s, const uint32_t *counts, const ui
     NoAllocDevice *nd = nullptr;
     LT_NA_TRY(noalloc_scratch(dev, &nd));
     uint8_t *state = nd->base, *Y = nd->base + kNaStateBytes, *Wi = nd->base + kNaStateBytes + kNaYBytes;
-    const size_t moff = ((size_t)P * 5 + 7) & ~(size_t)7;
-    int32_t meta[kRecent + 2];
-    for (int k = 0; k < kRecent; k++) meta[k] = recent[k];
-    meta[kRecent] = cur;
-    memcpy(&meta[kRecent + 1], &w_total, 4);
+    int32_t head[kStateHead / 4];
+    pack_state_head(head, counts, P, recent, cur, w_total);
     hipStream_t st = s->stream;
-    MLP_HIP(hipMemcpyAsync(state, counts, 4 * (size_t)P, hipMemcpyHostToDevice, st));
-    MLP_HIP(hipMemcpyAsync(state + 4 * (size_t)P, resident, P, hipMemcpyHostToDevice, st));
-    MLP_HIP(hipMemcpyAsync(state + moff, meta, sizeof(meta), hipMemcpyHostToDevice, st));
+    MLP_HIP(hipMemcpyAsync(state, head, kStateHead, hipMemcpyHostToDevice, st));
+    MLP_HIP(hipMemcpyAsync(state + kStateHead, counts, 4 * (size_t)P, hipMemcpyHostToDevice, st));
+    MLP_HIP(hipMemcpyAsync(state + kStateHead + 4 * (size_t)P, age, 4 * (size_t)P, hipMemcpyHostToDevice, st));
+    MLP_HIP(hipMemcpyAsync(state + kStateHead + 8 * (size_t)P, resident, P, hipMemcpyHostToDevice, st));
     {
         std::lock_guard<std::mutex> lock(g_noalloc_mu);
         if (nd->wi_uid != s->uid || nd->wi_ver != s->wver) {  // the scratch holds other weights
@@ -864,9 +866,7 @@ lt_status lt_scorer_infer_noalloc(lt_scorer *s, const uint32_t *counts, const ui
             nd->wi_ver = s->wver;
         }
     }
-    const size_t bytes = moff + 4 * (kRecent + 2);
-    infer_kernel<<<(P + 7) / 8, 256, (bytes + 15) & ~(size_t)15, st>>>(P, state, s->d_vsh, s->d_esh, s->Wb,
-                                                                         (const uint16_t *)Wi, (float *)Y);
+    infer_kernel<<<(P + 7) / 8, 256, 0, st>>>(P, state, s->d_vsh, s->d_esh, s->Wb, (const uint16_t *)Wi, (float *)Y);
     MLP_HIP(hipGetLastError());
     MLP_HIP(hipMemcpyAsync(out, Y, sizeof(float) * P * kOut, hipMemcpyDeviceToHost, st));
     MLP_HIP(hipStreamSynchronize(st));
@@ -882,7 +882,7 @@ lt_status lt_scorer_fit(lt_scorer *s, uint32_t epochs, uint32_t batch, float lr,
         set_error("no training samples: add traces first");
         return LT_E_ARG;
     }
-    return lt_scorer_train(s, d.n, d.counts.data(), d.resident.data(), d.recent.data(), d.cur.data(),
+    return lt_scorer_train(s, d.n, d.counts.data(), d.resident.data(), d.age.data(), d.recent.data(), d.cur.data(),
                            d.w_total.data(), d.next.data(), d.keep.data(), d.wa.data(), d.wb.data(), epochs, batch, lr,
                            seed, loss_out);
 }
@@ -904,7 +904,7 @@ lt_status lt_scorer_set_params(lt_scorer *s, const float *in) {
     return LT_OK;
 }
 
-// File: "LTMLP1\0\0" | u32 features | u32 hidden | u32 outputs | u32 pad | u16 bf16 weights[kParams]
+// File: "LTMLP2\0\0" | u32 features | u32 hidden | u32 outputs | u32 pad | u16 bf16 weights[kParams]
 lt_status lt_scorer_save(const lt_scorer *s, const char *path) {
     if (!s || !path) return LT_E_ARG;
     std::vector<uint16_t> wb(kParams);
@@ -914,7 +914,7 @@ lt_status lt_scorer_save(const lt_scorer *s, const char *path) {
         set_error("cannot create %s", path);
         return LT_E_IO;
     }
-    const char magic[8] = {'L', 'T', 'M', 'L', 'P', '1', 0, 0};
+    const char magic[8] = {'L', 'T', 'M', 'L', 'P', '2', 0, 0};
     const uint32_t hdr[4] = {kFeat, kHidden, kOut, 0};
     bool ok = fwrite(magic, 1, 8, f) == 8 && fwrite(hdr, 4, 4, f) == 4 && fwrite(wb.data(), 2, kParams, f) == kParams;
     ok = fclose(f) == 0 && ok;
@@ -937,7 +937,7 @@ lt_status lt_scorer_load(lt_scorer *s, const char *path) {
     std::vector<uint16_t> wb(kParams);
     bool ok = fread(magic, 1, 8, f) == 8 && fread(hdr, 4, 4, f) == 4 && fread(wb.data(), 2, kParams, f) == kParams;
     fclose(f);
-    if (!ok || memcmp(magic, "LTMLP1", 6) != 0 || hdr[0] != kFeat || hdr[1] != kHidden || hdr[2] != kOut) {
+    if (!ok || memcmp(magic, "LTMLP2", 6) != 0 || hdr[0] != kFeat || hdr[1] != kHidden || hdr[2] != kOut) {
         set_error("%s is not a scorer weights file", path);
         return LT_E_FORMAT;
     }

@@ -7,8 +7,9 @@
 // It times each configuration on every sample_every-th query (a warm second
 // job), then the three fastest and the fastest of each copy mode again on a
 // sample four times larger, since a small sample walks few steps per
-// partition load and so over-weights the copies; the pick is the fastest of
-// that second stage.
+// partition load and so over-weights the copies; then the fastest of that
+// second stage with each fixed number of walkers per walk-kernel lane.  The
+// pick is the fastest of stages 2 and 3.
 #include <stdio.h>
 #include <string.h>
 
@@ -43,6 +44,7 @@ lt_status time_trial(const lt_graph *g, const lt_partition *p, const Sample &s, 
     cfg.slice_chunks = t.slice_chunks;
     cfg.zc_batch = t.zc_batch;
     cfg.pack_partitions = t.pack_partitions;
+    cfg.walkers_per_thread = t.walkers_per_thread;
     lt_engine *e = nullptr;
     lt_status st = lt_engine_create(g, p, &cfg, &e);
     if (st != LT_OK) return st;
@@ -133,8 +135,8 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
         }
     }
     std::vector<lt_tune_trial> last;
+    const Sample s2 = take_sample(nq, sources, counts, std::max(1u, every / 4));
     if (every > 1) {
-        const Sample s2 = take_sample(nq, sources, counts, std::max(1u, every / 4));
         for (uint32_t mb : kSizesMb) {
             if (!std::any_of(pick.begin(), pick.end(), [&](const lt_tune_trial &t) { return t.partition_mb == mb; }))
                 continue;
@@ -156,12 +158,33 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
     } else {
         last = trials;
     }
-    const lt_tune_trial *best = &last[0];
+    lt_tune_trial best = last[0];
     for (const lt_tune_trial &t : last)
-        if (t.running_ms < best->running_ms) best = &t;
-    *best_out = *best;
-    trials.insert(trials.end(), last.begin(), last.end());
-    if (every <= 1) trials.resize(trials.size() - last.size());  // stage 1 was the last stage
+        if (t.running_ms < best.running_ms) best = t;
+    if (every > 1) trials.insert(trials.end(), last.begin(), last.end());
+    // stage 3: the pick with 1, 2 and 4 walkers per lane (the device loop's
+    // default is 1, so that variant is the pick itself there)
+    {
+        lt_partition *p = nullptr;
+        lt_status st = lt_partition_build(g, (uint64_t)best.partition_mb << 20, &p);
+        if (st != LT_OK) return st;
+        const lt_tune_trial pick = best;
+        for (int32_t wpt : {1, 2, 4}) {
+            if (pick.device_schedule && wpt == 1) continue;
+            lt_tune_trial t = pick;
+            t.stage = 3;
+            t.walkers_per_thread = wpt;
+            if ((st = time_trial(g, p, s2, prob, t)) != LT_OK) {
+                lt_partition_free(p);
+                return st;
+            }
+            trials.push_back(t);
+            if (t.running_ms < best.running_ms) best = t;
+        }
+        lt_graph_drop_packed(gw);
+        lt_partition_free(p);
+    }
+    *best_out = best;
     if (trials_out) memcpy(trials_out, trials.data(), sizeof(lt_tune_trial) * std::min<size_t>(trials_cap, trials.size()));
     if (ntrials_out) *ntrials_out = (uint32_t)trials.size();
     return LT_OK;

@@ -10,8 +10,10 @@ partitions one zero-copy launch walks, and whether partitions are copied raw,
 gap-coded and unpacked in HBM (host scheduling), or kept packed in HBM as
 16-edge blocks.  A small sample walks few steps per partition load, so it
 over-weights the copies: the three fastest configurations, and the fastest of
-each copy mode, are timed again on a sample four times larger, and the pick
-is the fastest of those.  Every configuration produces the same endpoints.
+each copy mode, are timed again on a sample four times larger; the fastest of
+those is timed again with 1, 2 and 4 walkers per walk-kernel lane (instead of
+the launch-size rule), and the pick is the fastest of the last two stages.
+Every configuration produces the same endpoints.
 """
 import ctypes
 
@@ -27,7 +29,8 @@ class TuneTrial(ctypes.Structure):
     _fields_ = [("stage", ctypes.c_uint32), ("partition_mb", ctypes.c_uint32), ("slots", ctypes.c_uint32),
                 ("partitions", ctypes.c_uint32), ("cache_slots", ctypes.c_uint32), ("span", ctypes.c_int32),
                 ("device_schedule", ctypes.c_int32), ("slice_chunks", ctypes.c_int32), ("zc_batch", ctypes.c_int32),
-                ("pack_partitions", ctypes.c_int32), ("running_ms", ctypes.c_double), ("h2d_bytes", ctypes.c_uint64),
+                ("pack_partitions", ctypes.c_int32), ("walkers_per_thread", ctypes.c_int32),
+                ("running_ms", ctypes.c_double), ("h2d_bytes", ctypes.c_uint64),
                 ("iterations", ctypes.c_uint64)]
 
     def as_dict(self):
@@ -46,8 +49,9 @@ _lib.lt_tune_ppr_batch.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_vo
 def tune_ppr_batch(graph, sources, counts, prob=0.15, cache_bytes=8 << 30, sample_every=16, sizes_mb=None,
                    log=None):
     """Returns (best, trials) as dicts: trials holds every configuration timed
-    on the sample (`stage` 1) and the refined ones timed on the larger sample
-    (`stage` 2); best is the fastest of the last stage."""
+    on the sample (`stage` 1), the refined ones timed on the larger sample
+    (`stage` 2) and stage 2's pick with fixed walkers per lane (`stage` 3);
+    best is the fastest of stages 2 and 3."""
     src = np.ascontiguousarray(sources, dtype=np.uint32)
     cnt = np.ascontiguousarray(counts, dtype=np.uint64)
     sizes = None if sizes_mb is None else np.ascontiguousarray(sizes_mb, dtype=np.uint32)

@@ -49,8 +49,11 @@ def shares(node_off, edge_off):
     return vsh, esh
 
 
-def features(counts, resident, recent, cur, w_total, vsh, esh):
-    """Features of one decision, shape (P, 16)."""
+def features(counts, resident, age, recent, cur, w_total, vsh, esh):
+    """Features of one decision, shape (P, 16): queue length (three scalings),
+    residency, chosen now, loaded 0..6 decisions ago, age a/(a+P) (a = walks
+    of resident partitions since the partition's own last one), vertex and
+    edge shares, queued walkers / W."""
     c = np.asarray(counts, np.float64)
     P = len(c)
     cmax = max(c.max(), 1.0)
@@ -63,8 +66,10 @@ def features(counts, resident, recent, cur, w_total, vsh, esh):
     X[:, 2] = np.log1p(c / mean)
     X[:, 3] = np.asarray(resident) != 0
     X[:, 4] = np.arange(P) == cur
-    for k in range(R):
+    for k in range(R - 1):
         X[:, 5 + k] = np.arange(P) == recent[k]
+    a = np.asarray(age, np.float64)
+    X[:, 12] = a / (a + P)
     X[:, 13] = vsh
     X[:, 14] = esh
     X[:, 15] = csum / W
@@ -83,7 +88,7 @@ def forward(Wb, X):
 def loss_and_grad(params_flat, samples, vsh, esh, rounded=True, want_grad=True):
     """Mean losses over the batch and the gradient w.r.t. the (rounded) weights.
 
-    samples: list of dicts with counts, resident, recent, cur, w_total,
+    samples: list of dicts with counts, resident, age, recent, cur, w_total,
     next (int, -1 = none), keep (P floats), wa, wb.
     """
     Wf = unpack(params_flat)
@@ -92,7 +97,7 @@ def loss_and_grad(params_flat, samples, vsh, esh, rounded=True, want_grad=True):
     grads = {k: np.zeros_like(v) for k, v in Wb.items()}
     la = lb = 0.0
     for s in samples:
-        X = features(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"], vsh, esh)
+        X = features(s["counts"], s["resident"], s["age"], s["recent"], s["cur"], s["w_total"], vsh, esh)
         P = X.shape[0]
         A1, A2, Y = forward(Wb, X)
         dY = np.zeros_like(Y)

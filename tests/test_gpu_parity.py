@@ -103,6 +103,14 @@ CONFIGS = [
     ("blocks-device-sched", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "pack_partitions": 2}),
     ("blocks-device-sched-zc", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "pack_partitions": 2,
                                           "zerocopy_threshold": 2000, "span": "resident"}),
+    # fixed walkers per walk-kernel lane, host and device loops
+    ("wpt4-capped", 16 << 10, {"num_device_partitions": 3, "walkers_per_thread": 4}),
+    ("wpt2-blocks-batch", 16 << 10, {"num_device_partitions": 4, "span": "resident", "zerocopy_threshold": 3000,
+                                     "zc_batch": 16, "pack_partitions": 2, "walkers_per_thread": 2}),
+    ("wpt4-device-sched-blocks", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "pack_partitions": 2,
+                                            "walkers_per_thread": 4}),
+    ("wpt2-device-sched-zc", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "zerocopy_threshold": 2000,
+                                        "span": "resident", "walkers_per_thread": 2}),
 ]
 
 
@@ -233,7 +241,8 @@ def test_ppr_batch_bit_exact(g12, kw):
 def test_autotuner_picks_a_trial():
     """The batched-PPR autotuner tries every configuration on a sample, times
     the three fastest and the fastest of each copy mode again on a larger
-    sample and returns the fastest of those; every trial walks the same
+    sample, then the fastest of those with fixed walkers per lane, and
+    returns the fastest of the last two stages; every trial walks the same
     queries."""
     from lighttraffic_amd.tune import tune_ppr_batch
     log("autotuner")
@@ -255,7 +264,14 @@ def test_autotuner_picks_a_trial():
                            key=lambda t: t["running_ms"])[k] for k in keys))
     assert sorted(tuple(t[k] for k in keys) for t in refined) == sorted(want)
     assert {t["pack_partitions"] for t in refined} == {0, 1, 2}
-    assert best in refined and best["running_ms"] == min(t["running_ms"] for t in refined)
+    # stage 3: stage 2's pick with 1, 2 and 4 walkers per lane (the device
+    # loop's default is 1, so it is not timed twice there)
+    third = [t for t in trials if t["stage"] == 3]
+    pick2 = min(refined, key=lambda t: t["running_ms"])
+    assert pick2["walkers_per_thread"] == 0
+    assert sorted(t["walkers_per_thread"] for t in third) == ([2, 4] if pick2["device_schedule"] else [1, 2, 4])
+    assert all(tuple(t[k] for k in keys) == tuple(pick2[k] for k in keys) for t in third)
+    assert best in refined + third and best["running_ms"] == min(t["running_ms"] for t in refined + third)
     assert all(t["cache_slots"] >= t["slots"] and t["iterations"] > 0 for t in trials)
     assert {t["zc_batch"] for t in first} == {1, 16}
     assert {t["pack_partitions"] for t in first} == {0, 1, 2}

@@ -32,6 +32,7 @@ def _partition_handle(node, edge):
 def _sample(rng, P, with_next=True):
     counts = (rng.integers(0, 5, P) * rng.integers(0, 100000, P)).astype(np.uint32)
     return {"counts": counts, "resident": (rng.random(P) < 0.5).astype(np.uint8),
+            "age": rng.integers(0, 3 * P, P).astype(np.uint32),
             "recent": rng.integers(-1, P, 8).astype(np.int32), "cur": int(rng.integers(0, P)),
             "w_total": float(counts.sum() * 3 + 1), "next": int(rng.integers(0, P)) if with_next else -1,
             "keep": (rng.random(P) * 2).astype(np.float32), "wa": 1.0, "wb": 1.0}
@@ -69,17 +70,35 @@ def test_inference_matches_oracle_and_weights_file(tmp_path):
     part = _partition_handle(node, edge)
     sc = lt.Scorer(part, seed=5)
     s = _sample(rng, P)
-    out = sc.infer(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"])
+    out = sc.infer(s["counts"], s["resident"], s["age"], s["recent"], s["cur"], s["w_total"])
     Wb = {k: M.bf16_round(v) for k, v in M.unpack(sc.get_params().astype(np.float64)).items()}
     vsh, esh = M.shares(node, edge)
-    _, _, Y = M.forward(Wb, M.features(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"], vsh, esh))
+    _, _, Y = M.forward(Wb, M.features(s["counts"], s["resident"], s["age"], s["recent"], s["cur"], s["w_total"], vsh, esh))
     assert np.allclose(out, Y, rtol=1e-3, atol=1e-4 * np.abs(Y).max())
     path = str(tmp_path / "w.ltmlp")
     sc.save(path)
     sc2 = lt.Scorer(part, seed=99)
     sc2.load(path)
     assert np.array_equal(sc2.get_params(), M.bf16_round(sc.get_params()).astype(np.float32))
-    assert np.array_equal(sc2.infer(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"]), out)
+    assert np.array_equal(sc2.infer(s["counts"], s["resident"], s["age"], s["recent"], s["cur"], s["w_total"]), out)
+
+
+def test_inference_at_many_partitions():
+    """Each inference workgroup reads only its own rows and the state's
+    header, so scorers of many partitions run too (20,000 here)."""
+    log("scorer inference, 20,000 partitions")
+    rng = np.random.default_rng(12)
+    P = 20000
+    node, edge = _part(P, rng)
+    part = _partition_handle(node, edge)
+    sc = lt.Scorer(part, seed=6)
+    s = _sample(rng, P)
+    out = sc.infer(s["counts"], s["resident"], s["age"], s["recent"], s["cur"], s["w_total"])
+    Wb = {k: M.bf16_round(v) for k, v in M.unpack(sc.get_params().astype(np.float64)).items()}
+    vsh, esh = M.shares(node, edge)
+    _, _, Y = M.forward(Wb, M.features(s["counts"], s["resident"], s["age"], s["recent"], s["cur"], s["w_total"],
+                                       vsh, esh))
+    assert np.allclose(out, Y, rtol=1e-3, atol=1e-4 * np.abs(Y).max())
 
 
 def test_training_reduces_loss():
@@ -106,7 +125,7 @@ def test_training_reduces_loss():
     assert after[0] < 0.5 * before[0] and after[1] < 0.5 * before[1]
     hits = 0
     for s in samples[:64]:
-        y = sc.infer(s["counts"], s["resident"], s["recent"], s["cur"], s["w_total"])
+        y = sc.infer(s["counts"], s["resident"], s["age"], s["recent"], s["cur"], s["w_total"])
         cand = (s["resident"] == 0) & (s["counts"] > 0)
         if cand.any():
             hits += int(np.argmax(np.where(cand, y[:, 0], -np.inf)) == s["next"])
@@ -159,12 +178,13 @@ def test_learned_policy_without_trained_scorer_falls_back():
 
 
 def test_trained_scorer_gains_over_untrained_weights():
-    """What training buys: on a capped cache, the learned policy with the
-    trained network prefetches partitions that are walked next far more often
-    than the same policy with the network's initial (untrained) weights, and
-    so copies fewer partitions.  Prefetching still copies more than the plain
-    rules on this small graph (63 partitions, +25 % to +60 %); on C5 the two load
-    about the same.  Visit counts are identical in all three runs."""
+    """What training buys, and how it compares with the plain rules: on a
+    capped cache, the learned loop with the trained network prefetches
+    partitions that are walked next far more often than with the network's
+    initial (untrained) weights, and so copies fewer partitions; its loads
+    are bounded against the plain scheduler and against the same loop with
+    plain-rule prefetch targets and victims (prefetch_plain 1).  Visit counts
+    are identical in every run."""
     log("trained vs untrained scorer")
     g = lt.Graph.rmat(20, 16)
     part = lt.Partition.build(g, 2 << 20)
@@ -178,12 +198,14 @@ def test_trained_scorer_gains_over_untrained_weights():
     trained = lt.Scorer(part, seed=5)
     untrained = lt.Scorer(part, seed=5)
     untrained.set_params(untrained.get_params())  # the initial weights, marked usable
-    trained.add_trace(plain, True, True)
+    trained.add_trace(plain, True, 2)
     trained.fit(epochs=20)
     res = {}
-    for name, sc in (("trained", trained), ("untrained", untrained)):
-        eng = lt.Engine(g, part, policy="learned", **common)
-        eng.set_scorer(sc)
+    for name, sc, kw in (("trained", trained, {}), ("untrained", untrained, {}), ("rule", None, {"prefetch_plain": 1}),
+                         ("trained-rule-targets", trained, {"prefetch_plain": 2})):
+        eng = lt.Engine(g, part, policy="learned", **kw, **common)
+        if sc is not None:
+            eng.set_scorer(sc)
         eng.pagerank(w, 10)
         counts = eng.pagerank(w, 10)
         assert np.array_equal(counts, ref)
@@ -195,9 +217,12 @@ def test_trained_scorer_gains_over_untrained_weights():
     assert res["trained"][0] >= 0.6
     assert res["trained"][0] >= res["untrained"][0] + 0.1
     assert res["trained"][1] < res["untrained"][1]
-    # prefetch decisions follow copy timing, so load counts vary from run to
-    # run (135-172 seen against the plain scheduler's 108); the bound is loose
-    assert res["trained"][1] <= 2 * plain_loads
+    # prefetching copies partitions ahead of need, so every prefetching loop
+    # loads more than the plain scheduler on this small graph; the bounds
+    # hold with room for the run-to-run spread of timing-driven decisions
+    assert res["trained"][1] <= 1.6 * plain_loads
+    assert res["trained"][1] <= 1.3 * res["rule"][1]
+    assert res["trained-rule-targets"][1] <= 1.3 * res["rule"][1]
 
 
 def test_weights_carry_over_to_another_graph(tmp_path):

@@ -226,6 +226,15 @@ def test_engine_rejects_unknown_pack_mode():
         assert e.value.status == 6 and "pack_partitions" in str(e.value)  # LT_E_ARG
 
 
+def test_engine_rejects_unknown_walkers_per_thread():
+    g = lt.Graph.open(os.path.join(GOLDEN, "rmat_s8_ef4.bcsr"))
+    p = lt.Partition.build(g, 1 << 20)
+    for wpt in (-1, 3, 8):
+        with pytest.raises(lt.LTError) as e:
+            lt.Engine(g, p, walkers_per_thread=wpt)
+        assert e.value.status == 6 and "walkers_per_thread" in str(e.value)  # LT_E_ARG
+
+
 @pytest.mark.skipif(not _no_device(), reason="a HIP device is visible")
 def test_bandwidth_probes_without_device():
     import ctypes

@@ -27,6 +27,7 @@ def _sample(rng, P, with_next=True):
     counts = (rng.integers(0, 5, P) * rng.integers(0, 10000, P)).astype(np.uint32)
     lab = int(rng.integers(0, P)) if with_next else -1
     return {"counts": counts, "resident": (rng.random(P) < 0.5).astype(np.uint8),
+            "age": rng.integers(0, 3 * P, P).astype(np.uint32),
             "recent": rng.integers(-1, P, 8).astype(np.int32), "cur": int(rng.integers(0, P)),
             "w_total": float(counts.sum() * 2 + 1), "next": lab, "keep": rng.random(P).astype(np.float32) * 2,
             "wa": 1.0, "wb": 1.0}
@@ -34,15 +35,16 @@ def _sample(rng, P, with_next=True):
 
 def test_features_by_hand():
     counts = np.array([0, 10, 30, 0], np.uint32)
-    X = M.features(counts, [1, 0, 0, 1], [2, -1, -1, -1, -1, -1, -1, 0], 1, 100.0,
+    X = M.features(counts, [1, 0, 0, 1], [0, 4, 12, 1], [2, -1, -1, -1, -1, -1, 3, 0], 1, 100.0,
                    np.array([1.0, 2.0, 0.5, 0.5]), np.array([0.25, 1.0, 2.0, 0.75]))
     assert X.shape == (4, 16)
     assert np.allclose(X[:, 0], np.log1p([0, 10, 30, 0]) / np.log1p(100))
     assert np.allclose(X[:, 1], [0, 1 / 3, 1, 0])
     assert np.allclose(X[:, 2], np.log1p(np.array([0, 10, 30, 0]) / 10.0))
     assert list(X[:, 3]) == [1, 0, 0, 1] and list(X[:, 4]) == [0, 1, 0, 0]
-    assert list(X[:, 5]) == [0, 0, 1, 0] and list(X[:, 12]) == [1, 0, 0, 0]
-    assert np.all(X[:, 6:12] == 0)
+    assert list(X[:, 5]) == [0, 0, 1, 0] and list(X[:, 11]) == [0, 0, 0, 1]
+    assert np.all(X[:, 6:11] == 0)  # the 8th recent load (partition 0) is not a feature
+    assert list(X[:, 12]) == [0, 0.5, 0.75, 0.2]  # a / (a + P), P = 4
     assert np.allclose(X[:, 15], 0.4)
 
 

@@ -27,7 +27,8 @@ def _scorer(P, seed):
 def _state(P, seed):
     rng = np.random.default_rng(seed)
     counts = (rng.integers(0, 5, P) * rng.integers(0, 100000, P)).astype(np.uint32)
-    return (counts, (rng.random(P) < 0.5).astype(np.uint8), rng.integers(-1, P, 8).astype(np.int32),
+    return (counts, (rng.random(P) < 0.5).astype(np.uint8), rng.integers(0, 3 * P, P).astype(np.uint32),
+            rng.integers(-1, P, 8).astype(np.int32),
             int(rng.integers(0, P)), float(counts.sum() * 3 + 1))
 
 
