@@ -52,9 +52,10 @@ TRAIN_SEED = 0x4C54524C
 
 WORKLOADS = {
     "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10, zc=0),
-    "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10, span="resident"),
+    "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10, span="resident",
+               eng=dict(resident_batch=16)),
     "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26,
-               eng=dict(span="resident", zc_batch=16, pack_partitions=2)),
+               eng=dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2)),
     "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
     "c5": dict(scale=26, ef=27, mb=32, app="pagerank", slots=256, l=10),
 }
@@ -317,7 +318,7 @@ def main():
         belady = {}
         for mode, tag in ((1, "_packed"), (2, "_blocks")):
             eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
-                            pack_partitions=mode, record_trace=1, **base)
+                            resident_batch=16, pack_partitions=mode, record_trace=1, **base)
             plain["tuned" + tag] = measure(eng, wl, w, args.warmup, args.steps,
                                            f"plain/resident S/128 zc-batch 16{tag}")
             if mode == 2:
@@ -336,7 +337,7 @@ def main():
         # prefetch targets follow the plain rule).
         t1 = time.perf_counter()
         eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
-                        pack_partitions=2, record_trace=1, seed=TRAIN_SEED, **base)
+                        resident_batch=16, pack_partitions=2, record_trace=1, seed=TRAIN_SEED, **base)
         n_train = 0
         for i in range(4):
             run_job(eng, wl, w)
@@ -455,7 +456,7 @@ def main():
         log(f"autotuner picked {best} in {tune_s:.1f} s")
         part = lt.Partition.build(g, best["partition_mb"] << 20)
         cfg = dict(num_device_partitions=best["slots"], span=best["span"], device_schedule=best["device_schedule"],
-                   slice_chunks=best["slice_chunks"], zc_batch=best["zc_batch"],
+                   slice_chunks=best["slice_chunks"], zc_batch=best["zc_batch"], resident_batch=best["zc_batch"],
                    pack_partitions=best["pack_partitions"], walkers_per_thread=best["walkers_per_thread"])
         eng = lt.Engine(g, part, **cfg)
         # walk-kernel time for the roofline: the same partitioning, host-scheduled, event timers on
@@ -467,7 +468,7 @@ def main():
                 f"p=0.15, 8 GB cache; autotuned: {best['partition_mb']} MB partitions (P={part.count}), "
                 f"{best['cache_slots']} cache slots, span {best['span']}, "
                 f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}, "
-                f"zero-copy batch {best['zc_batch']}, "
+                f"zero-copy and resident batches of {best['zc_batch']}, "
                 f"{('raw', 'gap-coded', 'block')[best['pack_partitions']]} copies, walkers per lane "
                 f"{best['walkers_per_thread'] or 'by launch size'}")
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],

@@ -115,6 +115,12 @@ typedef struct {
                                        by launch size (4, 2 or 1, host
                                        scheduling; 1 in the device loop), or
                                        exactly 1, 2 or 4                        */
+    int32_t  resident_batch;        /* span RESIDENT: a launch over a resident
+                                       partition's queue also takes the largest
+                                       other resident queues of at least 4096
+                                       walkers, up to this many partitions (at
+                                       most 16; 0 = 16 for LEARNED, 1
+                                       otherwise)                              */
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines
@@ -151,6 +157,8 @@ typedef struct {
     uint64_t zerocopy_launches;     /* walk launches whose source queues were
                                        read over PCIe (part of kernel_launches) */
     double   walk_kernel_zc_ms;     /* their summed time (part of walk_kernel_ms) */
+    uint64_t resident_batched;      /* resident queues walked in another
+                                       resident queue's launch (resident_batch) */
     uint64_t reserved[1];
 } lt_stats;
 
@@ -303,7 +311,8 @@ lt_status lt_make_ppr_queries(const lt_graph *g, uint32_t nq, uint64_t lo, uint6
  * size sizes_mb[0..nsizes) (NULL: 16, 32 and 64 MB) with cache_bytes of
  * cache, span partition or resident,
  * host or device scheduling, 2048 or 8192 chunks per walk launch, zero-copy
- * batches of 1 or 16 (host scheduling, span resident), raw / gap-coded /
+ * and resident batches of 1 or 16 (zc_batch = resident_batch; host
+ * scheduling, span resident), raw / gap-coded /
  * block copies (gap-coded needs host scheduling).  Then it times the three
  * fastest and the fastest of each copy mode again on a sample four times
  * larger, then the fastest of those with each fixed number of walkers per

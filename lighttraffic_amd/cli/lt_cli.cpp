@@ -114,7 +114,7 @@ int main(int argc, char **argv) {
         s = lt_tune_ppr_batch(g, (uint32_t)qsrc.size(), qsrc.data(), qcnt.data(), c.prob, cache, nullptr, 0, 16,
                               &best, nullptr, 0, nullptr);
         if (s != LT_OK) return fail(s, "autotune");
-        printf("autotuned: %u MB partitions, %u cache slots, span %s, %s scheduling, slice %d, zero-copy batch %d, "
+        printf("autotuned: %u MB partitions, %u cache slots, span %s, %s scheduling, slice %d, batches of %d, "
                "%s copies, walkers per lane %d\n",
                best.partition_mb, best.slots, best.span == LT_SPAN_RESIDENT ? "resident" : "partition",
                best.device_schedule ? "device" : "host", best.slice_chunks, best.zc_batch,
@@ -127,6 +127,7 @@ int main(int argc, char **argv) {
         cfg.device_schedule = best.device_schedule;
         cfg.slice_chunks = best.slice_chunks;
         cfg.zc_batch = best.zc_batch;
+        cfg.resident_batch = best.zc_batch;
         cfg.pack_partitions = best.pack_partitions;
         cfg.walkers_per_thread = best.walkers_per_thread;
     }

@@ -151,11 +151,12 @@ struct WalkArgs {
     int zerocopy;         // src slice is read over PCIe
     int span_resident;    // keep walking through HBM-resident partitions
     unsigned long long *stats;
-    // Zero-copy batch (span_resident only): nsrc > 0 consumes the whole queues
-    // of srcs[0..nsrc), all read over PCIe and entered in the partition table
-    // for the launch, so no walker is appended to a queue being consumed.
-    // Queue k occupies [voff[k], voff[k+1]) of one virtual range, each start
-    // a multiple of kChunk.
+    // Batch (span_resident only): nsrc > 0 consumes the whole queues of
+    // srcs[0..nsrc), each entered in the partition table for the launch
+    // (resident slots, or views of the pinned host graph for a zero-copy
+    // batch), so a walker moving between them keeps walking and no queue
+    // being consumed is appended to.  Queue k occupies [voff[k], voff[k+1])
+    // of one virtual range, each start a multiple of kChunk.
     uint32_t nsrc;
     uint32_t srcs[kMaxZcBatch];
     uint64_t voff[kMaxZcBatch + 1];

@@ -260,6 +260,7 @@ struct Engine {
     uint64_t *d_rowptr_all = nullptr;
     uint64_t pending_h2d = 0;      // one-off H2D bytes not yet charged to a run
     uint32_t zc_batch = 1;         // partitions per zero-copy launch
+    uint32_t res_batch = 1;        // partitions per launch over resident queues
     uint32_t slice = kSliceChunks;
     std::vector<hipEvent_t> pool_events;  // reusable prefetch-completion events
     std::vector<float> scores;            // scorer output of the current decision
@@ -290,7 +291,8 @@ struct Engine {
     hipEvent_t ev_evicted = nullptr;  // table updates of prefetch evictions, awaited by the copy stream
     lt_status process(uint32_t p, bool zerocopy, uint64_t walkers_hint);
     lt_status process_zc(uint32_t c, const uint8_t *pending);
-    lt_status process_zc_batch(const std::vector<uint32_t> &parts);
+    lt_status walk_batch(const std::vector<uint32_t> &parts, bool zerocopy);
+    lt_status process_resident(uint32_t c, int kind, uint8_t *prefetched);
     void record(uint32_t p, int kind, int32_t loaded, int32_t evicted);
     lt_status evict(uint32_t p);
     lt_status load_async(uint32_t p);
@@ -553,6 +555,11 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
                                 : (cfg.policy == LT_SCHED_LEARNED && cfg.span == LT_SPAN_RESIDENT ? kMaxZcBatch : 1);
     zc_batch = std::min(zc_batch, kMaxZcBatch);
     if (cfg.span != LT_SPAN_RESIDENT) zc_batch = 1;  // a batch relies on walking through the table
+    res_batch = cfg.resident_batch > 0
+                    ? (uint32_t)cfg.resident_batch
+                    : (cfg.policy == LT_SCHED_LEARNED && cfg.span == LT_SPAN_RESIDENT ? kMaxZcBatch : 1);
+    res_batch = std::min(res_batch, kMaxZcBatch);
+    if (cfg.span != LT_SPAN_RESIDENT) res_batch = 1;
     slice = cfg.slice_chunks > 0 ? (uint32_t)cfg.slice_chunks : kSliceChunks;
     LT_TRY(dmalloc(&d_stats, (size_t)kNumStats * kStatSlots));
     counts.assign(P, 0);
@@ -1094,15 +1101,51 @@ lt_status Engine::process_zc(uint32_t c, const uint8_t *pending) {
     for (uint32_t p : batch) record(p, 1, -1, -1);
     st.zerocopy_calls += batch.size();
     if (batch.size() == 1) return process(c, true, counts[c]);
-    return process_zc_batch(batch);
+    return walk_batch(batch, true);
 }
 
-// One launch over the queues of `parts`, all read from the pinned host graph.
-// The partitions are entered in the partition table as host-memory views for
-// the launch (and flagged in zc_part for the PCIe byte count), so a walker
-// that moves between them keeps walking and no queue being consumed is
-// appended to; every queue is then consumed whole.
-lt_status Engine::process_zc_batch(const std::vector<uint32_t> &parts) {
+// Walk the queue of the resident partition c (its decision already
+// recorded); with resident_batch > 1 also the largest other resident queues
+// of at least one chunk, in one launch, each recorded as a decision of the
+// same kind.  Partitions brought in by a prefetch and walked here count as
+// prefetch hits.
+lt_status Engine::process_resident(uint32_t c, int kind, uint8_t *prefetched) {
+    std::vector<uint32_t> batch{c};
+    uint64_t chunks = (counts[c] + kChunk - 1) / kChunk;
+    if (res_batch > 1 && !spill_on && chunks < slice) {
+        std::vector<uint32_t> cand;
+        for (uint32_t i = 0; i < P; i++)
+            if (i != c && resident[i] && counts[i] >= kChunk) cand.push_back(i);
+        const size_t take = std::min<size_t>(cand.size(), res_batch - 1);
+        std::partial_sort(cand.begin(), cand.begin() + take, cand.end(), [&](uint32_t x, uint32_t y) {
+            return counts[x] != counts[y] ? counts[x] > counts[y] : x < y;
+        });
+        for (size_t k = 0; k < take; k++) {
+            const uint64_t ch = (counts[cand[k]] + kChunk - 1) / kChunk;
+            if (chunks + ch > slice) break;  // the pool holds at most `slice` consumed chunks per launch
+            chunks += ch;
+            batch.push_back(cand[k]);
+        }
+    }
+    if (batch.size() == 1) return process(c, false, counts[c]);
+    for (size_t k = 1; k < batch.size(); k++) {
+        record(batch[k], kind, -1, -1);
+        if (prefetched && prefetched[batch[k]]) {
+            st.prefetch_hits++;
+            prefetched[batch[k]] = 0;
+        }
+    }
+    st.resident_batched += batch.size() - 1;
+    return walk_batch(batch, false);
+}
+
+// One launch over the whole queues of `parts` (span resident).  Resident
+// partitions are in the partition table already; with zerocopy, the
+// partitions are entered in it as views of the pinned host graph for the
+// launch (and flagged in zc_part for the PCIe byte count).  Either way a
+// walker that moves between them keeps walking, and no queue being consumed
+// is appended to.
+lt_status Engine::walk_batch(const std::vector<uint32_t> &parts, bool zerocopy) {
     WalkArgs a{};
     a.app = app;
     a.q = q;
@@ -1133,20 +1176,26 @@ lt_status Engine::process_zc_batch(const std::vector<uint32_t> &parts) {
         a.voff[k + 1] = a.voff[k] + (counts[p] + kChunk - 1) / kChunk * kChunk;
         total += counts[p];
     }
-    hipEvent_t tp = t_zc.begin(s_compute);
-    set_zc_tables_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, d_zc_part, b, 1);
-    LT_HIP(hipGetLastError());
-    hipEvent_t tk = t_kzc.begin(s_compute);
+    EventTimer &phase = zerocopy ? t_zc : t_compute;
+    EventTimer &kt = zerocopy ? t_kzc : t_kernel;
+    hipEvent_t tp = phase.begin(s_compute);
+    if (zerocopy) {
+        set_zc_tables_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, d_zc_part, b, 1);
+        LT_HIP(hipGetLastError());
+    }
+    hipEvent_t tk = kt.begin(s_compute);
     LT_TRY(launch_walk(a, total, s_compute));
-    t_kzc.end(tk, s_compute);
-    st.zerocopy_launches++;
+    kt.end(tk, s_compute);
+    st.zerocopy_launches += zerocopy ? 1 : 0;
     for (uint32_t p : parts) fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, p, 0, 0xFFFFFFFFu, 1, nullptr);
     LT_HIP(hipGetLastError());
     meta_dirty = false;
-    set_zc_tables_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, d_zc_part, b, 0);
-    LT_HIP(hipGetLastError());
+    if (zerocopy) {
+        set_zc_tables_kernel<<<1, 64, 0, s_compute>>>(d_rowptr_tab, d_col_tab, d_zc_part, b, 0);
+        LT_HIP(hipGetLastError());
+    }
     st.kernel_launches++;
-    t_zc.end(tp, s_compute);
+    phase.end(tp, s_compute);
     return LT_OK;
 }
 
@@ -1273,7 +1322,7 @@ lt_status Engine::run_partition(uint32_t p, int32_t &prev_loaded) {
                 if (c < 0) break;
                 st.computing_calls++;
                 record((uint32_t)c, 2, -1, -1);
-                LT_TRY(process((uint32_t)c, false, counts[c]));
+                LT_TRY(process_resident((uint32_t)c, 2, nullptr));
                 LT_HIP(hipStreamSynchronize(s_compute));
             }
             // the pipelined work changed the queues; p's count is re-read below
@@ -1287,7 +1336,7 @@ lt_status Engine::run_partition(uint32_t p, int32_t &prev_loaded) {
     }
     prev_loaded = (int32_t)p;
     st.computing_calls++;
-    return process(p, false, counts[p]);
+    return process_resident(p, 0, nullptr);
 }
 
 // Scheduler::start (src/Scheduler.h:425-438).
@@ -1365,7 +1414,7 @@ lt_status Engine::scheduler_loop_learned() {
             if (p2 < 0) break;
             st.computing_calls++;
             record((uint32_t)p2, 2, -1, -1);
-            LT_TRY(process((uint32_t)p2, false, counts[p2]));
+            LT_TRY(process_resident((uint32_t)p2, 2, prefetched.data()));
             LT_HIP(hipStreamSynchronize(s_compute));
         }
         return read_meta();
@@ -1463,7 +1512,7 @@ lt_status Engine::scheduler_loop_learned() {
             }
         }
         st.computing_calls++;
-        LT_TRY(zc ? process_zc((uint32_t)c, pending.data()) : process((uint32_t)c, false, counts[c]));
+        LT_TRY(zc ? process_zc((uint32_t)c, pending.data()) : process_resident((uint32_t)c, 0, prefetched.data()));
         if (want && !free_slots.empty()) {
             if (!plain_victims) LT_TRY(infer_scores(c));  // while c is walked
             while ((scores_valid || plain_victims) && inflight.size() < depth && !free_slots.empty()) {

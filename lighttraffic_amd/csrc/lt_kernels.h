@@ -584,20 +584,20 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
             ct.vcnt = s_vcnt;
         }
     }
-    if (a.nsrc) {  // zero-copy batch: every source is read over PCIe
+    if (a.nsrc) {  // batch: the whole queues of several partitions, each in the partition table
         const uint64_t vend = a.voff[a.nsrc];
         for (uint64_t base = (uint64_t)blockIdx.x * BW; base < vend; base += (uint64_t)gridDim.x * BW) {
             uint32_t k = 0;
             while (k + 1 < a.nsrc && base >= a.voff[k + 1]) k++;
             const uint32_t src = a.srcs[k];
-            Span sp;
-            sp.rowptr = a.ss.zc_rowptr + a.pt.node_off[src];
-            sp.col = a.ss.zc_col + a.pt.edge_off[src];
-            sp.words = nullptr;
+            Span sp;  // as a walker entering src reads it (walk_residency)
+            sp.rowptr = a.pt.rowptr_tab[src];
+            sp.col = a.pt.col_tab[src];
+            sp.words = a.pt.words_tab[src];
             sp.nb = a.pt.node_off[src];
             sp.ne = a.pt.node_off[src + 1];
             sp.eb = a.pt.edge_off[src];
-            sp.zc = 1;
+            sp.zc = a.pt.zc_part[src];
             const uint64_t local = base - a.voff[k], n = qlen_of(a.q, src);
             if (local < n) walk_pass<APP, WPT>(a, L, s_wsum, src, sp, local, n, ct);  // uniform per workgroup
         }

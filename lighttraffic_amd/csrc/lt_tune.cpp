@@ -43,6 +43,7 @@ lt_status time_trial(const lt_graph *g, const lt_partition *p, const Sample &s, 
     cfg.device_schedule = t.device_schedule;
     cfg.slice_chunks = t.slice_chunks;
     cfg.zc_batch = t.zc_batch;
+    cfg.resident_batch = t.zc_batch;  // one batch size for zero-copy and resident launches
     cfg.pack_partitions = t.pack_partitions;
     cfg.walkers_per_thread = t.walkers_per_thread;
     lt_engine *e = nullptr;

@@ -5,8 +5,9 @@ The C entry point tries engine configurations on a sample of the queries:
 partition size (16/32/64 MB; the cache keeps the same number of bytes), span
 (partition | resident), scheduling on the host or on the GPU (the
 hipGraph-captured device loop), the number of 4096-walker chunks one walk
-launch consumes, (span resident, host scheduling) how many sub-threshold
-partitions one zero-copy launch walks, and whether partitions are copied raw,
+launch consumes, (span resident, host scheduling) how many queues one launch
+walks (sub-threshold partitions over PCIe, or cached partitions), and whether
+partitions are copied raw,
 gap-coded and unpacked in HBM (host scheduling), or kept packed in HBM as
 16-edge blocks.  A small sample walks few steps per partition load, so it
 over-weights the copies: the three fastest configurations, and the fastest of

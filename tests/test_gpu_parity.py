@@ -103,6 +103,11 @@ CONFIGS = [
     ("blocks-device-sched", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "pack_partitions": 2}),
     ("blocks-device-sched-zc", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "pack_partitions": 2,
                                           "zerocopy_threshold": 2000, "span": "resident"}),
+    # several resident queues walked by one launch (span resident)
+    ("resident-batch", 16 << 10, {"num_device_partitions": 6, "span": "resident", "resident_batch": 16}),
+    ("resident-batch-blocks-zc", 16 << 10, {"num_device_partitions": 4, "span": "resident", "resident_batch": 4,
+                                            "zc_batch": 16, "zerocopy_threshold": 3000, "pack_partitions": 2}),
+    ("resident-batch-all-cached", 16 << 10, {"span": "resident", "resident_batch": 16, "walkers_per_thread": 2}),
     # fixed walkers per walk-kernel lane, host and device loops
     ("wpt4-capped", 16 << 10, {"num_device_partitions": 3, "walkers_per_thread": 4}),
     ("wpt2-blocks-batch", 16 << 10, {"num_device_partitions": 4, "span": "resident", "zerocopy_threshold": 3000,
@@ -275,6 +280,33 @@ def test_autotuner_picks_a_trial():
     assert all(t["cache_slots"] >= t["slots"] and t["iterations"] > 0 for t in trials)
     assert {t["zc_batch"] for t in first} == {1, 16}
     assert {t["pack_partitions"] for t in first} == {0, 1, 2}
+
+
+def test_resident_batches_walk_several_queues():
+    """With every partition cached (a second job) and span resident, the
+    first walks find the walkers created in every partition's queue:
+    resident_batch 16 walks
+    up to 16 of those queues (those of at least 4096 walkers) per launch, so
+    there are fewer launches, with counts identical to the oracle and to
+    resident_batch 1."""
+    log("resident batches")
+    g = lt.Graph.rmat(16, 16)
+    part = lt.Partition.build(g, 1 << 20)
+    w = 2 * g.num_node
+    assert part.count >= 3 and w // part.count >= 2 * 4096
+    rc, re, _ = oracle.pagerank(g.rowptr(), g.col(), w, 10, endpoints=True)
+    launches = {}
+    for rb in (1, 16):
+        eng = lt.Engine(g, part, span="resident", resident_batch=rb)
+        eng.pagerank(w, 10)  # loads every partition; the second job starts with all of them cached
+        counts, ends = eng.pagerank(w, 10, endpoints=True)
+        st = eng.stats()
+        eng.close()
+        assert np.array_equal(counts, rc) and np.array_equal(ends, re), rb
+        launches[rb] = st["kernel_launches"]
+        log(f"  resident_batch {rb}: {st['kernel_launches']} launches, {st['resident_batched']} queues batched")
+        assert (st["resident_batched"] > 0) == (rb > 1)
+    assert launches[16] < launches[1]
 
 
 def test_schedule_independence(g12):
