@@ -478,11 +478,13 @@ def main():
                     "roofline_pcie": roofline_pcie(m, h2d_peak),
                     "stats_per_step": m["per_job"],
                     "autotune": {"seconds": tune_s, "sample": "every 16th query; the 3 fastest and the fastest of each copy mode "
-                                           "again on every 4th",
+                                           "again on every 4th; the fastest of those with 1, 2 and 4 walkers per "
+                                           "lane on every 4th",
                                  "best": best,
                                  "trials": [{k: t[k] for k in ("stage", "partition_mb", "span", "device_schedule",
                                                                "slice_chunks", "zc_batch", "pack_partitions",
-                                                               "cache_slots", "running_ms", "h2d_bytes")}
+                                                               "walkers_per_thread", "cache_slots", "running_ms",
+                                                               "h2d_bytes")}
                                             for t in trials]}})
     else:
         ekw = dict(wl.get("eng", {}))
