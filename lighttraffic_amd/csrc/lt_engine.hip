@@ -266,6 +266,8 @@ struct Engine {
     std::vector<float> scores;            // scorer output of the current decision
     bool scores_valid = false;
     uint64_t scores_clock = 0;            // use_clock when the scores were computed
+    bool infer_pending = false;           // an inference launched by infer_launch, not yet collected
+    uint64_t pending_clock = 0;           // use_clock at its launch
     uint64_t w_run = 0;                   // walkers of the current wave
     EventTimer t_load, t_compute, t_zc, t_wload, t_wevict, t_kernel, t_kzc;  // t_kzc: walk launches over PCIe
     Scheduler sched;
@@ -311,6 +313,8 @@ struct Engine {
     lt_status scheduler_loop_device();
     SchedState sched_state(bool device) const;
     lt_status infer_scores(int32_t cur);
+    lt_status infer_launch(int32_t cur);
+    lt_status infer_collect();
     int32_t learned_victim(int32_t a, int32_t b);
     void recent_loads(int32_t *out) const;
     uint32_t age_of(uint32_t p) const;
@@ -1227,6 +1231,15 @@ void Engine::recent_loads(int32_t *out) const {
 lt_status Engine::infer_scores(int32_t cur) {
     scores_valid = false;
     if (!scorer || !scorer_trained(scorer)) return LT_OK;
+    LT_TRY(infer_launch(cur));
+    return infer_collect();
+}
+
+// Launch the inference of the decision that chose `cur` without waiting for
+// it (the state is copied at once); infer_collect makes its scores current.
+lt_status Engine::infer_launch(int32_t cur) {
+    LT_TRY(infer_collect());  // one inference in flight
+    if (!scorer || !scorer_trained(scorer)) return LT_OK;
     std::vector<uint32_t> c32(P), a32(P);
     for (uint32_t i = 0; i < P; i++) {
         c32[i] = (uint32_t)std::min<uint64_t>(counts[i], 0xFFFFFFFFu);
@@ -1239,9 +1252,19 @@ lt_status Engine::infer_scores(int32_t cur) {
     recent_loads(d.recent);
     d.cur = cur;
     d.w_total = (float)w_run;
+    LT_TRY(scorer_infer_launch(scorer, d, nullptr));
+    infer_pending = true;
+    pending_clock = use_clock;
+    return LT_OK;
+}
+
+lt_status Engine::infer_collect() {
+    if (!infer_pending) return LT_OK;
+    infer_pending = false;
+    scores_valid = false;
     scores.resize((size_t)P * kOut);
-    LT_TRY(scorer_infer(scorer, d, nullptr, scores.data()));
-    scores_clock = use_clock;
+    LT_TRY(scorer_infer_wait(scorer, scores.data()));
+    scores_clock = pending_clock;
     bool ok = true;
     for (float v : scores) ok = ok && std::isfinite(v);
     scores_valid = ok;
@@ -1387,6 +1410,7 @@ lt_status Engine::scheduler_loop_learned() {
     // isolates what the network adds); with 2 the scorer picks the victims.
     const bool plain_victims = cfg.prefetch_plain == 1;
     const bool plain_targets = cfg.prefetch_plain != 0;
+    LT_TRY(infer_collect());
     scores_valid = false;
     sched.begin_run();
     // make completed prefetches resident, in order; with `upto`, wait for it
@@ -1422,6 +1446,7 @@ lt_status Engine::scheduler_loop_learned() {
     while (true) {
         LT_TRY(read_meta());
         LT_TRY(retire(-1));
+        LT_TRY(infer_collect());  // the previous decision's scores, computed while its walk ran
         int32_t c = sched.next(counts.data(), prev, sync);
         if (c < 0) {
             if (sync) break;
@@ -1513,9 +1538,12 @@ lt_status Engine::scheduler_loop_learned() {
         }
         st.computing_calls++;
         LT_TRY(zc ? process_zc((uint32_t)c, pending.data()) : process_resident((uint32_t)c, 0, prefetched.data()));
+        // this decision's scores, computed while c is walked: the next
+        // decision's victims, and (prefetch_plain 0) this decision's targets
+        if (!plain_victims) LT_TRY(infer_launch(c));
         if (want && !free_slots.empty()) {
-            if (!plain_victims) LT_TRY(infer_scores(c));  // while c is walked
-            while ((scores_valid || plain_victims) && inflight.size() < depth && !free_slots.empty()) {
+            if (!plain_targets) LT_TRY(infer_collect());
+            while ((scores_valid || plain_targets) && inflight.size() < depth && !free_slots.empty()) {
                 int32_t q = -1;
                 float best = 0.f;
                 for (uint32_t i = 0; i < P; i++) {
@@ -1543,6 +1571,7 @@ lt_status Engine::scheduler_loop_learned() {
         }
     }
     LT_TRY(retire(inflight.empty() ? -1 : inflight.back().part));
+    LT_TRY(infer_collect());
     LT_HIP(hipDeviceSynchronize());
     return LT_OK;
 }

@@ -55,5 +55,11 @@ bool scorer_trained(const Scorer *s);
 // hipStream_t passed as void*, nullptr = the scorer's own stream) and
 // synchronises with it.
 lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream, float *out);
+// The same in two halves: launch the inference of one decision (the state is
+// copied at once), then wait for its scores.  One inference in flight per
+// scorer; a launch waits for the previous one's kernel first.
+lt_status scorer_infer_launch(Scorer *s, const DecisionState &d, void *stream);
+lt_status scorer_infer_wait(Scorer *s, float *out);
+bool scorer_infer_pending(const Scorer *s);
 
 }  // namespace lt

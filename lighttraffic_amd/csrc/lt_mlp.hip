@@ -488,6 +488,7 @@ struct Scorer {
     float *h_Y = nullptr;        // its scores, written by the kernel into pinned mapped memory
     uint8_t *d_stage = nullptr;  // device views of h_stage and h_Y
     float *d_Y = nullptr;
+    hipStream_t pending = nullptr;  // stream of the launched inference not yet waited for
     Samples data;
 
     ~Scorer();
@@ -497,6 +498,7 @@ struct Scorer {
 };
 
 Scorer::~Scorer() {
+    if (pending) (void)hipStreamSynchronize(pending);
     if (stream) (void)hipStreamSynchronize(stream);
     void *ptrs[] = {d_vsh, d_esh, W, G, Mo, Vo, Wb, Wi, d_counts, d_age, d_res, d_recent, d_cur, d_next, d_wtot, d_keep, d_wa,
                     d_wb, X, A1, A2, Y, dY, dA1, dA2, partial, d_loss};
@@ -620,9 +622,16 @@ uint32_t scorer_partitions(const Scorer *s) { return s->P; }
 bool scorer_trained(const Scorer *s) { return s && s->trained; }
 
 // Stage one decision's state in pinned mapped memory in the packed layout of
-// infer_kernel, which reads it over the bus; the scores come back the same way.
-lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float *out) {
+// infer_kernel, which reads it over the bus, and launch the fused MLP on
+// `stream` (nullptr = the scorer's own); scorer_infer_wait returns the scores,
+// which come back the same way.  A launch first waits for the one before it,
+// whose kernel may still be reading the staged state.
+lt_status scorer_infer_launch(Scorer *s, const DecisionState &d, void *stream_v) {
     hipStream_t st = stream_v ? (hipStream_t)stream_v : s->stream;
+    if (s->pending) {
+        MLP_HIP(hipStreamSynchronize(s->pending));
+        s->pending = nullptr;
+    }
     const uint32_t P = s->P;
     uint8_t *h = s->h_stage;
     pack_state_head((int32_t *)h, d.counts, P, d.recent, d.cur, d.w_total);
@@ -636,12 +645,31 @@ lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float 
     }
     infer_kernel<<<(P + 7) / 8, 256, 0, st>>>(P, s->d_stage, s->d_vsh, s->d_esh, s->Wb, s->Wi, s->d_Y);
     MLP_HIP(hipGetLastError());
-    hipError_t e;  // poll: this wait is on the scheduler's critical path
-    while ((e = hipStreamQuery(st)) == hipErrorNotReady) {
-    }
-    MLP_HIP(e);
-    memcpy(out, s->h_Y, sizeof(float) * P * kOut);
+    s->pending = st;
     return LT_OK;
+}
+
+// Scores (out[P*2]) of the last launched inference.  Polls: this wait can be
+// on the scheduler's critical path.
+lt_status scorer_infer_wait(Scorer *s, float *out) {
+    if (!s->pending) {
+        set_error("no inference in flight");
+        return LT_E_INTERNAL;
+    }
+    hipError_t e;
+    while ((e = hipStreamQuery(s->pending)) == hipErrorNotReady) {
+    }
+    s->pending = nullptr;
+    MLP_HIP(e);
+    memcpy(out, s->h_Y, sizeof(float) * s->P * kOut);
+    return LT_OK;
+}
+
+bool scorer_infer_pending(const Scorer *s) { return s && s->pending; }
+
+lt_status scorer_infer(Scorer *s, const DecisionState &d, void *stream_v, float *out) {
+    lt_status st = scorer_infer_launch(s, d, stream_v);
+    return st != LT_OK ? st : scorer_infer_wait(s, out);
 }
 
 // ---- inference without allocation ----------------------------------------------
