@@ -54,8 +54,11 @@ WORKLOADS = {
     "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10, zc=0),
     "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10, span="resident",
                eng=dict(resident_batch=16)),
-    "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26,
-               eng=dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2)),
+    # c3's value: the prefetching loop without a network (largest queues needing
+    # host data copied ahead, plain victims) at the reference's S/256 threshold
+    "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26, zc_div=256,
+               eng=dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2, policy="learned",
+                        prefetch_plain=1)),
     "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
     "c5": dict(scale=26, ef=27, mb=32, app="pagerank", slots=256, l=10),
 }
@@ -283,6 +286,8 @@ def main():
         base["num_device_walkers"] = wl["wdev"]
     if "zc" in wl:  # c1: load the one partition (its 2|V| walkers are below S/256) so it stays cached
         base["zerocopy_threshold"] = wl["zc"]
+    if "zc_div" in wl:
+        base["zerocopy_threshold"] = part.bytes // wl["zc_div"]
     desc = (f"{args.workload}: R-MAT scale {wl['scale']} ef {wl['ef']} (V={g.num_node}, E={g.num_edge}), "
             f"{wl['app']} w=2|V|={w}" + (f" l={wl['l']}" if wl['l'] else " p=0.15") +
             f", {wl['mb']} MB partitions (P={part.count}), cache " +
@@ -515,14 +520,16 @@ def main():
                                 ("blocks", dict(pack_partitions=2), 2),
                                 ("span_resident_zc_batch16_packed", dict(span="resident", zc_batch=16,
                                                                          pack_partitions=1), 2)):
-                e2 = lt.Engine(g, part, **kw, **base)
+                e2 = lt.Engine(g, part, **dict(base, **kw))
                 v = measure(e2, wl, w, 1, k, f"c3/{name}")
                 variants[name] = {x: v[x] for x in keep}
                 e2.close()
             out["c3_variants"] = variants
         out.update({"value": m["steps_per_s"], "ms_per_step": m["ms_per_step"],
                     "median_ms_per_step": m["median_ms_per_step"], "e2e_ms_per_step": m["e2e_ms_per_step"],
-                    "config": {"workload": desc, "scheduler": "greedy",
+                    "config": {"workload": desc,
+                               "scheduler": ("plain-rule prefetch (depth 2, plain victims, no network)"
+                                             if ekw.get("prefetch_plain") == 1 else "greedy"),
                                **{k: v for k, v in ekw.items()}, "device": info["name"]},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, args.workload, span),
                     **({"roofline_pcie": roofline_pcie(m, h2d_peak)} if wl["slots"] else {}),

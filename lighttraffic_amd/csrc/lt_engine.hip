@@ -1386,12 +1386,12 @@ lt_status Engine::scheduler_loop() {
 
 // Learned scheduling.  Decisions follow the plain scheduler (choice, zero-copy
 // rule, pipelined compute during a load), with two changes: while the chosen
-// partition is walked, the scorer's best guess for the next partition that
-// will need host data is copied into a slot ahead of time (prefetch), and a
-// victim is the resident partition with the fewest queued walkers, ties
-// broken by the lowest keep score: trained on Belady's next-use labels, it
-// names the partition needed farthest ahead (the plain rule breaks ties by
-// partition id).  Evictions happen only while the compute stream is idle, and
+// partition is walked, the partitions that will need host data next (the
+// scorer's best guesses, or with prefetch_plain the largest such queues) are
+// copied into slots ahead of time (prefetch), and a victim is the resident
+// partition with the fewest queued walkers, ties broken by the lowest keep
+// score: trained on Belady's next-use labels, it names the partition needed
+// farthest ahead (the plain rule breaks ties by partition id).  Evictions happen only while the compute stream is idle, and
 // a prefetched slot joins the resident table only after its copy completed.
 lt_status Engine::scheduler_loop_learned() {
     int32_t prev = -1, prev_loaded = -1;

@@ -10,7 +10,8 @@ same graph, seed and walker ids:
   C3  R-MAT s25 ef44, 4 GB cache (128 slots of 32 MB), 2^26 device walkers,
       ppr from vertex 0, w = 2|V|, p = 0.15 (app/ppr.cu:22-42): every visit
       count and endpoint, for the reference's rules and for the benchmark's
-      configuration (span resident, zero-copy batches, block slots);
+      configuration (span resident, zero-copy and resident batches, block
+      slots, the prefetching loop with plain targets and victims);
   C4  the same graph, 4,096 batched PPR queries, 8 GB cache: sampled queries
       (each walked by the oracle over its own walker ids) and the endpoint
       total, host- and device-scheduled;
@@ -64,13 +65,18 @@ def s25():
     # does not run.
     pytest.param("reference rules", {}, marks=pytest.mark.skip(
         reason="GPU memory fault in two earlier runs, cause not found; kept off the GPU")),
-    ("bench value: span resident, zc batches, block slots", dict(span="resident", zc_batch=16, pack_partitions=2)),
+    ("bench value: span resident, zc and resident batches, block slots, plain-rule prefetch",
+     dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2, policy="learned", prefetch_plain=1,
+          zc_div=256)),
 ])
 def test_c3_ppr_capped_cache(s25, label, kw):
     log(f"C3: ppr from vertex 0, 4 GB cache, 2^26 device walkers, {label}")
     g = s25
     part = lt.Partition.build(g, 32 << 20)
     w = 2 * g.num_node
+    kw = dict(kw)
+    if "zc_div" in kw:  # zero-copy threshold S / zc_div
+        kw["zerocopy_threshold"] = part.bytes // kw.pop("zc_div")
     eng = lt.Engine(g, part, num_device_partitions=128, num_device_walkers=1 << 26, **kw)
     counts, ends = eng.ppr(w, source=0, prob=0.15, endpoints=True)
     st = eng.stats()

@@ -108,6 +108,11 @@ CONFIGS = [
     ("resident-batch-blocks-zc", 16 << 10, {"num_device_partitions": 4, "span": "resident", "resident_batch": 4,
                                             "zc_batch": 16, "zerocopy_threshold": 3000, "pack_partitions": 2}),
     ("resident-batch-all-cached", 16 << 10, {"span": "resident", "resident_batch": 16, "walkers_per_thread": 2}),
+    # the prefetching loop without a network: largest-queue targets, plain victims
+    ("prefetch-rule-blocks", 16 << 10, {"num_device_partitions": 4, "span": "resident", "policy": "learned",
+                                        "prefetch_plain": 1, "zerocopy_threshold": 3000, "pack_partitions": 2}),
+    ("prefetch-rule-depth3", 16 << 10, {"num_device_partitions": 6, "span": "resident", "policy": "learned",
+                                        "prefetch_plain": 1, "prefetch_depth": 3, "resident_batch": 4}),
     # fixed walkers per walk-kernel lane, host and device loops
     ("wpt4-capped", 16 << 10, {"num_device_partitions": 3, "walkers_per_thread": 4}),
     ("wpt2-blocks-batch", 16 << 10, {"num_device_partitions": 4, "span": "resident", "zerocopy_threshold": 3000,
