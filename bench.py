@@ -253,6 +253,10 @@ def main():
                     help="single-policy workloads only (default partition)")
     ap.add_argument("--epochs", type=int, default=20, help="scorer training epochs (c5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c4-config", default=None, metavar="JSON",
+                    help="c4: skip the autotuner and run this configuration (a JSON object with the fields of the "
+                         "autotuner's pick: partition_mb, slots, span, device_schedule, slice_chunks, zc_batch, "
+                         "pack_partitions, walkers_per_thread)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run copy_probe_kernel over MIB MiB, twice (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -456,7 +460,10 @@ def main():
         src, cnt = g.ppr_queries(wl["nq"])
         QUERIES["table"] = (src, cnt)
         t1 = time.perf_counter()
-        best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=wl["slots"] * (wl["mb"] << 20), log=log)
+        if args.c4_config:  # a fixed configuration, e.g. to profile the walk without the tuner's trials
+            best, trials = dict(json.loads(args.c4_config), stage=0, cache_slots=0), []
+        else:
+            best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=wl["slots"] * (wl["mb"] << 20), log=log)
         tune_s = time.perf_counter() - t1
         log(f"autotuner picked {best} in {tune_s:.1f} s")
         part = lt.Partition.build(g, best["partition_mb"] << 20)
@@ -470,8 +477,9 @@ def main():
         m = measure(eng, wl, w, args.warmup, args.steps, args.workload, timed_engine=teng)
         desc = (f"c4: R-MAT scale {wl['scale']} ef {wl['ef']} (V={g.num_node}, E={g.num_edge}), batched ppr "
                 f"{wl['nq']} queries (sources ~ degree, counts log-uniform [1e3,1e6), {int(cnt.sum())} walkers), "
-                f"p=0.15, 8 GB cache; autotuned: {best['partition_mb']} MB partitions (P={part.count}), "
-                f"{best['cache_slots']} cache slots, span {best['span']}, "
+                f"p=0.15, 8 GB cache; {'autotuned' if trials else 'fixed configuration'}: "
+                f"{best['partition_mb']} MB partitions (P={part.count}), "
+                f"{eng.stats()['cache_slots']} cache slots, span {best['span']}, "
                 f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}, "
                 f"zero-copy and resident batches of {best['zc_batch']}, "
                 f"{('raw', 'gap-coded', 'block')[best['pack_partitions']]} copies, walkers per lane "
