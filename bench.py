@@ -322,20 +322,25 @@ def main():
                 eng = lt.Engine(g, part, span=span, pack_partitions=mode, **base)
                 plain[span + tag] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}{tag}")
                 eng.close()
+        # The learned configuration's settings (a sweep over depth 2/3, thresholds
+        # S/128 and S/192 and the resident rowptr picked these for the learned
+        # and the plain-rule prefetching loops alike)
+        zt, depth = part.bytes // 192, 3
         # plain rules with all of the learned configuration's non-learned settings:
-        # byte break-even zero-copy threshold, zero-copy batches, packed copies
+        # zero-copy threshold, zero-copy and resident batches, packed copies
         belady = {}
         for mode, tag in ((1, "_packed"), (2, "_blocks")):
-            eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
+            eng = lt.Engine(g, part, span="resident", zerocopy_threshold=zt, zc_batch=16,
                             resident_batch=16, pack_partitions=mode, record_trace=1, **base)
             plain["tuned" + tag] = measure(eng, wl, w, args.warmup, args.steps,
-                                           f"plain/resident S/128 zc-batch 16{tag}")
+                                           f"plain/resident S/192 batches 16{tag}")
             if mode == 2:
                 belady["plain_same_settings"] = belady_bound(eng, part)
             eng.close()
         # the learned loop without its network: prefetch the largest queues
         # that need host data, evict by the plain rule (what the network adds)
-        eng = lt.Engine(g, part, span="resident", policy="learned", prefetch_plain=1, pack_partitions=2, **base)
+        eng = lt.Engine(g, part, span="resident", policy="learned", prefetch_plain=1, prefetch_depth=depth,
+                        zerocopy_threshold=zt, pack_partitions=2, **base)
         plain["prefetch_rule_blocks"] = measure(eng, wl, w, args.warmup, args.steps, "plain-rule prefetch blocks")
         eng.close()
         # Training traces: the plain rules with the learned configuration's other
@@ -345,6 +350,8 @@ def main():
         # and Belady's next-use quantity (output 1, which picks victims;
         # prefetch targets follow the plain rule).
         t1 = time.perf_counter()
+        # (at the S/128 threshold: networks trained on S/192 traces chose worse
+        # victims in the S/192 loop, 204 against 175-178 ms a job)
         eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
                         resident_batch=16, pack_partitions=2, record_trace=1, seed=TRAIN_SEED, **base)
         n_train = 0
@@ -359,7 +366,7 @@ def main():
         train_s = time.perf_counter() - t1
         log(f"scorer trained on {n_train} decisions in {train_s:.1f} s (traces {prep_s:.1f} s), "
             f"losses {loss.tolist()}")
-        lcfg = dict(span="resident", policy="learned", prefetch_plain=2)
+        lcfg = dict(span="resident", policy="learned", prefetch_plain=2, prefetch_depth=depth, zerocopy_threshold=zt)
         eng = lt.Engine(g, part, **lcfg, **base)
         eng.set_scorer(sc)
         learned_raw = measure(eng, wl, w, args.warmup, args.steps, "learned/resident")
@@ -402,9 +409,9 @@ def main():
             "e2e_ms_per_step": learned["e2e_ms_per_step"],
             "config": {"workload": desc, "scheduler": "learned (victims: fewest queued walkers, then the BF16 "
                                                       "MLP's lowest keep score; "
-                                                      "prefetch depth 2 of the largest queues needing host data; "
-                                                      "span resident, zero-copy threshold S/128, zero-copy "
-                                                      "batches of 16), partitions kept packed in HBM (16-edge "
+                                                      "prefetch depth 3 of the largest queues needing host data; "
+                                                      "span resident, zero-copy threshold S/192, zero-copy and "
+                                                      "resident batches of 16), partitions kept packed in HBM (16-edge "
                                                       "frame-of-reference blocks, "
                                                       f"{slots} cache slots in the 8 GB)",
                        "device": info["name"]},
@@ -415,8 +422,8 @@ def main():
                       "span_resident": {k: plain["resident"][k] for k in keep},
                       "span_resident_packed": {k: plain["resident_packed"][k] for k in keep},
                       "span_resident_blocks": {k: plain["resident_blocks"][k] for k in keep},
-                      "span_resident_S128_zcbatch16_packed": {k: plain["tuned_packed"][k] for k in keep},
-                      "span_resident_S128_zcbatch16_blocks": {k: plain["tuned_blocks"][k] for k in keep},
+                      "span_resident_S192_batch16_packed": {k: plain["tuned_packed"][k] for k in keep},
+                      "span_resident_S192_batch16_blocks": {k: plain["tuned_blocks"][k] for k in keep},
                       "plain_rule_prefetch_blocks": {k: plain["prefetch_rule_blocks"][k] for k in keep}},
             "learned": {"blocks": {k: learned[k] for k in keep}, "packed": {k: learned_gap[k] for k in keep},
                         "unpacked": {k: learned_raw[k] for k in keep},
@@ -425,7 +432,7 @@ def main():
             "learned_vs_plain": {
                 # the headline pair: the plain rules with every non-learned setting of the
                 # learned configuration (span, zero-copy threshold and batches, block copies)
-                "like_for_like": dict(vs(plain["tuned_blocks"]), plain="span_resident_S128_zcbatch16_blocks",
+                "like_for_like": dict(vs(plain["tuned_blocks"]), plain="span_resident_S192_batch16_blocks",
                                       target={"speedup": 1.3, "h2d_bytes_ratio": 0.75}),
                 "vs_plain_rule_prefetch": dict(vs(plain["prefetch_rule_blocks"]), note=(
                     "the same prefetching loop with the same targets and the plain victim rule")),
@@ -448,7 +455,7 @@ def main():
                 "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
                 "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],
                 "scorer_training_s": train_s, "training_decisions": n_train,
-                "training_walks": f"3 warm pagerank jobs of the plain tuned_blocks configuration, Philox key "
+                "training_walks": f"3 warm pagerank jobs of the plain tuned_blocks configuration at S/128, Philox key "
                                   f"{TRAIN_SEED:#x} (timed jobs: 0x4c54524b)",
                 "labels": "next partition needing host data (output 0), Belady next-use (output 1, use_keep=2)"},
             "roofline": roofline(learned, "c5", "resident"),
