@@ -1262,6 +1262,8 @@ lt_status Engine::infer_collect() {
     if (!infer_pending) return LT_OK;
     infer_pending = false;
     scores_valid = false;
+    // another caller of the scorer (after a run that ended early) waited for it
+    if (!scorer || !scorer_infer_pending(scorer)) return LT_OK;
     scores.resize((size_t)P * kOut);
     LT_TRY(scorer_infer_wait(scorer, scores.data()));
     scores_clock = pending_clock;
@@ -2137,6 +2139,8 @@ lt_status lt_scorer_add_trace(lt_scorer *s_, const lt_engine *e, int32_t use_nex
 
 lt_status lt_engine_set_scorer(lt_engine *e, lt_scorer *s) {
     if (!e) return LT_E_ARG;
+    LT_TRY(e->infer_collect());  // an inference of the previous scorer still in flight
+    e->scores_valid = false;
     e->scorer = (lt::Scorer *)s;
     return LT_OK;
 }
