@@ -162,6 +162,33 @@ def test_learned_policy_keeps_results_bit_exact():
     assert np.array_equal(c0, rc)
 
 
+def test_trace_ages_follow_the_walks():
+    """The age a traced decision sees (lt_engine_trace_state) is the number of
+    walks from HBM (records of kind 0, 2 and 3) since the partition's own
+    last one: recomputed here from the records alone, it matches at every
+    record after each partition's first walk."""
+    log("trace ages")
+    g = lt.Graph.rmat(14, 16)
+    part = lt.Partition.build(g, 32 << 10)
+    eng = lt.Engine(g, part, num_device_partitions=max(3, part.count // 3), record_trace=1, zerocopy_threshold=500)
+    w = 2 * g.num_node
+    eng.pagerank(w, 10)
+    tr = eng.trace()
+    _, _, age, _ = eng.trace_state()
+    assert len(tr) > 20 and age.shape == (len(tr), part.count)
+    last = {}  # partition -> number of HBM walks before and including its last one
+    walks = 0
+    checked = 0
+    for j, (p, kind, *_rest) in enumerate(tr):
+        for q, n in last.items():
+            assert age[j][q] == walks - n, (j, q)
+            checked += 1
+        if kind != 1:
+            walks += 1
+            last[p] = walks
+    assert checked > 100
+
+
 def test_learned_policy_without_trained_scorer_falls_back():
     log("learned policy fallback")
     g = lt.Graph.rmat(12, 8)
