@@ -385,6 +385,13 @@ def main():
             eng.set_scorer(sc)
             ablate[tag] = measure(eng, wl, w, args.warmup, args.steps, f"learned blocks, shortlist {k}")
             eng.close()
+        # the network also choosing the next partition among near-ties of the
+        # greedy choice (queues within 50 % of it)
+        eng = lt.Engine(g, part, pack_partitions=2, learned_next=50, **lcfg, **base)
+        eng.set_scorer(sc)
+        ablate["next50"] = measure(eng, wl, w, args.warmup, args.steps, "learned blocks, next by the network")
+        ablate["next50"]["next_overrides_last_job"] = eng.stats()["next_overrides"]
+        eng.close()
         eng = lt.Engine(g, part, pack_partitions=2, record_trace=1, **lcfg, **base)
         eng.set_scorer(sc)
         learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident blocks")
@@ -429,7 +436,8 @@ def main():
             "learned": {"blocks": {k: learned[k] for k in keep}, "packed": {k: learned_gap[k] for k in keep},
                         "unpacked": {k: learned_raw[k] for k in keep},
                         "blocks_lru_only": {k: ablate["lru_only"][k] for k in keep},
-                        "blocks_shortlist4": {k: ablate["shortlist4"][k] for k in keep}},
+                        "blocks_shortlist4": {k: ablate["shortlist4"][k] for k in keep},
+                        "blocks_next50": {k: ablate["next50"][k] for k in keep + ("next_overrides_last_job",)}},
             "learned_vs_plain": {
                 # the headline pair: the plain rules with every non-learned setting of the
                 # learned configuration (span, zero-copy threshold and batches, block copies)
@@ -442,6 +450,9 @@ def main():
                     "network's victim choice adds")),
                 "vs_shortlist4": dict(vs(ablate["shortlist4"]), note=(
                     "the network choosing among the 4 least recently walked fewest-walker candidates")),
+                "vs_next50": dict(vs(ablate["next50"]), note=(
+                    "the network also choosing the next partition (learned_next 50): when the greedy "
+                    "choice needs host data, the highest 'next' score among queues within 50 % of it")),
                 "vs_reference_plain": vs(ref_plain),
                 "vs_plain_same_copies": {k: vs(v) for k, v in same_copy.items()},
                 "unpacked_learned_vs_reference_plain": {

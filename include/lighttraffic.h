@@ -121,6 +121,14 @@ typedef struct {
                                        walkers, up to this many partitions (at
                                        most 16; 0 = 16 for LEARNED, 1
                                        otherwise)                              */
+    int32_t  learned_next;          /* LEARNED, with a scorer: when the greedy
+                                       choice (Scheduler::nextPartition,
+                                       src/Scheduler.h:368-388) needs host
+                                       data, walk instead the partition with
+                                       the highest "next" score among those
+                                       whose queue is at least this % of the
+                                       greedy choice's (0 = off, the default;
+                                       1-100)                                  */
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines
@@ -159,7 +167,8 @@ typedef struct {
     double   walk_kernel_zc_ms;     /* their summed time (part of walk_kernel_ms) */
     uint64_t resident_batched;      /* resident queues walked in another
                                        resident queue's launch (resident_batch) */
-    uint64_t reserved[1];
+    uint64_t next_overrides;        /* decisions where learned_next walked
+                                       another partition than the greedy one   */
 } lt_stats;
 
 /* ---- errors ------------------------------------------------------------- */

@@ -52,7 +52,8 @@ class Config(ctypes.Structure):
                 ("record_trace", _i32), ("prefetch_depth", _i32), ("prefetch_stick_pct", _i32),
                 ("device_schedule", _i32), ("slice_chunks", _i32), ("zc_batch", _i32),
                 ("rowptr_resident", _i32), ("pack_partitions", _i32), ("prefetch_plain", _i32),
-                ("evict_shortlist", _i32), ("walkers_per_thread", _i32), ("resident_batch", _i32)]
+                ("evict_shortlist", _i32), ("walkers_per_thread", _i32), ("resident_batch", _i32),
+                ("learned_next", _i32)]
 
 
 class Stats(ctypes.Structure):
@@ -69,7 +70,7 @@ class Stats(ctypes.Structure):
                 ("learned_decisions", _u64), ("fallback_decisions", _u64), ("prefetch_issued", _u64),
                 ("prefetch_hits", _u64), ("h2d_bytes_unpacked", _u64), ("packed_image_bytes", _u64),
                 ("packed_col_bytes", _u64), ("cache_slots", _u64), ("zerocopy_launches", _u64),
-                ("walk_kernel_zc_ms", ctypes.c_double), ("resident_batched", _u64), ("reserved", _u64 * 1)]
+                ("walk_kernel_zc_ms", ctypes.c_double), ("resident_batched", _u64), ("next_overrides", _u64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}

@@ -400,6 +400,10 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
         set_error("evict_shortlist must be >= 0 (got %d)", cfg.evict_shortlist);
         return LT_E_ARG;
     }
+    if (cfg.learned_next < 0 || cfg.learned_next > 100) {
+        set_error("learned_next must be 0-100 (got %d)", cfg.learned_next);
+        return LT_E_ARG;
+    }
     if (cfg.prefetch_plain < 0 || cfg.prefetch_plain > 2) {
         set_error("prefetch_plain must be 0, 1 or 2 (got %d)", cfg.prefetch_plain);
         return LT_E_ARG;
@@ -1463,6 +1467,32 @@ lt_status Engine::scheduler_loop_learned() {
                 if ((pending[i] || prefetched[i]) && counts[i] >= zc_threshold && (s2 < 0 || counts[i] > counts[s2]))
                     s2 = (int32_t)i;
             if (s2 >= 0 && counts[s2] * 100 >= counts[c] * (uint64_t)cfg.prefetch_stick_pct) c = s2;
+        }
+        // The scorer's next partition (learned_next): when c still needs host
+        // data, walk instead the highest "next" score among the partitions
+        // whose queue is within learned_next % of c's (the greedy rule's
+        // exclusion of the previous partition kept).  Scores are the last
+        // inference's, normally the previous decision's.
+        if (cfg.learned_next > 0 && !plain_victims && !resident[c] && !pending[c] && scorer &&
+            scorer_trained(scorer)) {
+            if (!scores_recent()) LT_TRY(infer_scores(prev));
+            if (scores_valid) {
+                int32_t s2 = -1;
+                float best = 0.f;
+                for (uint32_t i = 0; i < P; i++) {
+                    if (counts[i] == 0 || counts[i] * 100 < counts[c] * (uint64_t)cfg.learned_next) continue;
+                    if ((int32_t)i == prev && !sync) continue;
+                    const float v = scores[(size_t)i * kOut];
+                    if (s2 < 0 || v > best) {
+                        best = v;
+                        s2 = (int32_t)i;
+                    }
+                }
+                if (s2 >= 0 && s2 != c) {
+                    st.next_overrides++;
+                    c = s2;
+                }
+            }
         }
         sync = false;
         prev = c;

@@ -439,8 +439,12 @@ def test_c5_full_size():
     st = packed.stats()
     assert st["packed_image_bytes"] > 0 and st["packed_col_bytes"] > 1.5 * st["packed_image_bytes"]
     packed.close()
-    # the bench's headline configuration: blocks kept packed in the slots
-    blocks = lt.Engine(g, part, num_device_partitions=256, span="resident", policy="learned", pack_partitions=2)
+    # the bench's headline configuration: blocks kept packed in the slots,
+    # largest-queue prefetch targets at depth 3, the network's victims,
+    # threshold S/192, resident and zero-copy batches, 1 walker per lane
+    blocks = lt.Engine(g, part, num_device_partitions=256, span="resident", policy="learned", pack_partitions=2,
+                       prefetch_plain=2, prefetch_depth=3, zerocopy_threshold=part.bytes // 192,
+                       walkers_per_thread=1)
     blocks.set_scorer(sc)
     bc, be = blocks.pagerank(w, l, endpoints=True)
     assert np.array_equal(bc, counts) and np.array_equal(be, ends)

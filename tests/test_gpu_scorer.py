@@ -146,16 +146,20 @@ def test_learned_policy_keeps_results_bit_exact():
     assert n > 10
     loss = sc.fit(epochs=10, batch=16, lr=1e-3)
     assert np.all(np.isfinite(loss))
-    for span in ("partition", "resident"):
-        eng = lt.Engine(g, part, num_device_partitions=slots, policy="learned", span=span, zerocopy_threshold=500)
+    # the third run lets the scorer pick the next partition among near-ties
+    for span, kw in (("partition", {}), ("resident", {}), ("resident", {"learned_next": 50})):
+        eng = lt.Engine(g, part, num_device_partitions=slots, policy="learned", span=span, zerocopy_threshold=500,
+                        **kw)
         eng.set_scorer(sc)
         counts, ends = eng.pagerank(w, 10, endpoints=True)
         rc, re, _ = oracle.pagerank(g.rowptr(), g.col(), w, 10, endpoints=True)
         assert np.array_equal(counts, rc) and np.array_equal(ends, re)
         st = eng.stats()
         log(f"{span}: prefetch issued {st['prefetch_issued']} hits {st['prefetch_hits']} "
-            f"learned evictions {st['learned_decisions']} loads {st['graph_loading_calls']}")
+            f"learned evictions {st['learned_decisions']} loads {st['graph_loading_calls']} "
+            f"next overrides {st['next_overrides']}")
         assert st["prefetch_issued"] > 0 and st["learned_decisions"] > 0
+        assert (st["next_overrides"] > 0) == ("learned_next" in kw)
         ends2, _ = eng.genericwalk(w, 8)
         re2, _, _ = oracle.genericwalk(g.rowptr(), g.col(), w, 8)
         assert np.array_equal(ends2, re2)
