@@ -55,10 +55,12 @@ WORKLOADS = {
     "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10, span="resident",
                eng=dict(resident_batch=16, walkers_per_thread=2)),
     # c3's value: the prefetching loop without a network (largest queues needing
-    # host data copied ahead, plain victims) at the reference's S/256 threshold
+    # host data copied ahead, plain victims) at the reference's S/256 threshold,
+    # with the whole rowptr in HBM (170 MB, 7 of the cache's slots) so that a
+    # zero-copied move reads only its column over PCIe
     "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26, zc_div=256,
                eng=dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2, policy="learned",
-                        prefetch_plain=1, walkers_per_thread=1)),
+                        prefetch_plain=1, walkers_per_thread=1, rowptr_resident=1)),
     "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
     "c5": dict(scale=26, ef=27, mb=32, app="pagerank", slots=256, l=10),
 }

@@ -65,9 +65,9 @@ def s25():
     # does not run.
     pytest.param("reference rules", {}, marks=pytest.mark.skip(
         reason="GPU memory fault in two earlier runs, cause not found; kept off the GPU")),
-    ("bench value: span resident, zc and resident batches, block slots, plain-rule prefetch",
+    ("bench value: span resident, zc and resident batches, block slots, plain-rule prefetch, resident rowptr",
      dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2, policy="learned", prefetch_plain=1,
-          zc_div=256)),
+          zc_div=256, walkers_per_thread=1, rowptr_resident=1)),
 ])
 def test_c3_ppr_capped_cache(s25, label, kw):
     log(f"C3: ppr from vertex 0, 4 GB cache, 2^26 device walkers, {label}")
