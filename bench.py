@@ -259,6 +259,9 @@ def main():
                     help="c4: skip the autotuner and run this configuration (a JSON object with the fields of the "
                          "autotuner's pick: partition_mb, slots, span, device_schedule, slice_chunks, zc_batch, "
                          "pack_partitions, walkers_per_thread)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="c2, c3: measure the value's configuration only, without the other configurations "
+                         "the line reports beside it (for profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run copy_probe_kernel over MIB MiB, twice (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -531,12 +534,12 @@ def main():
         # at most one other engine alive while a variant is measured (hardware queues)
         teng.close()
         eng.close()
-        if args.workload == "c2":  # the reference's span: walks stop at every partition boundary
+        if args.workload == "c2" and not args.no_variants:  # the reference's span: walks stop at every partition boundary
             e2 = lt.Engine(g, part, span="partition", **base)
             v = measure(e2, wl, w, 1, 2, "c2/span partition")
             out["c2_variants"] = {"span_partition": {k: v[k] for k in ("steps_per_s", "ms_per_step")}}
             e2.close()
-        if args.workload == "c3":
+        if args.workload == "c3" and not args.no_variants:
             keep = ("steps_per_s", "ms_per_step", "median_ms_per_step", "e2e_ms_per_step", "h2d_bytes_per_step",
                     "h2d_GB_per_job", "pcie_GBps")
             variants = {}
