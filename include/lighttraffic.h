@@ -435,6 +435,11 @@ lt_status lt_scorer_set_params(lt_scorer *s, const float *in);
  * (files of the earlier feature set, "LTMLP1", are refused). */
 lt_status lt_scorer_save(const lt_scorer *s, const char *path);
 lt_status lt_scorer_load(lt_scorer *s, const char *path);
+/* The scorer the LEARNED policy of e consults (NULL: the plain rules).  The
+ * engine keeps the pointer: s must stay alive while e may run with it (set
+ * NULL or destroy e first).  A scorer serves one engine at a time (its
+ * inference staging is its own).  LT_E_ARG if s was built for another
+ * partition count. */
 lt_status lt_engine_set_scorer(lt_engine *e, lt_scorer *s);
 
 /* ---- pure scheduler rules (host, no GPU) -------------------------------- */

@@ -1914,9 +1914,27 @@ lt_status lt_engine_create(const lt_graph *g, const lt_partition *p, const lt_co
 
 void lt_engine_destroy(lt_engine *e) { delete e; }
 
+// Walker ids are 64-bit: w * runs walkers must not wrap.
+static bool walkers_fit(uint64_t w, uint32_t runs) {
+    if (w > UINT64_MAX / runs) {
+        set_error("w * runs = %llu * %u walkers exceed 64-bit walker ids", (unsigned long long)w, runs);
+        return false;
+    }
+    return true;
+}
+
+// The termination probability of a PPR walk (app/ppr.cu:22-42), in (0, 1].
+static bool prob_ok(float prob) {
+    if (!(prob > 0.0f && prob <= 1.0f)) {
+        set_error("prob must be in (0, 1] (got %g)", (double)prob);
+        return false;
+    }
+    return true;
+}
+
 lt_status lt_run_pagerank(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, uint32_t *counts_out,
                           uint32_t *endpoints_out) {
-    if (!e || w == 0 || runs == 0) return LT_E_ARG;
+    if (!e || w == 0 || runs == 0 || !walkers_fit(w, runs)) return LT_E_ARG;
     RunSpec rs;
     rs.app = APP_PAGERANK;
     rs.total = w * runs;
@@ -1927,7 +1945,7 @@ lt_status lt_run_pagerank(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, u
 
 lt_status lt_run_ppr(lt_engine *e, uint64_t w, uint32_t source, float prob, uint32_t *counts_out,
                      uint32_t *endpoints_out) {
-    if (!e || w == 0 || !(prob > 0.0f)) return LT_E_ARG;
+    if (!e || w == 0 || !prob_ok(prob)) return LT_E_ARG;
     if (source >= e->V) {
         set_error("source %u >= number of vertices %u", source, e->V);
         return LT_E_ARG;
@@ -1943,7 +1961,7 @@ lt_status lt_run_ppr(lt_engine *e, uint64_t w, uint32_t source, float prob, uint
 
 lt_status lt_run_genericwalk(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, uint32_t *endpoints_out,
                              uint32_t *moves_out) {
-    if (!e || w == 0 || runs == 0) return LT_E_ARG;
+    if (!e || w == 0 || runs == 0 || !walkers_fit(w, runs)) return LT_E_ARG;
     RunSpec rs;
     rs.app = APP_GENERIC;
     rs.total = w * runs;
@@ -1953,7 +1971,7 @@ lt_status lt_run_genericwalk(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs
 }
 
 lt_status lt_run_genericwalk_paths(lt_engine *e, uint64_t w, uint32_t l, uint32_t runs, uint32_t *paths_out) {
-    if (!e || w == 0 || runs == 0 || !paths_out) return LT_E_ARG;
+    if (!e || w == 0 || runs == 0 || !paths_out || !walkers_fit(w, runs)) return LT_E_ARG;
     RunSpec rs;
     rs.app = APP_GENERIC;
     rs.total = w * runs;
@@ -1965,16 +1983,23 @@ lt_status lt_run_genericwalk_paths(lt_engine *e, uint64_t w, uint32_t l, uint32_
 
 lt_status lt_run_ppr_batch(lt_engine *e, uint32_t nq, const uint32_t *sources, const uint64_t *counts, float prob,
                            uint32_t *endpoints_out) {
-    if (!e || nq == 0 || !sources || !counts || !(prob > 0.0f)) return LT_E_ARG;
+    if (!e || nq == 0 || !sources || !counts || !prob_ok(prob)) return LT_E_ARG;
     uint64_t total = 0;
     for (uint32_t i = 0; i < nq; i++) {
         if (sources[i] >= e->V) {
             set_error("query %u source %u >= number of vertices %u", i, sources[i], e->V);
             return LT_E_ARG;
         }
+        if (counts[i] > UINT64_MAX - total) {
+            set_error("the queries' walk counts exceed 64-bit walker ids");
+            return LT_E_ARG;
+        }
         total += counts[i];
     }
-    if (total == 0) return LT_E_ARG;
+    if (total == 0) {
+        set_error("the queries hold no walks");
+        return LT_E_ARG;
+    }
     RunSpec rs;
     rs.app = APP_PPR;
     rs.total = total;
@@ -2169,6 +2194,10 @@ lt_status lt_scorer_add_trace(lt_scorer *s_, const lt_engine *e, int32_t use_nex
 
 lt_status lt_engine_set_scorer(lt_engine *e, lt_scorer *s) {
     if (!e) return LT_E_ARG;
+    if (s && scorer_partitions((lt::Scorer *)s) != e->P) {  // its state and scores are sized by P
+        set_error("scorer built for %u partitions, engine has %u", scorer_partitions((lt::Scorer *)s), e->P);
+        return LT_E_ARG;
+    }
     LT_TRY(e->infer_collect());  // an inference of the previous scorer still in flight
     e->scores_valid = false;
     e->scorer = (lt::Scorer *)s;

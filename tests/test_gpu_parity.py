@@ -314,6 +314,27 @@ def test_resident_batches_walk_several_queues():
     assert launches[16] < launches[1]
 
 
+def test_run_arguments_are_checked(g12):
+    """Bad run arguments are refused with LT_E_ARG before any walk, and the
+    engine runs normally afterwards."""
+    log("run argument checks")
+    part = lt.Partition.build(g12, 16 << 10)
+    eng = lt.Engine(g12, part, num_device_partitions=3)
+    V = g12.num_node
+    bad = [lambda: eng.ppr(100, source=V), lambda: eng.ppr(100, prob=0.0), lambda: eng.ppr(100, prob=1.5),
+           lambda: eng.ppr(100, prob=float("nan")), lambda: eng.pagerank(0, 10), lambda: eng.pagerank(4, 10, runs=0),
+           lambda: eng.ppr_batch([0, 1], [0, 0]), lambda: eng.ppr_batch([0, V], [5, 5]),
+           lambda: eng.ppr_batch([1], [10], prob=2.0)]
+    for k, f in enumerate(bad):
+        with pytest.raises(lt.LTError) as e:
+            f()
+        assert e.value.status == 6, k  # LT_E_ARG
+    counts = eng.ppr(1000, source=7, prob=1.0)  # the largest probability: walks of one move
+    rc, _, _ = oracle.ppr(g12.rowptr(), g12.col(), 1000, source=7, prob=1.0)
+    assert np.array_equal(counts, rc) and int(counts.sum()) == 1000
+    eng.close()
+
+
 def test_schedule_independence(g12):
     """Identical results whatever the cache size, policy and span."""
     log("schedule independence")

@@ -256,6 +256,26 @@ def test_trained_scorer_gains_over_untrained_weights():
     assert res["trained-rule-targets"][1] <= 1.3 * res["rule"][1]
 
 
+def test_scorer_of_another_partition_count_is_refused():
+    """An engine refuses a scorer built for another partition count (its
+    decision state and scores are sized by it)."""
+    log("scorer of another partition count")
+    g = lt.Graph.rmat(12, 8)
+    p1 = lt.Partition.build(g, 16 << 10)
+    p2 = lt.Partition.build(g, 64 << 10)
+    assert p1.count != p2.count
+    eng = lt.Engine(g, p1, num_device_partitions=3, policy="learned")
+    with pytest.raises(lt.LTError) as e:
+        eng.set_scorer(lt.Scorer(p2, seed=1))
+    assert e.value.status == 6 and "partitions" in str(e.value)  # LT_E_ARG
+    eng.set_scorer(lt.Scorer(p1, seed=1))
+    eng.set_scorer(None)
+    counts = eng.pagerank(2 * g.num_node, 10)
+    rc, _, _ = oracle.pagerank(g.rowptr(), g.col(), 2 * g.num_node, 10)
+    assert np.array_equal(counts, rc)
+    eng.close()
+
+
 def test_weights_carry_over_to_another_graph(tmp_path):
     """Weights trained on one R-MAT graph, saved and loaded into the scorer of
     another graph of the same shape (other generator seed, its own partition
