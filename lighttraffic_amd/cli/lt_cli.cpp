@@ -104,6 +104,12 @@ int main(int argc, char **argv) {
     if (c.has_zc) cfg.zerocopy_threshold = c.zc;
     if (c.prefetch_depth >= 0) cfg.prefetch_depth = c.prefetch_depth;
     cfg.zc_batch = c.zc_batch;
+    cfg.resident_batch = c.resident_batch;
+    cfg.walkers_per_thread = c.walkers_per_thread;
+    cfg.slice_chunks = c.slice_chunks;
+    cfg.prefetch_plain = c.prefetch_plain;
+    cfg.evict_shortlist = c.evict_shortlist;
+    cfg.learned_next = c.learned_next;
     cfg.device_schedule = c.device_schedule ? 1 : 0;
     cfg.pack_partitions = c.pack;
     cfg.rowptr_resident = c.rowptr_resident ? 1 : 0;

@@ -17,7 +17,10 @@
 //   --no-pipeline, --zerocopy-threshold <n>, --span partition|resident,
 //   --prefetch-depth <n>, --zc-batch <n>, --device-schedule, --pack,
 //   --pack-blocks, --rowptr-resident, --seed <n>, --timers, --json,
-//   --queries <file>, --tune (with --queries: autotune the configuration)
+//   --queries <file>, --tune (with --queries: autotune the configuration),
+//   --resident-batch <n>, --walkers-per-thread <n>, --slice-chunks <n>,
+//   --prefetch-plain <0|1|2>, --evict-shortlist <n>, --learned-next <pct>
+//   (the lt_config fields of the same names; 0 = the library's default)
 #pragma once
 #include <stdint.h>
 #include <stdio.h>
@@ -42,6 +45,12 @@ struct CliConfig {
     std::string scorer;      // trained scorer weights (lt_scorer_save) for --scheduler learned
     int prefetch_depth = -1;  // -1: library default
     int zc_batch = 0;         // 0: library default
+    int resident_batch = 0;   // 0: library default
+    int walkers_per_thread = 0;
+    int slice_chunks = 0;
+    int prefetch_plain = 0;
+    int evict_shortlist = 0;
+    int learned_next = 0;
     bool device_schedule = false;
     int pack = 0;                // lt_config.pack_partitions: 1 --pack, 2 --pack-blocks
     bool rowptr_resident = false;
@@ -99,6 +108,12 @@ static bool cli_parse(int argc, char **argv, CliConfig &c) {
         else if (a == "--scorer") c.scorer = next("scorer");
         else if (a == "--prefetch-depth") c.prefetch_depth = cli_num<int>(next("prefetch-depth"));
         else if (a == "--zc-batch") c.zc_batch = cli_num<int>(next("zc-batch"));
+        else if (a == "--resident-batch") c.resident_batch = cli_num<int>(next("resident-batch"));
+        else if (a == "--walkers-per-thread") c.walkers_per_thread = cli_num<int>(next("walkers-per-thread"));
+        else if (a == "--slice-chunks") c.slice_chunks = cli_num<int>(next("slice-chunks"));
+        else if (a == "--prefetch-plain") c.prefetch_plain = cli_num<int>(next("prefetch-plain"));
+        else if (a == "--evict-shortlist") c.evict_shortlist = cli_num<int>(next("evict-shortlist"));
+        else if (a == "--learned-next") c.learned_next = cli_num<int>(next("learned-next"));
         else if (a == "--device-schedule") c.device_schedule = true;
         else if (a == "--pack") c.pack = 1;
         else if (a == "--pack-blocks") c.pack = 2;

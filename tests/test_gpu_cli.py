@@ -123,3 +123,14 @@ def test_learned_scheduler_from_saved_scorer(graph_file, tmp_path):
               "--scorer", weights, "--span", "resident", "--pack-blocks", "--json")
     assert "result correct" in out and "H2D bytes unpacked" in out
     assert grep_num(r'"moves": (\d+)', out) == ref_moves
+    # the learned loop's settings as flags: largest-queue prefetch targets at
+    # depth 3 with the network's victims, resident batches, 1 walker per lane,
+    # the network's next partition among near-ties; then the same loop with
+    # plain victims and no scorer
+    for extra in (["--scorer", weights, "--prefetch-plain", 2, "--learned-next", 50, "--evict-shortlist", 2],
+                  ["--prefetch-plain", 1]):
+        out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 60000, "-l", 10, "--scheduler", "learned",
+                  "--span", "resident", "--pack-blocks", "--prefetch-depth", 3, "--resident-batch", 16,
+                  "--walkers-per-thread", 1, "--slice-chunks", 64, "--json", *extra)
+        assert "result correct" in out
+        assert grep_num(r'"moves": (\d+)', out) == ref_moves
