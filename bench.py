@@ -49,16 +49,22 @@ HBM_PEAK_GBS = 8000.0
 # Philox key of the walks the scorer is trained on (the timed walks use the
 # engine's default key, 0x4C54524B): training and timing see different jobs
 TRAIN_SEED = 0x4C54524C
+# C5's learned configuration splits the graph for block slots of this size
+# (lt_partition_build_blocks) instead of the reference's 32 MB raw split
+BLOCK_SLOT = 32 << 20
 
 WORKLOADS = {
     "c1": dict(scale=16, ef=16, mb=32, app="pagerank", slots=0, l=10, zc=0),
     "c2": dict(scale=22, ef=16, mb=32, app="genericwalk", slots=0, l=10, span="resident",
                eng=dict(resident_batch=16, walkers_per_thread=2)),
     # c3's value: the prefetching loop without a network (largest queues needing
-    # host data copied ahead, plain victims) at the reference's S/256 threshold,
-    # with the whole rowptr in HBM (170 MB, 7 of the cache's slots) so that a
-    # zero-copied move reads only its column over PCIe
-    "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26, zc_div=256,
+    # host data copied ahead, plain victims) on the graph split for 32 MiB
+    # block slots (lt_partition_build_blocks) in the same 4 GB, zero-copy
+    # threshold slot/128, with the whole rowptr in HBM (170 MB of the cache) so
+    # that a zero-copied move reads only its column over PCIe; the variants
+    # use the reference's 32 MB split and its S/256 threshold
+    "c3": dict(scale=25, ef=44, mb=32, app="ppr", slots=128, l=0, wdev=1 << 26, zc_div=256, blocks_mb=32,
+               blocks_zc_div=128,
                eng=dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2, policy="learned",
                         prefetch_plain=1, walkers_per_thread=1, rowptr_resident=1)),
     "c4": dict(scale=25, ef=44, mb=32, app="ppr_batch", slots=256, l=0, nq=4096),
@@ -311,55 +317,66 @@ def main():
         # One engine at a time (plus the scorer's stream): HIP maps streams onto
         # GPU_MAX_HW_QUEUES hardware queues (4 by default), and an engine whose
         # copy and compute streams share one loses its copy/compute overlap.
-        sc = lt.Scorer(part, seed=1)
         plain = {}
         keep = ("steps_per_s", "ms_per_step", "median_ms_per_step", "e2e_ms_per_step", "h2d_bytes_per_step",
                 "h2d_GB_per_job", "pcie_GBps")
+        # The reference's 32 MB split (EqualSizeStrategy): its rules with raw,
+        # gap-coded (1) and block (2) copies, span partition and resident
         for span in ("partition", "resident"):
             eng = lt.Engine(g, part, span=span, **base)
             plain[span] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}")
             if span == "resident":
                 ref = eng.pagerank(w, wl["l"])
             eng.close()
-            # the same rules with packed partition copies: gap-coded and unpacked
-            # in HBM (1), or blocks kept packed in the slots (2)
             for mode, tag in ((1, "_packed"), (2, "_blocks")):
                 eng = lt.Engine(g, part, span=span, pack_partitions=mode, **base)
                 plain[span + tag] = measure(eng, wl, w, args.warmup, args.steps, f"plain/{span}{tag}")
                 eng.close()
-        # The learned configuration's settings (a sweep over depth 2/3, thresholds
-        # S/128 and S/192 and the resident rowptr picked these for the learned
-        # and the plain-rule prefetching loops alike)
-        zt, depth = part.bytes // 192, 3
         base["walkers_per_thread"] = 1  # from here on: 1 walker per lane measured 1-2 % faster
-        # plain rules with all of the learned configuration's non-learned settings:
-        # zero-copy threshold, zero-copy and resident batches, packed copies
-        belady = {}
-        for mode, tag in ((1, "_packed"), (2, "_blocks")):
-            eng = lt.Engine(g, part, span="resident", zerocopy_threshold=zt, zc_batch=16,
-                            resident_batch=16, pack_partitions=mode, record_trace=1, **base)
+        # the same rules with zero-copy threshold S/192 and zero-copy and resident
+        # batches of 16, on that split
+        for mode, tag in ((1, "_packed"), (2, "_blocks_32MB_split")):
+            eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 192, zc_batch=16,
+                            resident_batch=16, pack_partitions=mode, **base)
             plain["tuned" + tag] = measure(eng, wl, w, args.warmup, args.steps,
                                            f"plain/resident S/192 batches 16{tag}")
-            if mode == 2:
-                belady["plain_same_settings"] = belady_bound(eng, part)
             eng.close()
+        g.drop_packed()  # the 32 MB split's images; the balanced split gets its own
+        # The learned configuration's table: partitions split so that each one's
+        # rowptr slice + block image fills a 32 MiB block slot
+        # (lt_partition_build_blocks); the 8 GB cache then holds 0.87 of the
+        # graph's partitions instead of 0.72 with the 32 MB split, whose block
+        # slots are sized by its largest image.
+        t1 = time.perf_counter()
+        pblk = lt.Partition.build_blocks(g, BLOCK_SLOT)
+        split_s = time.perf_counter() - t1
+        bbase = dict(base, num_device_partitions=(wl["slots"] * (wl["mb"] << 20)) // pblk.bytes)
+        log(f"balanced split: {pblk.count} partitions of <= {BLOCK_SLOT >> 20} MiB block slots in {split_s:.1f} s")
+        # The learned configuration's settings (a sweep over prefetch depth 2/3
+        # and thresholds slot/128 and slot/192 picked them)
+        zt, depth = BLOCK_SLOT // 128, 2
+        common = dict(span="resident", zerocopy_threshold=zt, zc_batch=16, resident_batch=16, pack_partitions=2)
+        belady = {}
+        # plain rules with all of the learned configuration's non-learned settings
+        eng = lt.Engine(g, pblk, record_trace=1, **common, **bbase)
+        plain["tuned_blocks"] = measure(eng, wl, w, args.warmup, args.steps, "plain/balanced blocks slot/128")
+        belady["plain_same_settings"] = belady_bound(eng, pblk)
+        eng.close()
         # the learned loop without its network: prefetch the largest queues
         # that need host data, evict by the plain rule (what the network adds)
-        eng = lt.Engine(g, part, span="resident", policy="learned", prefetch_plain=1, prefetch_depth=depth,
-                        zerocopy_threshold=zt, pack_partitions=2, **base)
+        eng = lt.Engine(g, pblk, policy="learned", prefetch_plain=1, prefetch_depth=depth, **common, **bbase)
         plain["prefetch_rule_blocks"] = measure(eng, wl, w, args.warmup, args.steps, "plain-rule prefetch blocks")
         eng.close()
         # Training traces: the plain rules with the learned configuration's other
-        # settings, on walks of another Philox key (held out from the timed
-        # jobs), warm jobs only (the first fills an empty cache and evicts
-        # nothing).  Labels: the next partition needing host data (output 0)
-        # and Belady's next-use quantity (output 1, which picks victims;
-        # prefetch targets follow the plain rule).
+        # settings (threshold slot/192), on walks of another Philox key (held
+        # out from the timed jobs), warm jobs only (the first fills an empty
+        # cache and evicts nothing).  Labels: the next partition needing host
+        # data (output 0) and Belady's next-use quantity (output 1, which
+        # picks victims; prefetch targets follow the plain rule).
         t1 = time.perf_counter()
-        # (at the S/128 threshold: networks trained on S/192 traces chose worse
-        # victims in the S/192 loop, 204 against 175-178 ms a job)
-        eng = lt.Engine(g, part, span="resident", zerocopy_threshold=part.bytes // 128, zc_batch=16,
-                        resident_batch=16, pack_partitions=2, record_trace=1, seed=TRAIN_SEED, **base)
+        sc = lt.Scorer(pblk, seed=1)
+        eng = lt.Engine(g, pblk, record_trace=1, seed=TRAIN_SEED, **dict(common, zerocopy_threshold=BLOCK_SLOT // 192),
+                        **bbase)
         n_train = 0
         for i in range(4):
             run_job(eng, wl, w)
@@ -372,44 +389,36 @@ def main():
         train_s = time.perf_counter() - t1
         log(f"scorer trained on {n_train} decisions in {train_s:.1f} s (traces {prep_s:.1f} s), "
             f"losses {loss.tolist()}")
-        lcfg = dict(span="resident", policy="learned", prefetch_plain=2, prefetch_depth=depth, zerocopy_threshold=zt)
-        eng = lt.Engine(g, part, **lcfg, **base)
-        eng.set_scorer(sc)
-        learned_raw = measure(eng, wl, w, args.warmup, args.steps, "learned/resident")
-        eng.close()
-        eng = lt.Engine(g, part, pack_partitions=1, **lcfg, **base)
-        eng.set_scorer(sc)
-        learned_gap = measure(eng, wl, w, args.warmup, args.steps, "learned/resident packed")
-        eng.close()
+        lcfg = dict(common, policy="learned", prefetch_plain=2, prefetch_depth=depth)
         # what the network's victim choice adds: the same loop evicting the
         # least recently walked candidate (shortlist 1), and the network
         # choosing among the 4 least recently walked (shortlist 4)
         ablate = {}
         for k, tag in ((1, "lru_only"), (4, "shortlist4")):
-            eng = lt.Engine(g, part, pack_partitions=2, evict_shortlist=k, **lcfg, **base)
+            eng = lt.Engine(g, pblk, evict_shortlist=k, **lcfg, **bbase)
             eng.set_scorer(sc)
             ablate[tag] = measure(eng, wl, w, args.warmup, args.steps, f"learned blocks, shortlist {k}")
             eng.close()
         # the network also choosing the next partition among near-ties of the
         # greedy choice (queues within 50 % of it)
-        eng = lt.Engine(g, part, pack_partitions=2, learned_next=50, **lcfg, **base)
+        eng = lt.Engine(g, pblk, learned_next=50, **lcfg, **bbase)
         eng.set_scorer(sc)
         ablate["next50"] = measure(eng, wl, w, args.warmup, args.steps, "learned blocks, next by the network")
         ablate["next50"]["next_overrides_last_job"] = eng.stats()["next_overrides"]
         eng.close()
-        eng = lt.Engine(g, part, pack_partitions=2, record_trace=1, **lcfg, **base)
+        eng = lt.Engine(g, pblk, record_trace=1, **lcfg, **bbase)
         eng.set_scorer(sc)
         learned = measure(eng, wl, w, args.warmup, args.steps, "learned/resident blocks")
-        belady["learned"] = belady_bound(eng, part)
+        belady["learned"] = belady_bound(eng, pblk)
         same = bool(np.array_equal(eng.pagerank(w, wl["l"]), ref))
         slots = eng.stats()["cache_slots"]
         eng.close()
-        teng = lt.Engine(g, part, pack_partitions=2, event_timers=1, **lcfg, **base)
+        teng = lt.Engine(g, pblk, event_timers=1, **lcfg, **bbase)
         teng.set_scorer(sc)
         learned.update(timed_walk(teng, wl, w))
         teng.close()
         ref_plain = plain["partition"]
-        same_copy = {k: v for k, v in plain.items() if k.endswith("blocks")}
+        same_copy = {k: v for k, v in plain.items() if "blocks" in k}
 
         def vs(m):
             return {"speedup": learned["steps_per_s"] / m["steps_per_s"],
@@ -422,11 +431,14 @@ def main():
             "e2e_ms_per_step": learned["e2e_ms_per_step"],
             "config": {"workload": desc, "scheduler": "learned (victims: fewest queued walkers, then the BF16 "
                                                       "MLP's lowest keep score; "
-                                                      "prefetch depth 3 of the largest queues needing host data; "
-                                                      "span resident, zero-copy threshold S/192, zero-copy and "
-                                                      "resident batches of 16), partitions kept packed in HBM (16-edge "
-                                                      "frame-of-reference blocks, "
-                                                      f"{slots} cache slots in the 8 GB)",
+                                                      f"prefetch depth {depth} of the largest queues needing host "
+                                                      "data; span resident, zero-copy threshold slot/128, "
+                                                      "zero-copy and resident batches of 16)",
+                       "partitions": (f"{pblk.count} partitions sized for {BLOCK_SLOT >> 20} MiB block slots "
+                                      f"(lt_partition_build_blocks), kept packed in HBM as 16-edge "
+                                      f"frame-of-reference blocks, {slots} cache slots in the 8 GB; the plain "
+                                      f"variants named *_32MB_split and span_* use the reference's 32 MB split "
+                                      f"(P={part.count})"),
                        "device": info["name"]},
             "h2d_bytes_per_step": learned["h2d_bytes_per_step"],
             "plain": {"span_partition": {k: ref_plain[k] for k in keep},
@@ -436,17 +448,18 @@ def main():
                       "span_resident_packed": {k: plain["resident_packed"][k] for k in keep},
                       "span_resident_blocks": {k: plain["resident_blocks"][k] for k in keep},
                       "span_resident_S192_batch16_packed": {k: plain["tuned_packed"][k] for k in keep},
-                      "span_resident_S192_batch16_blocks": {k: plain["tuned_blocks"][k] for k in keep},
+                      "span_resident_S192_batch16_blocks_32MB_split": {
+                          k: plain["tuned_blocks_32MB_split"][k] for k in keep},
+                      "balanced_blocks_slot128_batch16": {k: plain["tuned_blocks"][k] for k in keep},
                       "plain_rule_prefetch_blocks": {k: plain["prefetch_rule_blocks"][k] for k in keep}},
-            "learned": {"blocks": {k: learned[k] for k in keep}, "packed": {k: learned_gap[k] for k in keep},
-                        "unpacked": {k: learned_raw[k] for k in keep},
+            "learned": {"blocks": {k: learned[k] for k in keep},
                         "blocks_lru_only": {k: ablate["lru_only"][k] for k in keep},
                         "blocks_shortlist4": {k: ablate["shortlist4"][k] for k in keep},
                         "blocks_next50": {k: ablate["next50"][k] for k in keep + ("next_overrides_last_job",)}},
             "learned_vs_plain": {
                 # the headline pair: the plain rules with every non-learned setting of the
-                # learned configuration (span, zero-copy threshold and batches, block copies)
-                "like_for_like": dict(vs(plain["tuned_blocks"]), plain="span_resident_S192_batch16_blocks",
+                # learned configuration (split, span, zero-copy threshold and batches, block copies)
+                "like_for_like": dict(vs(plain["tuned_blocks"]), plain="balanced_blocks_slot128_batch16",
                                       target={"speedup": 1.3, "h2d_bytes_ratio": 0.75}),
                 "vs_plain_rule_prefetch": dict(vs(plain["prefetch_rule_blocks"]), note=(
                     "the same prefetching loop with the same targets and the plain victim rule")),
@@ -460,20 +473,19 @@ def main():
                     "choice needs host data, the highest 'next' score among queues within 50 % of it")),
                 "vs_reference_plain": vs(ref_plain),
                 "vs_plain_same_copies": {k: vs(v) for k, v in same_copy.items()},
-                "unpacked_learned_vs_reference_plain": {
-                    "speedup": learned_raw["steps_per_s"] / ref_plain["steps_per_s"],
-                    "h2d_bytes_ratio": learned_raw["h2d_bytes_per_step"] / ref_plain["h2d_bytes_per_step"]},
-                "gap_coded_learned_vs_plain_gap_coded": {
-                    k: {"speedup": learned_gap["steps_per_s"] / plain[k]["steps_per_s"],
-                        "h2d_bytes_ratio": learned_gap["h2d_bytes_per_step"] / plain[k]["h2d_bytes_per_step"]}
-                    for k in plain if k.endswith("packed")},
+                "balanced_split_gain_plain": {
+                    "speedup": plain["tuned_blocks"]["steps_per_s"] / plain["tuned_blocks_32MB_split"]["steps_per_s"],
+                    "h2d_bytes_ratio": (plain["tuned_blocks"]["h2d_bytes_per_step"]
+                                        / plain["tuned_blocks_32MB_split"]["h2d_bytes_per_step"]),
+                    "note": "the plain rules on the balanced split against the same rules on the 32 MB split "
+                            "(threshold slot/128 against S/192)"},
                 "belady": belady,
                 "visit_counts_identical": same,
                 "prefetch_issued_per_job": learned["per_job"]["prefetch_issued"],
                 "prefetch_hits_per_job": learned["per_job"]["prefetch_hits"],
                 "scorer_training_s": train_s, "training_decisions": n_train,
-                "training_walks": f"3 warm pagerank jobs of the plain tuned_blocks configuration at S/128, Philox key "
-                                  f"{TRAIN_SEED:#x} (timed jobs: 0x4c54524b)",
+                "training_walks": f"3 warm pagerank jobs of the plain rules on the balanced split at slot/192, "
+                                  f"Philox key {TRAIN_SEED:#x} (timed jobs: 0x4c54524b)",
                 "labels": "next partition needing host data (output 0), Belady next-use (output 1, use_keep=2)"},
             "roofline": roofline(learned, "c5", "resident"),
             "roofline_pcie": roofline_pcie(learned, h2d_peak),
@@ -527,8 +539,13 @@ def main():
         ekw = dict(wl.get("eng", {}))
         ekw["span"] = args.span or ekw.get("span", wl.get("span", "partition"))
         span = ekw["span"]
-        eng = lt.Engine(g, part, **ekw, **base)
-        teng = lt.Engine(g, part, event_timers=1, **ekw, **base)
+        vpart, vbase = part, base
+        if "blocks_mb" in wl:  # the value's engine: the graph split for block slots, the same cache bytes
+            vpart = lt.Partition.build_blocks(g, wl["blocks_mb"] << 20)
+            vbase = dict(base, num_device_partitions=(wl["slots"] * (wl["mb"] << 20)) // vpart.bytes,
+                         zerocopy_threshold=(wl["blocks_mb"] << 20) // wl["blocks_zc_div"])
+        eng = lt.Engine(g, vpart, **ekw, **vbase)
+        teng = lt.Engine(g, vpart, event_timers=1, **ekw, **vbase)
         run_job(teng, wl, w)
         m = measure(eng, wl, w, args.warmup, args.steps, args.workload, timed_engine=teng)
         # at most one other engine alive while a variant is measured (hardware queues)
@@ -562,7 +579,10 @@ def main():
                     "config": {"workload": desc,
                                "scheduler": ("plain-rule prefetch (depth 2, plain victims, no network)"
                                              if ekw.get("prefetch_plain") == 1 else "greedy"),
-                               **{k: v for k, v in ekw.items()}, "device": info["name"]},
+                               **{k: v for k, v in ekw.items()}, "device": info["name"],
+                               **({"partitions": f"{vpart.count} partitions sized for {wl['blocks_mb']} MiB "
+                                                 f"block slots (lt_partition_build_blocks), zero-copy threshold "
+                                                 f"slot/{wl['blocks_zc_div']}"} if vpart is not part else {})},
                     "h2d_bytes_per_step": m["h2d_bytes_per_step"], "roofline": roofline(m, args.workload, span),
                     **({"roofline_pcie": roofline_pcie(m, h2d_peak)} if wl["slots"] else {}),
                     "stats_per_step": m["per_job"]})

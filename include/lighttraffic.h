@@ -258,6 +258,18 @@ lt_status lt_partition_pack_blocks(const lt_graph *g, const lt_partition *p, uin
  * lt_stats.cache_slots.  No counterpart in the reference. */
 lt_status lt_partition_slot_layout(const lt_graph *g, const lt_partition *p, int32_t pack_partitions,
                                    uint32_t num_device_partitions, uint64_t *slot_bytes, uint32_t *slots);
+/* Greedy contiguous split sized for block slots: each partition's rowptr
+ * slice ((nodes+1)*8 bytes) plus its block image (lt_partition_pack_blocks;
+ * its raw columns when the image is not smaller) fits slot_bytes, and the
+ * next vertex would not.  The reference splits by raw bytes only
+ * (EqualSizeStrategy::init, PartitionStrategy.h:189-235), so its block
+ * images vary in size and a block slot, sized by the largest, goes partly
+ * unused; balanced images let the same HBM cache hold more of the graph.
+ * partition_bytes is set to the largest partition's raw size
+ * ((nodes+1)*8 + edges*4), so raw slots hold these partitions as well.
+ * LT_E_OVERSIZE_VERTEX if one vertex does not fit.  No counterpart in the
+ * reference. */
+lt_status lt_partition_build_blocks(const lt_graph *g, uint64_t slot_bytes, lt_partition **out);
 /* An engine with pack_partitions = 1 or 2 builds the packed images of its
  * partition table once (pinned host memory; on R-MAT about half the graph's
  * column bytes for 1, 0.6 for 2) and caches them on the graph for later

@@ -112,6 +112,7 @@ _sig("lt_partition_find", _u32, _vp, _u32)
 _sig("lt_partition_free", None, _vp)
 _sig("lt_partition_pack", ctypes.c_int, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64))
 _sig("lt_partition_pack_blocks", ctypes.c_int, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64))
+_sig("lt_partition_build_blocks", ctypes.c_int, _vp, _u64, ctypes.POINTER(_vp))
 _sig("lt_partition_slot_layout", ctypes.c_int, _vp, _vp, ctypes.c_int32, _u32, ctypes.POINTER(_u64),
      ctypes.POINTER(_u32))
 _sig("lt_graph_drop_packed", None, _vp)
@@ -287,6 +288,13 @@ class Partition:
     def build(cls, graph, max_bytes):
         h = _vp()
         _check(_lib.lt_partition_build(graph._h, max_bytes, ctypes.byref(h)), "lt_partition_build")
+        return cls(h)
+
+    @classmethod
+    def build_blocks(cls, graph, slot_bytes):
+        """Partitions whose rowptr slice + block image fit slot_bytes each."""
+        h = _vp()
+        _check(_lib.lt_partition_build_blocks(graph._h, slot_bytes, ctypes.byref(h)), "lt_partition_build_blocks")
         return cls(h)
 
     @classmethod

@@ -299,4 +299,90 @@ lt_status lt_partition_pack_blocks(const lt_graph *g, const lt_partition *p, uin
     return pack_one("lt_partition_pack_blocks", 2, g, p, part, out, cap, bytes_out);
 }
 
+namespace {
+// Bytes a block slot needs for a vertex range while its vertices are added in
+// order: the rowptr slice plus the block image pack_blocks_encode writes for
+// the range's columns, or the raw columns when that image is not smaller
+// (the engine then copies them as they are; lt::slot_layout sizes slots the
+// same way).  Blocks are 16 consecutive edges counted from the range's first.
+struct BlockSizer {
+    uint64_t nv = 0, ne = 0;    // vertices and edges added
+    uint64_t nblk = 0, bits = 0;  // complete blocks and their stream bits
+    uint32_t open = 0, lo = 0, hi = 0;  // edges of the open block and their range
+
+    void add(const uint32_t *c, uint64_t d) {
+        nv++;
+        ne += d;
+        for (uint64_t k = 0; k < d; k++) {
+            const uint32_t x = c[k];
+            if (open == 0) {
+                lo = hi = x;
+            } else {
+                lo = std::min(lo, x);
+                hi = std::max(hi, x);
+            }
+            if (++open == lt::kBlockEdges) {
+                bits += (uint64_t)lt::kBlockEdges * lt::bitlen(hi - lo);
+                nblk++;
+                open = 0;
+            }
+        }
+    }
+    uint64_t slot_bytes() const {
+        const uint64_t tb = bits + (open ? (uint64_t)lt::kBlockEdges * lt::bitlen(hi - lo) : 0);
+        const uint64_t nb = nblk + (open ? 1 : 0);
+        const uint64_t img = tb > UINT32_MAX ? UINT64_MAX : 8 * (nb + 1) + 4 * ((tb + 31) / 32 + 2);
+        return (nv + 1) * 8 + std::min(img, 4 * ne);
+    }
+};
+}  // namespace
+
+lt_status lt_partition_build_blocks(const lt_graph *g, uint64_t slot_bytes, lt_partition **out) {
+    if (!g || !out || slot_bytes == 0) {
+        lt::set_error("lt_partition_build_blocks: null argument or zero slot size");
+        return LT_E_ARG;
+    }
+    std::unique_ptr<lt_partition> p(new lt_partition);
+    p->node_off.push_back(0);
+    p->edge_off.push_back(0);
+    const uint64_t *rp = g->rowptr;
+    uint64_t raw_max = 0;
+    uint32_t n = 0;  // first vertex of the open partition
+    auto close = [&](uint32_t end) {
+        raw_max = std::max<uint64_t>(raw_max, ((uint64_t)end - n + 1) * 8 + (rp[end] - rp[n]) * 4);
+        p->node_off.push_back(end);
+        p->edge_off.push_back(rp[end]);
+    };
+    BlockSizer cur;
+    for (uint32_t v = 0; v < g->V; v++) {
+        BlockSizer next = cur;
+        next.add(g->col + rp[v], rp[v + 1] - rp[v]);
+        if (next.slot_bytes() <= slot_bytes) {
+            cur = next;
+            continue;
+        }
+        // v does not fit the open partition: it starts the next one, where
+        // it must fit alone (when v == n it was alone already)
+        if (v > n) {
+            close(v);
+            n = v;
+            cur = BlockSizer();
+            cur.add(g->col + rp[v], rp[v + 1] - rp[v]);
+        }
+        if (v == n && (cur.nv == 0 || cur.slot_bytes() > slot_bytes)) {
+            lt::set_error("Node %u has degree %llu. It cannot fit into a block slot of %llu bytes", v,
+                          (unsigned long long)(rp[v + 1] - rp[v]), (unsigned long long)slot_bytes);
+            return LT_E_OVERSIZE_VERTEX;
+        }
+        if (p->node_off.size() > 65535) {
+            lt::set_error("more than 65535 partitions (u16 partition ids)");
+            return LT_E_ARG;
+        }
+    }
+    close(g->V);
+    p->bytes = raw_max;
+    *out = p.release();
+    return LT_OK;
+}
+
 }  // extern "C"

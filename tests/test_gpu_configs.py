@@ -65,24 +65,32 @@ def s25():
     # does not run.
     pytest.param("reference rules", {}, marks=pytest.mark.skip(
         reason="GPU memory fault in two earlier runs, cause not found; kept off the GPU")),
-    ("bench value: span resident, zc and resident batches, block slots, plain-rule prefetch, resident rowptr",
+    ("32 MB split: span resident, zc and resident batches, block slots, plain-rule prefetch, resident rowptr",
      dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2, policy="learned", prefetch_plain=1,
           zc_div=256, walkers_per_thread=1, rowptr_resident=1)),
+    ("bench value: the same on the split for 32 MiB block slots, threshold slot/128",
+     dict(span="resident", zc_batch=16, resident_batch=16, pack_partitions=2, policy="learned", prefetch_plain=1,
+          walkers_per_thread=1, rowptr_resident=1, blocks_mb=32, zerocopy_threshold=(32 << 20) // 128)),
 ])
 def test_c3_ppr_capped_cache(s25, label, kw):
     log(f"C3: ppr from vertex 0, 4 GB cache, 2^26 device walkers, {label}")
     g = s25
-    part = lt.Partition.build(g, 32 << 20)
-    w = 2 * g.num_node
     kw = dict(kw)
+    slots = 128
+    if "blocks_mb" in kw:  # the split for block slots, in the same 4 GB
+        part = lt.Partition.build_blocks(g, kw.pop("blocks_mb") << 20)
+        slots = (128 << 25) // part.bytes
+    else:
+        part = lt.Partition.build(g, 32 << 20)
+    w = 2 * g.num_node
     if "zc_div" in kw:  # zero-copy threshold S / zc_div
         kw["zerocopy_threshold"] = part.bytes // kw.pop("zc_div")
-    eng = lt.Engine(g, part, num_device_partitions=128, num_device_walkers=1 << 26, **kw)
+    eng = lt.Engine(g, part, num_device_partitions=slots, num_device_walkers=1 << 26, **kw)
     counts, ends = eng.ppr(w, source=0, prob=0.15, endpoints=True)
     st = eng.stats()
     eng.close()
     log(f"  {st['running_ms']:.1f} ms, loads {st['graph_loading_calls']}, zero-copy {st['zerocopy_calls']}")
-    assert part.count > 128 and st["graph_loading_calls"] > 0
+    assert part.count > st["cache_slots"] and st["graph_loading_calls"] > 0
     assert int(counts.sum()) == w  # one endpoint per walker (app/ppr.cu:82)
     rc, re, _ = oracle.ppr(g.rowptr(), g.col(), w, source=0, prob=0.15, endpoints=True, threads=THREADS)
     assert np.array_equal(ends, re)

@@ -121,13 +121,31 @@ CONFIGS = [
                                             "walkers_per_thread": 4}),
     ("wpt2-device-sched-zc", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "zerocopy_threshold": 2000,
                                         "span": "resident", "walkers_per_thread": 2}),
+    # partitions sized for block slots (lt_partition_build_blocks)
+    ("balanced-blocks-capped", ("blocks", 12 << 10), {"num_device_partitions": 3, "pack_partitions": 2}),
+    ("balanced-raw-slots", ("blocks", 12 << 10), {"num_device_partitions": 3, "zerocopy_threshold": 1500}),
+    ("balanced-prefetch-rule", ("blocks", 12 << 10), {"num_device_partitions": 3, "span": "resident",
+                                                      "policy": "learned", "prefetch_plain": 1, "prefetch_depth": 2,
+                                                      "zerocopy_threshold": 3000, "pack_partitions": 2,
+                                                      "zc_batch": 16, "resident_batch": 16, "walkers_per_thread": 1,
+                                                      "rowptr_resident": 1}),
+    ("balanced-device-sched", ("blocks", 12 << 10), {"num_device_partitions": 3, "device_schedule": 1,
+                                                     "pack_partitions": 2}),
 ]
+
+
+def table(g, pbytes):
+    """The partition table of a case: the reference's split of pbytes bytes,
+    or ("blocks", n): partitions sized for n-byte block slots."""
+    if isinstance(pbytes, tuple):
+        return lt.Partition.build_blocks(g, pbytes[1])
+    return lt.Partition.build(g, pbytes)
 
 
 @pytest.mark.parametrize("label,pbytes,kw", CONFIGS, ids=[c[0] for c in CONFIGS])
 def test_pagerank_bit_exact(g12, label, pbytes, kw):
     log(f"pagerank {label}")
-    part = lt.Partition.build(g12, pbytes)
+    part = table(g12, pbytes)
     eng = lt.Engine(g12, part, **kw)
     w, l = 2 * g12.num_node, 10
     counts, ends = eng.pagerank(w, l, endpoints=True)
@@ -142,7 +160,7 @@ def test_pagerank_bit_exact(g12, label, pbytes, kw):
 @pytest.mark.parametrize("label,pbytes,kw", CONFIGS, ids=[c[0] for c in CONFIGS])
 def test_ppr_bit_exact(g12, label, pbytes, kw):
     log(f"ppr {label}")
-    part = lt.Partition.build(g12, pbytes)
+    part = table(g12, pbytes)
     eng = lt.Engine(g12, part, **kw)
     w = 3 * g12.num_node
     for src in (0, g12.num_node - 1):
@@ -157,7 +175,7 @@ def test_ppr_bit_exact(g12, label, pbytes, kw):
 @pytest.mark.parametrize("label,pbytes,kw", CONFIGS, ids=[c[0] for c in CONFIGS])
 def test_genericwalk_bit_exact(g12, label, pbytes, kw):
     log(f"genericwalk {label}")
-    part = lt.Partition.build(g12, pbytes)
+    part = table(g12, pbytes)
     eng = lt.Engine(g12, part, **kw)
     w, l = 2 * g12.num_node, 10
     ends, moves = eng.genericwalk(w, l, runs=2, moves=True)
@@ -173,7 +191,7 @@ def test_genericwalk_bit_exact(g12, label, pbytes, kw):
 def test_genericwalk_paths_bit_exact(g12, label, pbytes, kw):
     """Whole paths (layout of src/walk/NodeSampler.h:36-49) against the oracle."""
     log(f"genericwalk paths {label}")
-    part = lt.Partition.build(g12, pbytes)
+    part = table(g12, pbytes)
     eng = lt.Engine(g12, part, **kw)
     w, l = 3 * g12.num_node, 9
     paths = eng.genericwalk_paths(w, l, runs=2)
@@ -342,7 +360,7 @@ def test_schedule_independence(g12):
     results = []
     for pbytes, kw in [(64 << 10, {}), (8 << 10, {"num_device_partitions": 2}),
                        (8 << 10, {"num_device_partitions": 7, "policy": "fifo", "span": "resident"})]:
-        part = lt.Partition.build(g12, pbytes)
+        part = table(g12, pbytes)
         eng = lt.Engine(g12, part, **kw)
         results.append(eng.pagerank(w, 12, runs=2))
     for r in results[1:]:
@@ -431,7 +449,8 @@ def test_c5_full_size():
     scheduler's run matches the oracle's full u32 visit counts and every
     endpoint; the learned scheduler's run, with raw copies, gap-coded copies
     and blocks kept packed in HBM, gives the same visit counts (and
-    endpoints)."""
+    endpoints), and so do the plain and learned runs on the graph split for
+    32 MiB block slots."""
     log("C5 full size")
     g = lt.Graph.rmat(26, 27)
     part = lt.Partition.build(g, 32 << 20)
@@ -460,17 +479,36 @@ def test_c5_full_size():
     st = packed.stats()
     assert st["packed_image_bytes"] > 0 and st["packed_col_bytes"] > 1.5 * st["packed_image_bytes"]
     packed.close()
-    # the bench's headline configuration: blocks kept packed in the slots,
-    # largest-queue prefetch targets at depth 3, the network's victims,
-    # threshold S/192, resident and zero-copy batches, 1 walker per lane
-    blocks = lt.Engine(g, part, num_device_partitions=256, span="resident", policy="learned", pack_partitions=2,
-                       prefetch_plain=2, prefetch_depth=3, zerocopy_threshold=part.bytes // 192,
-                       walkers_per_thread=1)
+    # blocks kept packed in the slots of the 32 MB split
+    blocks = lt.Engine(g, part, num_device_partitions=256, span="resident", policy="learned", pack_partitions=2)
     blocks.set_scorer(sc)
+    assert np.array_equal(blocks.pagerank(w, l), counts)
+    assert blocks.stats()["cache_slots"] > 256
+    blocks.close()
+    g.drop_packed()
+    # the bench's headline configuration: the graph split for 32 MiB block
+    # slots (the same 8 GB), blocks kept packed in the slots, largest-queue
+    # prefetch targets at depth 2, the network's victims, threshold slot/128,
+    # resident and zero-copy batches, 1 walker per lane
+    pblk = lt.Partition.build_blocks(g, 32 << 20)
+    slots = (8 << 30) // pblk.bytes
+    bplain = lt.Engine(g, pblk, num_device_partitions=slots, span="resident", pack_partitions=2, record_trace=1,
+                       zerocopy_threshold=(32 << 20) // 192)
+    assert np.array_equal(bplain.pagerank(w, l), counts)
+    sb = lt.Scorer(pblk, seed=1)
+    sb.add_trace(bplain, True, 2)
+    sb.fit(epochs=3)
+    bplain.close()
+    blocks = lt.Engine(g, pblk, num_device_partitions=slots, span="resident", policy="learned", pack_partitions=2,
+                       prefetch_plain=2, prefetch_depth=2, zerocopy_threshold=(32 << 20) // 128,
+                       walkers_per_thread=1)
+    blocks.set_scorer(sb)
     bc, be = blocks.pagerank(w, l, endpoints=True)
     assert np.array_equal(bc, counts) and np.array_equal(be, ends)
     st = blocks.stats()
-    assert st["cache_slots"] > 256 and st["packed_image_bytes"] > 0
+    log(f"  balanced split: P={pblk.count}, {st['cache_slots']} slots, {st['running_ms']:.1f} ms")
+    assert st["cache_slots"] > 0.8 * pblk.count and st["packed_image_bytes"] > 0
+    blocks.close()
 
 
 def test_spill_moves_walkers_through_host_memory(g12):

@@ -321,3 +321,64 @@ def test_slot_layout_rejects_foreign_partition_table():
         with pytest.raises(lt.LTError):
             part.slot_layout(other, 0)
     assert part.slot_layout(big, 0)[1] == part.count
+
+
+def block_slot_need(rowptr, col, n0, n1):
+    """Bytes a block slot needs for vertices [n0, n1), from the header's
+    format text: the rowptr slice ((n+1) * 8) and the block image (16-edge
+    blocks from the range's first edge, width bitlen(max - min) each, 8-byte
+    header pairs plus one, the streams' words plus two), or the raw columns
+    when the image is not smaller."""
+    eb, ee = int(rowptr[n0]), int(rowptr[n1])
+    m = ee - eb
+    img = 8 + 4 * 2
+    if m:
+        c = col[eb:ee].astype(np.int64)
+        starts = np.arange(0, m, 16)
+        span = np.maximum.reduceat(c, starts) - np.minimum.reduceat(c, starts)
+        width = np.where(span > 0, np.floor(np.log2(np.maximum(span, 1))).astype(np.int64) + 1, 0)
+        bits = 16 * int(width.sum())
+        img = 8 * (len(starts) + 1) + 4 * ((bits + 31) // 32 + 2)
+    return 8 * (n1 - n0 + 1) + min(img, 4 * m)
+
+
+def test_block_slot_partitions_hand_computed():
+    # one row 0..63 and 63 empty rows: vertex 0 alone needs 16 + 80 bytes, all
+    # 64 vertices 65 * 8 + 80 = 600; with 599 the first partition stops at 63
+    # vertices (64 * 8 + 80 = 592) and vertex 63 (no edges: 16 bytes) is the next
+    V = 64
+    rowptr = np.array([0, 64] + [64] * (V - 1), np.uint64)
+    g = lt.Graph.from_csr(rowptr, np.arange(64, dtype=np.uint32))
+    one = lt.Partition.build_blocks(g, 600)
+    assert one.count == 1 and one.slot_layout(g, 2) == (768, 1)
+    two = lt.Partition.build_blocks(g, 599)
+    assert list(two.node_offsets()) == [0, 63, 64] and list(two.edge_offsets()) == [0, 64, 64]
+    assert two.bytes == 64 * 8 + 64 * 4  # the larger partition's raw size
+    with pytest.raises(lt.LTError) as e:
+        lt.Partition.build_blocks(g, 95)
+    assert e.value.status == 3  # LT_E_OVERSIZE_VERTEX: vertex 0 needs 96 bytes
+
+
+@pytest.mark.parametrize("scale,ef,slot", [(12, 16, 16 << 10), (14, 8, 48 << 10), (11, 16, 5000)])
+def test_partitions_sized_for_block_slots(scale, ef, slot):
+    """Every partition's rowptr slice + block image fits the slot, the next
+    vertex would not, the engine's own image sizes agree, and the slot layout
+    of the table is no larger than the slot asked for."""
+    rowptr, col = oracle.rmat_graph(scale, ef, seed=scale)
+    g = lt.Graph.from_csr(rowptr, col)
+    part = lt.Partition.build_blocks(g, slot)
+    no, eo = part.node_offsets().astype(np.int64), part.edge_offsets()
+    assert no[0] == 0 and no[-1] == g.num_node and np.array_equal(eo, rowptr[no])
+    raw = 0
+    for p in range(part.count):
+        need = block_slot_need(rowptr, col, no[p], no[p + 1])
+        assert need <= slot, p
+        m = int(eo[p + 1] - eo[p])
+        b = len(part.pack(g, p, blocks=True))
+        assert need == 8 * int(no[p + 1] - no[p] + 1) + (b if b < 4 * m else 4 * m), p
+        if p + 1 < part.count:
+            assert block_slot_need(rowptr, col, no[p], no[p + 1] + 1) > slot, p
+        raw = max(raw, 8 * int(no[p + 1] - no[p] + 1) + 4 * m)
+    assert part.bytes == raw
+    sb, n = part.slot_layout(g, 2, 0)
+    assert sb <= (slot + 255) // 256 * 256 and n == part.count
