@@ -264,7 +264,7 @@ def main():
     ap.add_argument("--c4-config", default=None, metavar="JSON",
                     help="c4: skip the autotuner and run this configuration (a JSON object with the fields of the "
                          "autotuner's pick: partition_mb, slots, span, device_schedule, slice_chunks, zc_batch, "
-                         "pack_partitions, walkers_per_thread)")
+                         "pack_partitions, walkers_per_thread, split)")
     ap.add_argument("--no-variants", action="store_true",
                     help="c2, c3: measure the value's configuration only, without the other configurations "
                          "the line reports beside it (for profiling runs)")
@@ -502,7 +502,7 @@ def main():
             best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=wl["slots"] * (wl["mb"] << 20), log=log)
         tune_s = time.perf_counter() - t1
         log(f"autotuner picked {best} in {tune_s:.1f} s")
-        part = lt.Partition.build(g, best["partition_mb"] << 20)
+        part = (lt.Partition.build_blocks if best.get("split") else lt.Partition.build)(g, best["partition_mb"] << 20)
         cfg = dict(num_device_partitions=best["slots"], span=best["span"], device_schedule=best["device_schedule"],
                    slice_chunks=best["slice_chunks"], zc_batch=best["zc_batch"], resident_batch=best["zc_batch"],
                    pack_partitions=best["pack_partitions"], walkers_per_thread=best["walkers_per_thread"])
@@ -514,7 +514,8 @@ def main():
         desc = (f"c4: R-MAT scale {wl['scale']} ef {wl['ef']} (V={g.num_node}, E={g.num_edge}), batched ppr "
                 f"{wl['nq']} queries (sources ~ degree, counts log-uniform [1e3,1e6), {int(cnt.sum())} walkers), "
                 f"p=0.15, 8 GB cache; {'autotuned' if trials else 'fixed configuration'}: "
-                f"{best['partition_mb']} MB partitions (P={part.count}), "
+                f"{best['partition_mb']} MB " + ("block slots, the graph split for them" if best.get("split")
+                                                   else "partitions") + f" (P={part.count}), "
                 f"{eng.stats()['cache_slots']} cache slots, span {best['span']}, "
                 f"{'device' if best['device_schedule'] else 'host'} scheduling, slice {best['slice_chunks']}, "
                 f"zero-copy and resident batches of {best['zc_batch']}, "
@@ -532,7 +533,7 @@ def main():
                                  "best": best,
                                  "trials": [{k: t[k] for k in ("stage", "partition_mb", "span", "device_schedule",
                                                                "slice_chunks", "zc_batch", "pack_partitions",
-                                                               "walkers_per_thread", "cache_slots", "running_ms",
+                                                               "walkers_per_thread", "split", "cache_slots", "running_ms",
                                                                "h2d_bytes")}
                                             for t in trials]}})
     else:

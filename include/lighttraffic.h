@@ -334,8 +334,10 @@ lt_status lt_make_ppr_queries(const lt_graph *g, uint32_t nq, uint64_t lo, uint6
  * host or device scheduling, 2048 or 8192 chunks per walk launch, zero-copy
  * and resident batches of 1 or 16 (zc_batch = resident_batch; host
  * scheduling, span resident), raw / gap-coded /
- * block copies (gap-coded needs host scheduling).  Then it times the three
- * fastest and the fastest of each copy mode again on a sample four times
+ * block copies (gap-coded needs host scheduling), the block copies also on
+ * the split sized for block slots of S (lt_partition_build_blocks).  Then it
+ * times the three fastest and the fastest of each copy mode (block copies on
+ * either split) again on a sample four times
  * larger, then the fastest of those with each fixed number of walkers per
  * walk-kernel lane (1, 2, 4) on that sample, and returns the fastest of
  * stages 2 and 3 in *best_out.  Every trial is written to trials_out (up to
@@ -355,6 +357,9 @@ typedef struct {
     int32_t  zc_batch;
     int32_t  pack_partitions;
     int32_t  walkers_per_thread;
+    int32_t  split;            /* 0: lt_partition_build(S); 1 (block copies only):
+                                  lt_partition_build_blocks(S), slots =
+                                  cache_bytes / its partition_bytes             */
     double   running_ms;       /* the warm job's scheduler-loop time               */
     uint64_t h2d_bytes;        /* its explicit + zero-copied H2D bytes             */
     uint64_t iterations;       /* its scheduler iterations                         */

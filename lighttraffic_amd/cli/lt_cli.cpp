@@ -120,14 +120,17 @@ int main(int argc, char **argv) {
         s = lt_tune_ppr_batch(g, (uint32_t)qsrc.size(), qsrc.data(), qcnt.data(), c.prob, cache, nullptr, 0, 16,
                               &best, nullptr, 0, nullptr);
         if (s != LT_OK) return fail(s, "autotune");
-        printf("autotuned: %u MB partitions, %u cache slots, span %s, %s scheduling, slice %d, batches of %d, "
+        printf("autotuned: %u MB %s, %u cache slots, span %s, %s scheduling, slice %d, batches of %d, "
                "%s copies, walkers per lane %d\n",
-               best.partition_mb, best.slots, best.span == LT_SPAN_RESIDENT ? "resident" : "partition",
+               best.partition_mb, best.split ? "block slots (split for them)" : "partitions", best.slots,
+               best.span == LT_SPAN_RESIDENT ? "resident" : "partition",
                best.device_schedule ? "device" : "host", best.slice_chunks, best.zc_batch,
                best.pack_partitions == 2 ? "block" : best.pack_partitions == 1 ? "gap-coded" : "raw",
                best.walkers_per_thread);
         lt_partition_free(part);
-        if ((s = lt_partition_build(g, (uint64_t)best.partition_mb << 20, &part)) != LT_OK) return fail(s, "partition");
+        s = best.split ? lt_partition_build_blocks(g, (uint64_t)best.partition_mb << 20, &part)
+                       : lt_partition_build(g, (uint64_t)best.partition_mb << 20, &part);
+        if (s != LT_OK) return fail(s, "partition");
         cfg.num_device_partitions = best.slots;
         cfg.span = best.span;
         cfg.device_schedule = best.device_schedule;

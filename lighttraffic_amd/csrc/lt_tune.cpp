@@ -9,7 +9,10 @@
 // sample four times larger, since a small sample walks few steps per
 // partition load and so over-weights the copies; then the fastest of that
 // second stage with each fixed number of walkers per walk-kernel lane.  The
-// pick is the fastest of stages 2 and 3.
+// pick is the fastest of stages 2 and 3.  Each partition size is tried with
+// the reference's split (raw bytes per partition) and, for block copies,
+// with the split sized for block slots of that size
+// (lt_partition_build_blocks), whose images fill their slots.
 #include <stdio.h>
 #include <string.h>
 
@@ -32,6 +35,13 @@ Sample take_sample(uint32_t nq, const uint32_t *sources, const uint64_t *counts,
         s.cnt.push_back(counts[q]);
     }
     return s;
+}
+
+// The partition table of trial t: the reference's split of S = partition_mb,
+// or (split 1) the split for block slots of that size.
+lt_status build_table(const lt_graph *g, const lt_tune_trial &t, lt_partition **p) {
+    return t.split ? lt_partition_build_blocks(g, (uint64_t)t.partition_mb << 20, p)
+                   : lt_partition_build(g, (uint64_t)t.partition_mb << 20, p);
 }
 
 // Running time of a warm second job of `t`'s configuration on sample `s`.
@@ -85,12 +95,18 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
     const Sample s1 = take_sample(nq, sources, counts, every);
     std::vector<lt_tune_trial> trials;
     lt_graph *gw = const_cast<lt_graph *>(g);  // only its cache of packed images changes
-    for (uint32_t mb : kSizesMb) {
+    for (uint32_t mb : kSizesMb)
+    for (int32_t split : {0, 1}) {
+        lt_tune_trial proto{};
+        proto.partition_mb = mb;
+        proto.split = split;
         lt_partition *p = nullptr;
-        lt_status st = lt_partition_build(g, (uint64_t)mb << 20, &p);
+        lt_status st = build_table(g, proto, &p);
         if (st != LT_OK) return st;
         const uint32_t P = lt_partition_count(p);
-        const uint32_t slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(P, cache_bytes / ((uint64_t)mb << 20)));
+        // the same cache bytes: raw slots of the table's partition size
+        const uint32_t slots =
+            (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(P, cache_bytes / lt_partition_bytes(p)));
         for (int32_t span : kSpans)
             for (int32_t dev : {0, 1})
                 for (int32_t sl : kSlices)
@@ -100,9 +116,9 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
                             if (zb != kZcBatches[0] && (dev || span != LT_SPAN_RESIDENT))
                                 continue;  // zero-copy batches need host scheduling and the resident span
                             if (dev && pk == 1) continue;  // the device loop has no unpack step
-                            lt_tune_trial t{};
+                            if (split && pk != 2) continue;  // the split serves block slots
+                            lt_tune_trial t = proto;
                             t.stage = 1;
-                            t.partition_mb = mb;
                             t.slots = slots;
                             t.partitions = P;
                             t.span = span;
@@ -119,15 +135,17 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
         lt_graph_drop_packed(gw);  // the next partition size needs images of its own
         lt_partition_free(p);
     }
-    // stage 2: the three fastest and the fastest of each copy mode, on a sample four times larger
+    // stage 2: the three fastest and the fastest of each copy mode (block
+    // copies on either split), on a sample four times larger
     std::vector<lt_tune_trial> top(trials);
     std::sort(top.begin(), top.end(), [](const lt_tune_trial &a, const lt_tune_trial &b) {
         return a.running_ms < b.running_ms;
     });
     std::vector<lt_tune_trial> pick(top.begin(), top.begin() + std::min<size_t>(3, top.size()));
-    for (int32_t pk : kPacks) {
+    for (int32_t mode = 0; mode < 4; mode++) {  // raw, gap-coded, block; block on the split for its slots
+        const int32_t pk = mode < 3 ? kPacks[mode] : 2, split = mode == 3;
         for (const lt_tune_trial &t : top) {  // sorted, so the first of a mode is its fastest
-            if (t.pack_partitions != pk) continue;
+            if (t.pack_partitions != pk || t.split != split) continue;
             const bool have = std::any_of(pick.begin(), pick.end(), [&](const lt_tune_trial &x) {
                 return memcmp(&x, &t, sizeof(t)) == 0;
             });
@@ -138,14 +156,15 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
     std::vector<lt_tune_trial> last;
     const Sample s2 = take_sample(nq, sources, counts, std::max(1u, every / 4));
     if (every > 1) {
-        for (uint32_t mb : kSizesMb) {
-            if (!std::any_of(pick.begin(), pick.end(), [&](const lt_tune_trial &t) { return t.partition_mb == mb; }))
-                continue;
+        for (uint32_t mb : kSizesMb)
+        for (int32_t split : {0, 1}) {
+            auto same = [&](const lt_tune_trial &t) { return t.partition_mb == mb && t.split == split; };
+            if (!std::any_of(pick.begin(), pick.end(), same)) continue;
             lt_partition *p = nullptr;
-            lt_status st = lt_partition_build(g, (uint64_t)mb << 20, &p);
+            lt_status st = build_table(g, pick[std::find_if(pick.begin(), pick.end(), same) - pick.begin()], &p);
             if (st != LT_OK) return st;
             for (lt_tune_trial t : pick) {
-                if (t.partition_mb != mb) continue;
+                if (!same(t)) continue;
                 t.stage = 2;
                 if ((st = time_trial(g, p, s2, prob, t)) != LT_OK) {
                     lt_partition_free(p);
@@ -167,7 +186,7 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
     // default is 1, so that variant is the pick itself there)
     {
         lt_partition *p = nullptr;
-        lt_status st = lt_partition_build(g, (uint64_t)best.partition_mb << 20, &p);
+        lt_status st = build_table(g, best, &p);
         if (st != LT_OK) return st;
         const lt_tune_trial pick = best;
         for (int32_t wpt : {1, 2, 4}) {

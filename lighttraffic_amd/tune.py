@@ -9,7 +9,9 @@ launch consumes, (span resident, host scheduling) how many queues one launch
 walks (sub-threshold partitions over PCIe, or cached partitions), and whether
 partitions are copied raw,
 gap-coded and unpacked in HBM (host scheduling), or kept packed in HBM as
-16-edge blocks.  A small sample walks few steps per partition load, so it
+16-edge blocks, the latter also on the split sized for block slots of the
+partition size (lt_partition_build_blocks, `split` 1).  A small sample walks
+few steps per partition load, so it
 over-weights the copies: the three fastest configurations, and the fastest of
 each copy mode, are timed again on a sample four times larger; the fastest of
 those is timed again with 1, 2 and 4 walkers per walk-kernel lane (instead of
@@ -31,7 +33,7 @@ class TuneTrial(ctypes.Structure):
                 ("partitions", ctypes.c_uint32), ("cache_slots", ctypes.c_uint32), ("span", ctypes.c_int32),
                 ("device_schedule", ctypes.c_int32), ("slice_chunks", ctypes.c_int32), ("zc_batch", ctypes.c_int32),
                 ("pack_partitions", ctypes.c_int32), ("walkers_per_thread", ctypes.c_int32),
-                ("running_ms", ctypes.c_double), ("h2d_bytes", ctypes.c_uint64),
+                ("split", ctypes.c_int32), ("running_ms", ctypes.c_double), ("h2d_bytes", ctypes.c_uint64),
                 ("iterations", ctypes.c_uint64)]
 
     def as_dict(self):

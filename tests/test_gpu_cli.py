@@ -90,7 +90,8 @@ def test_ppr_queries_autotuned(graph_file, tmp_path):
     qfile.write_text("".join(f"{s} {c}\n" for s, c in zip(src, cnt)))
     run("pagerank", "-f", graph_file, "-m", "-s", 1)
     out = run("ppr", "-f", graph_file, "-p", 4, "--queries", qfile, "--tune")
-    m = re.search(r"autotuned: (\d+) MB partitions, (\d+) cache slots, span (\w+), (\w+) scheduling", out)
+    m = re.search(r"autotuned: (\d+) MB (?:partitions|block slots \(split for them\)), (\d+) cache slots, "
+                  r"span (\w+), (\w+) scheduling", out)
     assert m, out
     assert int(m.group(1)) in (16, 32, 64) and m.group(3) in ("partition", "resident")
     assert "result correct" in out

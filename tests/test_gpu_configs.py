@@ -101,14 +101,20 @@ def test_c3_ppr_capped_cache(s25, label, kw):
     ("64 MB, span resident, zc batches, block slots", 64, dict(span="resident", zc_batch=16, pack_partitions=2)),
     ("32 MB, device-scheduled", 32, dict(device_schedule=1, span="resident")),
     ("64 MB, device-scheduled, block slots", 64, dict(device_schedule=1, span="resident", pack_partitions=2)),
+    # the autotuner's pick: 64 MiB block slots on the split for them, every
+    # partition cached
+    ("64 MiB block slots, split for them", 64, dict(span="resident", zc_batch=16, resident_batch=16,
+                                                    pack_partitions=2, slice_chunks=8192, walkers_per_thread=1,
+                                                    split=1)),
 ])
 def test_c4_batched_ppr(s25, label, mb, kw):
     log(f"C4: 4,096 batched ppr queries, 8 GB cache, {label}")
     g = s25
     src, cnt = g.ppr_queries(4096)
     assert len(src) == 4096 and 1000 <= int(cnt.min()) and int(cnt.max()) < 1000000
-    part = lt.Partition.build(g, mb << 20)
-    eng = lt.Engine(g, part, num_device_partitions=(8 << 30) // (mb << 20), **kw)
+    kw = dict(kw)
+    part = (lt.Partition.build_blocks if kw.pop("split", 0) else lt.Partition.build)(g, mb << 20)
+    eng = lt.Engine(g, part, num_device_partitions=(8 << 30) // part.bytes, **kw)
     ends = eng.ppr_batch(src, cnt, prob=0.15, endpoints=True)
     st = eng.stats()
     eng.close()

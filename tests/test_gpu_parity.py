@@ -279,16 +279,19 @@ def test_autotuner_picks_a_trial():
     best, trials = tune_ppr_batch(g, src, cnt, cache_bytes=48 << 20, sample_every=4, sizes_mb=(16,))
     # host scheduling: (span partition x 2 slices + span resident x 2 slices x 2 batch
     # sizes) x raw, gap-coded or block copies; device scheduling: 2 spans (whole
-    # queues) x raw or block copies
+    # queues) x raw or block copies; then block copies on the split for block
+    # slots: 6 host and 2 device configurations
     first = [t for t in trials if t["stage"] == 1]
     refined = [t for t in trials if t["stage"] == 2]
-    assert len(first) == 3 * (2 + 4) + 2 * 2
+    assert len(first) == 3 * (2 + 4) + 2 * 2 + (6 + 2)
+    assert {t["pack_partitions"] for t in first if t["split"]} == {2}
+    assert sum(t["split"] for t in first) == 8
     assert {(t["span"], t["pack_partitions"]) for t in first if t["device_schedule"]} == {
         ("partition", 0), ("partition", 2), ("resident", 0), ("resident", 2)}
-    keys = ("partition_mb", "span", "device_schedule", "slice_chunks", "zc_batch", "pack_partitions")
+    keys = ("partition_mb", "span", "device_schedule", "slice_chunks", "zc_batch", "pack_partitions", "split")
     want = {tuple(t[k] for k in keys) for t in sorted(first, key=lambda t: t["running_ms"])[:3]}
-    for pk in (0, 1, 2):
-        want.add(tuple(min((t for t in first if t["pack_partitions"] == pk),
+    for pk, split in ((0, 0), (1, 0), (2, 0), (2, 1)):
+        want.add(tuple(min((t for t in first if t["pack_partitions"] == pk and t["split"] == split),
                            key=lambda t: t["running_ms"])[k] for k in keys))
     assert sorted(tuple(t[k] for k in keys) for t in refined) == sorted(want)
     assert {t["pack_partitions"] for t in refined} == {0, 1, 2}
