@@ -557,6 +557,14 @@ def main():
             v = measure(e2, wl, w, 1, 2, "c2/span partition")
             out["c2_variants"] = {"span_partition": {k: v[k] for k in ("steps_per_s", "ms_per_step")}}
             e2.close()
+            # the reference's figure-12 ablations of the append stage, with its span (every move
+            # out of the partition is appended to a queue): each walker stored from its own lane,
+            # and one queue reservation per walker
+            for mode, tag in ((1, "span_partition_no_random_access_reduce"), (2, "span_partition_one_level_queue")):
+                e2 = lt.Engine(g, part, span="partition", append_mode=mode, **base)
+                v = measure(e2, wl, w, 1, 3, f"c2/{tag}")
+                out["c2_variants"][tag] = {k: v[k] for k in ("steps_per_s", "ms_per_step")}
+                e2.close()
         if args.workload == "c3" and not args.no_variants:
             keep = ("steps_per_s", "ms_per_step", "median_ms_per_step", "e2e_ms_per_step", "h2d_bytes_per_step",
                     "h2d_GB_per_job", "pcie_GBps")

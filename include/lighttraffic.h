@@ -129,6 +129,18 @@ typedef struct {
                                        whose queue is at least this % of the
                                        greedy choice's (0 = off, the default;
                                        1-100)                                  */
+    int32_t  append_mode;           /* how the walk appends leaving walkers to
+                                       their queues: 0 (default) one queue
+                                       reservation per (workgroup, queue),
+                                       walkers staged in LDS for coalesced
+                                       stores; 1 the same reservations, each
+                                       walker stored from its own lane (the
+                                       reference's NO_RANDOM_ACCESS_REDUCE
+                                       build, src/WalkManager.h:98-118); 2 one
+                                       reservation per walker (its
+                                       ONE_LEVEL_QUEUE build,
+                                       src/WalkManager.h:195-209).  Figure 12's
+                                       ablations; results do not change      */
 } lt_config;
 
 /* Statistics of the last run call.  The first eight fields mirror the lines

@@ -53,7 +53,7 @@ class Config(ctypes.Structure):
                 ("device_schedule", _i32), ("slice_chunks", _i32), ("zc_batch", _i32),
                 ("rowptr_resident", _i32), ("pack_partitions", _i32), ("prefetch_plain", _i32),
                 ("evict_shortlist", _i32), ("walkers_per_thread", _i32), ("resident_batch", _i32),
-                ("learned_next", _i32)]
+                ("learned_next", _i32), ("append_mode", _i32)]
 
 
 class Stats(ctypes.Structure):

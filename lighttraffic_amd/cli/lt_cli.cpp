@@ -110,6 +110,7 @@ int main(int argc, char **argv) {
     cfg.prefetch_plain = c.prefetch_plain;
     cfg.evict_shortlist = c.evict_shortlist;
     cfg.learned_next = c.learned_next;
+    cfg.append_mode = c.append_mode;
     cfg.device_schedule = c.device_schedule ? 1 : 0;
     cfg.pack_partitions = c.pack;
     cfg.rowptr_resident = c.rowptr_resident ? 1 : 0;

@@ -20,7 +20,9 @@
 //   --queries <file>, --tune (with --queries: autotune the configuration),
 //   --resident-batch <n>, --walkers-per-thread <n>, --slice-chunks <n>,
 //   --prefetch-plain <0|1|2>, --evict-shortlist <n>, --learned-next <pct>
-//   (the lt_config fields of the same names; 0 = the library's default)
+//   (the lt_config fields of the same names; 0 = the library's default),
+//   --no-random-access-reduce, --one-level-queue (append_mode 1 and 2: the
+//   reference's figure-12 builds -DNO_RANDOM_ACCESS_REDUCE, -DONE_LEVEL_QUEUE)
 #pragma once
 #include <stdint.h>
 #include <stdio.h>
@@ -51,6 +53,7 @@ struct CliConfig {
     int prefetch_plain = 0;
     int evict_shortlist = 0;
     int learned_next = 0;
+    int append_mode = 0;
     bool device_schedule = false;
     int pack = 0;                // lt_config.pack_partitions: 1 --pack, 2 --pack-blocks
     bool rowptr_resident = false;
@@ -114,6 +117,8 @@ static bool cli_parse(int argc, char **argv, CliConfig &c) {
         else if (a == "--prefetch-plain") c.prefetch_plain = cli_num<int>(next("prefetch-plain"));
         else if (a == "--evict-shortlist") c.evict_shortlist = cli_num<int>(next("evict-shortlist"));
         else if (a == "--learned-next") c.learned_next = cli_num<int>(next("learned-next"));
+        else if (a == "--no-random-access-reduce") c.append_mode = 1;
+        else if (a == "--one-level-queue") c.append_mode = 2;
         else if (a == "--device-schedule") c.device_schedule = true;
         else if (a == "--pack") c.pack = 1;
         else if (a == "--pack-blocks") c.pack = 2;

@@ -150,6 +150,13 @@ struct WalkArgs {
     const uint32_t *words;   // bit streams of src's packed slot, nullptr for raw columns
     int zerocopy;         // src slice is read over PCIe
     int span_resident;    // keep walking through HBM-resident partitions
+    // How a pass appends its leaving walkers to their queues: 0, one queue
+    // reservation per (workgroup, destination) and the walkers staged in LDS
+    // so the stores are coalesced; 1, the same reservations with each walker
+    // stored from its own lane (the reference's NO_RANDOM_ACCESS_REDUCE
+    // build, src/WalkManager.h:98-118); 2, one reservation per walker (its
+    // ONE_LEVEL_QUEUE build, src/WalkManager.h:195-209).
+    int append_mode;
     unsigned long long *stats;
     // Batch (span_resident only): nsrc > 0 consumes the whole queues of
     // srcs[0..nsrc), each entered in the partition table for the launch

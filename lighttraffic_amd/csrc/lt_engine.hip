@@ -400,6 +400,10 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
         set_error("evict_shortlist must be >= 0 (got %d)", cfg.evict_shortlist);
         return LT_E_ARG;
     }
+    if (cfg.append_mode < 0 || cfg.append_mode > 2) {
+        set_error("append_mode must be 0, 1 or 2 (got %d)", cfg.append_mode);
+        return LT_E_ARG;
+    }
     if (cfg.learned_next < 0 || cfg.learned_next > 100) {
         set_error("learned_next must be 0-100 (got %d)", cfg.learned_next);
         return LT_E_ARG;
@@ -843,6 +847,7 @@ lt_status Engine::process(uint32_t p, bool zerocopy, uint64_t walkers) {
     const uint64_t host = spill_on ? hcount[p] : 0;
     if (walkers == 0 && host == 0) return LT_OK;
     WalkArgs a{};
+    a.append_mode = cfg.append_mode;
     a.app = app;
     a.q = q;
     a.pt.node_off = d_node_off;
@@ -1155,6 +1160,7 @@ lt_status Engine::process_resident(uint32_t c, int kind, uint8_t *prefetched) {
 // is appended to.
 lt_status Engine::walk_batch(const std::vector<uint32_t> &parts, bool zerocopy) {
     WalkArgs a{};
+    a.append_mode = cfg.append_mode;
     a.app = app;
     a.q = q;
     a.pt.node_off = d_node_off;
@@ -1648,6 +1654,7 @@ lt_status Engine::scheduler_loop_device() {
     LT_HIP(hipStreamSynchronize(s_compute));
 
     WalkArgs a{};
+    a.append_mode = cfg.append_mode;
     a.app = app;
     a.q = q;
     a.pt.node_off = d_node_off;

@@ -151,6 +151,42 @@ __device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint
     return total;
 }
 
+// Store walker w at position rank of slot sl's reservation (its first queue
+// slot in L.hist[sl], the chunks it touches in L.ch0/L.ch1).
+__device__ __forceinline__ void store_reserved(const Queues &q, const AppendLds &L, uint32_t sl, uint32_t rank,
+                                               const Walker &w) {
+    const uint32_t b = L.hist[sl];
+    const uint32_t s = b + rank;
+    const uint32_t c = (s >> kChunkLog2) == (b >> kChunkLog2) ? L.ch0[sl] : L.ch1[sl];
+    if (c < q.nchunks) q.slab[(size_t)c * kChunk + (s & (kChunk - 1))] = w;
+    else atomicOr(&q.ctl->overflow, 8u);
+}
+
+// One queue reservation per walker (append_mode 2).  Collective like
+// block_append: every walker first takes its queue slot; the walkers whose
+// slot starts a chunk link it, and after the barrier the others wait
+// (bounded) for links made by reservations of other workgroups.
+template <int WPT>
+__device__ void append_one_level(const Queues &q, const Walker (&w)[WPT], const uint32_t (&dst)[WPT]) {
+    uint32_t b[WPT], c[WPT];
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        c[k] = kEmpty;
+        if (dst[k] == kEmpty) continue;
+        b[k] = atomicAdd(&qlen_of(q, dst[k]), 1u);
+        if ((b[k] & (kChunk - 1)) == 0) c[k] = alloc_chunk(q, dst[k], b[k] >> kChunkLog2);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        if (dst[k] == kEmpty) continue;
+        if ((b[k] & (kChunk - 1)) != 0) c[k] = get_chunk(q, dst[k], b[k] >> kChunkLog2);
+        if (c[k] < q.nchunks) q.slab[(size_t)c[k] * kChunk + (b[k] & (kChunk - 1))] = w[k];
+        else atomicOr(&q.ctl->overflow, 8u);
+    }
+    __syncthreads();
+}
+
 // block_append for partition tables above kDirectMaxP: the pass's distinct
 // destinations are numbered through an open-addressed LDS table of
 // 2 * nslot entries (at most nslot keys, so every probe sequence finds its
@@ -158,7 +194,7 @@ __device__ uint32_t block_exclusive_scan(const uint32_t *in, uint32_t *out, uint
 // block_append then work on those slots instead of on partition ids.
 template <int WPT>
 __device__ void block_append_hashed(const Queues &q, const AppendLds &L, const Walker (&w)[WPT],
-                                    const uint32_t (&dst)[WPT], uint32_t *s_wsum) {
+                                    const uint32_t (&dst)[WPT], uint32_t *s_wsum, int mode) {
     const uint32_t tid = threadIdx.x;
     const uint32_t T = 2 * L.nslot;  // a power of two
     for (uint32_t i = tid; i < T; i += kBlock) L.keys[i] = kEmpty;
@@ -221,6 +257,13 @@ __device__ void block_append_hashed(const Queues &q, const AppendLds &L, const W
         }
     }
     __syncthreads();
+    if (mode == 1) {  // uniform: each walker stored from its own lane
+#pragma unroll
+        for (int k = 0; k < WPT; k++)
+            if (dst[k] != kEmpty) store_reserved(q, L, sl[k], rank[k], w[k]);
+        __syncthreads();
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < WPT; k++) {
         if (dst[k] != kEmpty) {
@@ -251,9 +294,13 @@ __device__ void block_append_hashed(const Queues &q, const AppendLds &L, const W
 // through a hash table (block_append_hashed).
 template <int WPT>
 __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, const Walker (&w)[WPT],
-                             const uint32_t (&dst)[WPT], uint32_t *s_wsum) {
+                             const uint32_t (&dst)[WPT], uint32_t *s_wsum, int mode = 0) {
+    if (mode == 2) {  // uniform
+        append_one_level<WPT>(q, w, dst);
+        return;
+    }
     if (L.hashed) {  // uniform: more partitions than LDS slots
-        block_append_hashed<WPT>(q, L, w, dst, s_wsum);
+        block_append_hashed<WPT>(q, L, w, dst, s_wsum, mode);
         return;
     }
     const uint32_t tid = threadIdx.x;
@@ -291,6 +338,13 @@ __device__ void block_append(const Queues &q, uint32_t P, const AppendLds &L, co
         }
     }
     __syncthreads();
+    if (mode == 1) {  // uniform: each walker stored from its own lane
+#pragma unroll
+        for (int k = 0; k < WPT; k++)
+            if (dst[k] != kEmpty) store_reserved(q, L, dst[k], rank[k], w[k]);
+        __syncthreads();
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < WPT; k++) {
         if (dst[k] != kEmpty) {
@@ -544,7 +598,7 @@ __device__ __forceinline__ void walk_walkers(const WalkArgs &a, const AppendLds 
     for (int k = 0; k < WPT; k++) {
         if ((uint32_t)k * kBlock + threadIdx.x < nvalid) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
     }
-    block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum);
+    block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum, a.append_mode);
     flush_visits(a.app, ct);
 }
 

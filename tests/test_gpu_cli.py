@@ -51,9 +51,10 @@ def test_pagerank_cli_capped_cache(graph_file):
     loads = grep_num(r"graph loading time: [0-9.]+ ms, calls: (\d+)", out)
     assert it >= ex >= 1 and loads >= nparts - 3
     assert re.search(r"Running Time: [0-9.]+ ms", out)
-    # figure 13/14 ablations as runtime flags: FIFO, no pipelining, zero copy off / always
+    # figure 12-14 ablations as runtime flags: FIFO, no pipelining, zero copy off / always,
+    # one queue reservation per walker, walkers stored from their own lanes
     for flags in (["--scheduler", "fifo"], ["--no-pipeline"], ["--zerocopy-threshold", 1],
-                  ["--zerocopy-threshold", 1 << 40]):
+                  ["--zerocopy-threshold", 1 << 40], ["--one-level-queue"], ["--no-random-access-reduce"]):
         out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, *flags)
         assert "result correct" in out
     out = run("pagerank", "-f", graph_file, "-p", 3, "-w", 50000, "--pack", "--zerocopy-threshold", 0)

@@ -121,6 +121,15 @@ CONFIGS = [
                                             "walkers_per_thread": 4}),
     ("wpt2-device-sched-zc", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "zerocopy_threshold": 2000,
                                         "span": "resident", "walkers_per_thread": 2}),
+    # the reference's figure-12 appends: each walker stored from its own lane
+    # (NO_RANDOM_ACCESS_REDUCE), one queue reservation per walker (ONE_LEVEL_QUEUE)
+    ("append-no-staging", 16 << 10, {"num_device_partitions": 3, "append_mode": 1}),
+    ("append-one-level", 16 << 10, {"num_device_partitions": 3, "append_mode": 2}),
+    ("append-one-level-batches", 16 << 10, {"num_device_partitions": 4, "span": "resident", "zc_batch": 16,
+                                            "resident_batch": 16, "zerocopy_threshold": 3000, "pack_partitions": 2,
+                                            "append_mode": 2, "walkers_per_thread": 4}),
+    ("append-no-staging-device", 16 << 10, {"num_device_partitions": 4, "device_schedule": 1, "append_mode": 1,
+                                            "walkers_per_thread": 2}),
     # partitions sized for block slots (lt_partition_build_blocks)
     ("balanced-blocks-capped", ("blocks", 12 << 10), {"num_device_partitions": 3, "pack_partitions": 2}),
     ("balanced-raw-slots", ("blocks", 12 << 10), {"num_device_partitions": 3, "zerocopy_threshold": 1500}),
@@ -425,7 +434,10 @@ def _many_partition_graph():
 @pytest.mark.parametrize("kw", [{"num_device_partitions": 600},
                                 {"num_device_partitions": 600, "span": "resident", "zerocopy_threshold": 40,
                                  "zc_batch": 16},
-                                {"num_device_partitions": 600, "device_schedule": 1}])
+                                {"num_device_partitions": 600, "device_schedule": 1},
+                                # the reference's figure-12 appends on the hashed path
+                                {"num_device_partitions": 600, "append_mode": 1},
+                                {"num_device_partitions": 600, "span": "resident", "append_mode": 2}])
 def test_many_partitions(S, kw):
     """About 2,400 and 12,000 partitions (above 1,024 the append stage numbers
     each pass's destinations through an LDS hash table; the reference's

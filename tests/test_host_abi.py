@@ -236,7 +236,7 @@ def test_engine_rejects_unknown_walkers_per_thread():
 
 
 @pytest.mark.parametrize("field,bad", [("prefetch_plain", -1), ("prefetch_plain", 3), ("evict_shortlist", -2),
-                                       ("learned_next", -1), ("learned_next", 101)])
+                                       ("learned_next", -1), ("learned_next", 101), ("append_mode", 3)])
 def test_engine_rejects_bad_learned_settings(field, bad):
     # checked before any device call, so on every host
     g = lt.Graph.open(os.path.join(GOLDEN, "rmat_s8_ef4.bcsr"))
@@ -249,7 +249,8 @@ def test_engine_rejects_bad_learned_settings(field, bad):
 def test_config_defaults_of_the_new_fields():
     c = lt.Config()
     lt.lib().lt_config_default(lt.ctypes.byref(c))
-    assert (c.prefetch_plain, c.evict_shortlist, c.walkers_per_thread, c.resident_batch, c.learned_next) == (0,) * 5
+    assert (c.prefetch_plain, c.evict_shortlist, c.walkers_per_thread, c.resident_batch, c.learned_next,
+            c.append_mode) == (0,) * 6
     assert (c.prefetch_depth, c.prefetch_stick_pct) == (2, 50)
 
 
