@@ -79,9 +79,9 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
                                        float prob, uint64_t cache_bytes, const uint32_t *sizes_mb, uint32_t nsizes,
                                        uint32_t sample_every, lt_tune_trial *best_out, lt_tune_trial *trials_out,
                                        uint32_t trials_cap, uint32_t *ntrials_out) {
-    if (!g || nq == 0 || !sources || !counts || !best_out || !(prob > 0.0f) || cache_bytes == 0 ||
+    if (!g || nq == 0 || !sources || !counts || !best_out || !(prob > 0.0f && prob <= 1.0f) || cache_bytes == 0 ||
         (sizes_mb && nsizes == 0)) {
-        lt::set_error("lt_tune_ppr_batch: null argument, no query, no cache, no partition size or prob <= 0");
+        lt::set_error("lt_tune_ppr_batch: null argument, no query, no cache, no partition size or prob not in (0, 1]");
         return LT_E_ARG;
     }
     static const uint32_t kDefaultSizesMb[] = {16, 32, 64};
@@ -161,7 +161,7 @@ extern "C" lt_status lt_tune_ppr_batch(const lt_graph *g, uint32_t nq, const uin
             auto same = [&](const lt_tune_trial &t) { return t.partition_mb == mb && t.split == split; };
             if (!std::any_of(pick.begin(), pick.end(), same)) continue;
             lt_partition *p = nullptr;
-            lt_status st = build_table(g, pick[std::find_if(pick.begin(), pick.end(), same) - pick.begin()], &p);
+            lt_status st = build_table(g, *std::find_if(pick.begin(), pick.end(), same), &p);
             if (st != LT_OK) return st;
             for (lt_tune_trial t : pick) {
                 if (!same(t)) continue;
