@@ -37,7 +37,7 @@ int main(int argc, char **argv) {
                 "          [--runs n] [--source v] [--prob p] [--scheduler greedy|fifo|learned]\n"
                 "          [--no-pipeline] [--zerocopy-threshold n] [--span partition|resident]\n"
                 "          [--seed n] [--timers] [--json] [--queries file [--tune]] [--pack | --pack-blocks]\n"
-                "          [--rowptr-resident]\n",
+                "          [--rowptr-resident] [--split-blocks] [--cache-mb MB]\n",
                 app.c_str());
         return 1;
     }
@@ -46,8 +46,10 @@ int main(int argc, char **argv) {
     if (s != LT_OK) return fail(s, "open graph");
     lt_partition *part = nullptr;
     if (c.materialize) {
-        // Config.h:62-66 + EqualSizeStrategy::init's report (PartitionStrategy.h:225-234)
-        s = lt_partition_build(g, 1024ull * 1024ull * c.partition_mb, &part);
+        // Config.h:62-66 + EqualSizeStrategy::init's report (PartitionStrategy.h:225-234);
+        // --split-blocks: partitions sized for block slots of -s MB instead
+        s = c.split_blocks ? lt_partition_build_blocks(g, 1024ull * 1024ull * c.partition_mb, &part)
+                           : lt_partition_build(g, 1024ull * 1024ull * c.partition_mb, &part);
         if (s != LT_OK) return fail(s, "partition");
         s = lt_partition_save(part, c.file.c_str());
         if (s != LT_OK) return fail(s, "write .part");
@@ -67,7 +69,9 @@ int main(int argc, char **argv) {
     const uint32_t V = lt_graph_num_node(g);
     const uint32_t P = lt_partition_count(part);
     const uint64_t w = c.has_w ? c.w : 2ull * V;
-    const uint64_t pdev = c.has_p ? std::min<uint64_t>(c.p, P) : P;
+    // --cache-mb: the raw slots of this table's partition size that fit that many MB
+    const uint64_t pcache = c.cache_mb ? std::max<uint64_t>(1, (c.cache_mb << 20) / lt_partition_bytes(part)) : P;
+    const uint64_t pdev = c.has_p ? std::min<uint64_t>(c.p, P) : std::min<uint64_t>(pcache, P);
     const uint64_t wdev = c.has_b ? std::min<uint64_t>(c.b, w) : w;
 
     // batched ppr queries: one "source count" pair per line

@@ -23,6 +23,10 @@
 //   (the lt_config fields of the same names; 0 = the library's default),
 //   --no-random-access-reduce, --one-level-queue (append_mode 1 and 2: the
 //   reference's figure-12 builds -DNO_RANDOM_ACCESS_REDUCE, -DONE_LEVEL_QUEUE)
+//   --split-blocks (with -m: lt_partition_build_blocks, partitions sized for
+//   block slots of -s MB), --cache-mb <MB> (without -p: as many cache slots
+//   of the table's partition size as fit, the way to size the cache of a
+//   --split-blocks table, whose partition size is its largest raw partition)
 #pragma once
 #include <stdint.h>
 #include <stdio.h>
@@ -57,6 +61,8 @@ struct CliConfig {
     bool device_schedule = false;
     int pack = 0;                // lt_config.pack_partitions: 1 --pack, 2 --pack-blocks
     bool rowptr_resident = false;
+    bool split_blocks = false;   // -m: lt_partition_build_blocks instead of lt_partition_build
+    uint64_t cache_mb = 0;       // without -p: cache slots = cache_mb MB / partition size
     bool tune = false;           // ppr --queries: pick the configuration with lt_tune_ppr_batch
     bool pipeline = true;
     bool has_zc = false;
@@ -123,6 +129,8 @@ static bool cli_parse(int argc, char **argv, CliConfig &c) {
         else if (a == "--pack") c.pack = 1;
         else if (a == "--pack-blocks") c.pack = 2;
         else if (a == "--rowptr-resident") c.rowptr_resident = true;
+        else if (a == "--split-blocks") c.split_blocks = true;
+        else if (a == "--cache-mb") c.cache_mb = cli_num<uint64_t>(next("cache-mb"));
         else if (a == "--no-pipeline") c.pipeline = false;
         else if (a == "--zerocopy-threshold") { c.zc = cli_num<uint64_t>(next("zerocopy-threshold")); c.has_zc = true; }
         else if (a == "--span") c.span = next("span");

@@ -136,3 +136,20 @@ def test_learned_scheduler_from_saved_scorer(graph_file, tmp_path):
                   "--walkers-per-thread", 1, "--slice-chunks", 64, "--json", *extra)
         assert "result correct" in out
         assert grep_num(r'"moves": (\d+)', out) == ref_moves
+
+
+def test_split_for_block_slots_from_the_cli(graph_file, tmp_path):
+    """-m --split-blocks writes the .part of lt_partition_build_blocks;
+    --cache-mb sizes the cache in bytes of that table's partition size; the
+    walk moves as on the reference's split."""
+    gf = str(tmp_path / "g.bcsr")
+    with open(graph_file, "rb") as src, open(gf, "wb") as dst:
+        dst.write(src.read())
+    out = run("pagerank", "-f", gf, "-m", "-s", 1, "--split-blocks", "-w", 60000, "--cache-mb", 3,
+              "--pack-blocks", "--span", "resident", "--json")
+    assert "result correct" in out
+    g = lt.Graph.open(gf)
+    part, ref = lt.Partition.load(gf), lt.Partition.build_blocks(g, 1 << 20)
+    assert np.array_equal(part.node_offsets(), ref.node_offsets()) and part.bytes == ref.bytes
+    base = run("pagerank", "-f", graph_file, "-w", 60000, "--span", "resident", "--json")
+    assert grep_num(r'"moves": (\d+)', out) == grep_num(r'"moves": (\d+)', base)
