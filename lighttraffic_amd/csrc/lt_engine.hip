@@ -492,10 +492,10 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     // a zero-copied move then reads only its column entry over PCIe.
     const uint64_t rp_bytes = 8ull * ((uint64_t)V + 1);
     const uint32_t rp_slots = (uint32_t)((rp_bytes + slot_bytes - 1) / slot_bytes);
-    // Off unless asked for: on R-MAT s26 pagerank with an 8 GB cache the slots
-    // it takes cost about the PCIe bytes
-    // it saves at the default threshold.  At least two slots stay, so that
-    // eviction, which spares the partition loaded last, has a victim.
+    // Off unless asked for: on R-MAT s26 pagerank with an 8 GB cache the
+    // slots it takes cost about the PCIe bytes it saves at the default
+    // threshold.  At least two slots stay, so that eviction, which spares the
+    // partition loaded last, has a victim.
     rp_resident = cfg.rowptr_resident > 0;
     if (nslots == P) rp_resident = false;  // nothing is zero-copied with every partition cached
     if (rp_resident && nslots >= rp_slots + 2) nslots -= rp_slots;
