@@ -153,7 +153,7 @@ struct Engine {
     hipStream_t s_compute = nullptr, s_copy = nullptr;
     int num_cu = 0;
     int max_grid = 0;           // init kernel (4 walkers per thread)
-    int walk_grid[5] = {0, 0, 0, 0, 0};  // walk kernel, by walkers per thread
+    int walk_grid[3][5] = {};  // resident workgroups of each walk kernel, by app and walkers per thread
 
     // partition table
     uint32_t *d_node_off = nullptr;
@@ -440,12 +440,21 @@ lt_status Engine::init(const lt_graph *g_, const lt_partition *p_, const lt_conf
     int occ = 0;
     LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, init_kernel<APP_PAGERANK>, kBlock, lds_bytes()));
     max_grid = std::max(1, occ) * num_cu;
-    const void *wk[5] = {nullptr, (const void *)walk_kernel<APP_PAGERANK, 1>, (const void *)walk_kernel<APP_PAGERANK, 2>,
-                         nullptr, (const void *)walk_kernel<APP_PAGERANK, 4>};
-    for (int wpt : {1, 2, 4}) {
-        LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wk[wpt], kBlock, lds_bytes(wpt)));
-        walk_grid[wpt] = std::max(1, occ) * num_cu;
-    }
+    // Each app's kernels have their own register counts, so their own
+    // occupancy: a grid sized by another kernel's would queue workgroups
+    // behind the resident ones and finish in a second, partial round.
+    const void *wk[3][5] = {
+        {nullptr, (const void *)walk_kernel<APP_PAGERANK, 1>, (const void *)walk_kernel<APP_PAGERANK, 2>, nullptr,
+         (const void *)walk_kernel<APP_PAGERANK, 4>},
+        {nullptr, (const void *)walk_kernel<APP_PPR, 1>, (const void *)walk_kernel<APP_PPR, 2>, nullptr,
+         (const void *)walk_kernel<APP_PPR, 4>},
+        {nullptr, (const void *)walk_kernel<APP_GENERIC, 1>, (const void *)walk_kernel<APP_GENERIC, 2>, nullptr,
+         (const void *)walk_kernel<APP_GENERIC, 4>}};
+    for (int app : {APP_PAGERANK, APP_PPR, APP_GENERIC})
+        for (int wpt : {1, 2, 4}) {
+            LT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wk[app][wpt], kBlock, lds_bytes(wpt)));
+            walk_grid[app][wpt] = std::max(1, occ) * num_cu;
+        }
 
     // partition table + vertex -> partition lookup table
     LT_TRY(dmalloc(&d_node_off, P + 1));
@@ -829,7 +838,8 @@ lt_status Engine::launch_walk(const WalkArgs &a, uint64_t walkers, hipStream_t s
                     : walkers >= 2 * fill         ? 2
                                                   : 1;
     const uint64_t bw = (uint64_t)kBlock * wpt;
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + bw - 1) / bw, (uint64_t)walk_grid[wpt]));
+    const int grid =
+        (int)std::max<uint64_t>(1, std::min<uint64_t>((walkers + bw - 1) / bw, (uint64_t)walk_grid[app_kind][wpt]));
     switch (app_kind) {
         case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, wpt, grid, lds_bytes(wpt), s); break;
         case APP_PPR: launch_walk_app<APP_PPR>(a, wpt, grid, lds_bytes(wpt), s); break;
@@ -1686,9 +1696,15 @@ lt_status Engine::scheduler_loop_device() {
         decide_kernel<<<1, 1024, 0, s_compute>>>(q, a.ss, a.pt, bi);
         load_kernel<<<load_grid, 256, 0, s_compute>>>(a.ss, a.pt, bi);
         switch (app_kind) {
-            case APP_PAGERANK: launch_walk_app<APP_PAGERANK>(a, dwpt, walk_grid[dwpt], lds_bytes(dwpt), s_compute); break;
-            case APP_PPR: launch_walk_app<APP_PPR>(a, dwpt, walk_grid[dwpt], lds_bytes(dwpt), s_compute); break;
-            default: launch_walk_app<APP_GENERIC>(a, dwpt, walk_grid[dwpt], lds_bytes(dwpt), s_compute); break;
+            case APP_PAGERANK:
+                launch_walk_app<APP_PAGERANK>(a, dwpt, walk_grid[APP_PAGERANK][dwpt], lds_bytes(dwpt), s_compute);
+                break;
+            case APP_PPR:
+                launch_walk_app<APP_PPR>(a, dwpt, walk_grid[APP_PPR][dwpt], lds_bytes(dwpt), s_compute);
+                break;
+            default:
+                launch_walk_app<APP_GENERIC>(a, dwpt, walk_grid[APP_GENERIC][dwpt], lds_bytes(dwpt), s_compute);
+                break;
         }
         fixup_kernel<<<1, kBlock, 0, s_compute>>>(q, kEmpty, 0, 0, 0, d_ds);
     }

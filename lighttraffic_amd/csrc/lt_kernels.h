@@ -604,7 +604,9 @@ __device__ __forceinline__ void walk_walkers(const WalkArgs &a, const AppendLds 
         if ((uint32_t)k * kBlock + threadIdx.x < nvalid) dst[k] = walk_residency<APP>(w[k], a, src, src_sp, ct);
     }
     block_append<WPT>(a.q, a.pt.P, L, w, dst, s_wsum, a.append_mode);
-    if ((++ct.passes & (kVisitFlushPasses - 1)) == 0) flush_visits(a.app, ct);
+    if constexpr (APP != APP_GENERIC) {  // genericwalk counts no visits
+        if ((++ct.passes & (kVisitFlushPasses - 1)) == 0) flush_visits(a.app, ct);
+    }
 }
 
 // One workgroup pass: walkers [base, base + kBlock * WPT) of src's queue,
@@ -660,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
             const uint64_t local = base - a.voff[k], n = qlen_of(a.q, src);
             if (local < n) walk_pass<APP, WPT>(a, L, s_wsum, src, sp, local, n, ct);  // uniform per workgroup
         }
-        flush_visits(a.app, ct);
+        if constexpr (APP != APP_GENERIC) flush_visits(a.app, ct);
         flush_counters(ct, a.stats);
         return;
     }
@@ -681,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
                 walk_walkers<APP, WPT>(a, L, s_wsum, src, src_sp, a.hptr[k] + local,
                                        min(a.hn[k] - local, (uint32_t)BW), ct);
         }
-        flush_visits(a.app, ct);
+        if constexpr (APP != APP_GENERIC) flush_visits(a.app, ct);
         flush_counters(ct, a.stats);
         return;
     }
@@ -711,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(WalkArgs a) {
     src_sp.eb = a.pt.edge_off[src];
     for (uint64_t base = lo64 + (uint64_t)blockIdx.x * BW; base < hi64; base += (uint64_t)gridDim.x * BW)
         walk_pass<APP, WPT>(a, L, s_wsum, src, src_sp, base, hi64, ct);
-    flush_visits(a.app, ct);
+    if constexpr (APP != APP_GENERIC) flush_visits(a.app, ct);
     flush_counters(ct, a.stats);
 }
 
